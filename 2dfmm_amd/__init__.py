@@ -1,0 +1,247 @@
+"""MI355X (gfx950) 2-D vortex-blob treecode: Python front end of libfmm2d.so.
+
+The library is the product; this module only marshals numpy arrays (or device
+pointers) into its C ABI (include/fmm2d.h).  The two reference entry points
+keep their names and meaning:
+
+    make_tree(vor_b, cir_b, cor_b)     ver-c/src/quadtree.c:102-171
+    biot_tree(x, y) -> (u, v)          ver-c/src/fmm.c:69-118
+
+Where the reference prints and carries on (it has no error path), this module
+raises FmmError with the library's message.  There is no CPU fallback: if the
+library is not built, importing this module fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfmm2d.so")
+
+MODE_TREECODE = 0
+
+
+class FmmError(RuntimeError):
+    pass
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                      f"python 2dfmm_amd/_build.py")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+lib = C.CDLL(LIB_PATH)
+lib.make_tree.restype = None
+lib.make_tree.argtypes = [C.c_int, _dp, _dp, _dp]
+lib.biot_tree.restype = None
+lib.biot_tree.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+lib.fmm_set_params.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+lib.fmm_use_param_file.argtypes = [C.c_char_p]
+lib.fmm_get_params.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                               C.POINTER(C.c_double)]
+lib.fmm_set_mode.argtypes = [C.c_int]
+lib.fmm_set_prefetch.argtypes = [C.c_int]
+lib.fmm_last_error.restype = C.c_char_p
+lib.fmm_make_tree_device.argtypes = [C.c_int, _vp, _vp, _vp]
+lib.fmm_eval_sources.argtypes = [_dp]
+lib.fmm_eval_sources_device.argtypes = [_vp]
+lib.fmm_eval_batch.argtypes = [C.c_int, _dp, _dp]
+lib.fmm_eval_batch_device.argtypes = [C.c_int, _vp, _vp]
+lib.fmm_tree_size.argtypes = [C.POINTER(C.c_int)] * 3
+lib.fmm_dump_tree.argtypes = [_up, _ip, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _dp]
+lib.fmm_dump_lists.argtypes = [C.c_int, _dp, C.c_int, _up, _ip, _ip]
+lib.fmm_direct.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_int]
+lib.fmm_direct_device.argtypes = [C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int]
+lib.fmm_set_timing.argtypes = [C.c_int]
+lib.fmm_get_timing.argtypes = [_dp, C.c_int]
+lib.fmm_count_interactions.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+lib.fmm_synchronize.argtypes = []
+lib.fmm_release.argtypes = []
+lib.fmm_release.restype = None
+
+
+_state = {"n": -1}
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise FmmError(f"{what}: {lib.fmm_last_error().decode(errors='replace')}")
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def last_error() -> str:
+    return lib.fmm_last_error().decode(errors="replace")
+
+
+def set_params(nterms: int, num_p: int, theta: float, r_min: float = 0.01) -> None:
+    _check(lib.fmm_set_params(int(nterms), int(num_p), float(theta), float(r_min)), "set_params")
+
+
+def use_param_file(path: str | None = None) -> None:
+    _check(lib.fmm_use_param_file(path.encode() if path else None), "use_param_file")
+
+
+def get_params():
+    a, b = C.c_int(), C.c_int()
+    c, d = C.c_double(), C.c_double()
+    _check(lib.fmm_get_params(C.byref(a), C.byref(b), C.byref(c), C.byref(d)), "get_params")
+    return a.value, b.value, c.value, d.value
+
+
+def set_mode(mode: int) -> None:
+    _check(lib.fmm_set_mode(int(mode)), "set_mode")
+
+
+def set_prefetch(on: bool) -> None:
+    _check(lib.fmm_set_prefetch(1 if on else 0), "set_prefetch")
+
+
+def make_tree(vor_b, cir_b, cor_b) -> None:
+    """vor_b: (N, 2) positions (memory x0,y0,x1,y1,...); cir_b, cor_b: (N,)."""
+    xy = _f64(vor_b).reshape(-1, 2)
+    n = xy.shape[0]
+    cir, cor = _f64(cir_b).ravel(), _f64(cor_b).ravel()
+    if cir.shape[0] != n or cor.shape[0] != n:
+        raise ValueError("vor_b, cir_b, cor_b disagree on N")
+    if n == 0:  # the C ABI takes no array for N = 0; pass placeholders
+        xy, cir, cor = np.zeros((1, 2)), np.zeros(1), np.zeros(1)
+    lib.make_tree(n, xy.ravel(), cir, cor)
+    _state["n"] = -1
+    _check(lib.fmm_tree_size(None, None, None), "make_tree")
+    _state["n"] = n
+
+
+def biot_tree(x: float, y: float):
+    u, v = C.c_double(), C.c_double()
+    lib.biot_tree(float(x), float(y), C.byref(u), C.byref(v))
+    return u.value, v.value
+
+
+def make_tree_device(n: int, xy_ptr: int, cir_ptr: int, cor_ptr: int) -> None:
+    """Device-resident inputs (e.g. torch tensors' data_ptr())."""
+    _state["n"] = -1
+    _check(lib.fmm_make_tree_device(int(n), xy_ptr, cir_ptr, cor_ptr), "make_tree_device")
+    _state["n"] = int(n)
+
+
+def eval_sources() -> np.ndarray:
+    """Velocities at all blobs of the last tree, caller's order, shape (N, 2)."""
+    n = _state["n"]
+    if n < 0:
+        raise FmmError("no valid tree: call make_tree first")
+    uv = np.zeros((max(n, 1), 2))
+    _check(lib.fmm_eval_sources(uv.ravel()), "eval_sources")
+    return uv[:n]
+
+
+def eval_sources_device(uv_ptr: int) -> None:
+    _check(lib.fmm_eval_sources_device(uv_ptr), "eval_sources_device")
+
+
+def eval_batch(txy) -> np.ndarray:
+    txy = _f64(txy).reshape(-1, 2)
+    nt = txy.shape[0]
+    uv = np.zeros((max(nt, 1), 2))
+    if nt:
+        _check(lib.fmm_eval_batch(nt, txy.ravel(), uv.ravel()), "eval_batch")
+    return uv[:nt]
+
+
+def eval_batch_device(nt: int, txy_ptr: int, uv_ptr: int) -> None:
+    _check(lib.fmm_eval_batch_device(int(nt), txy_ptr, uv_ptr), "eval_batch_device")
+
+
+def tree_size() -> dict:
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    _check(lib.fmm_tree_size(C.byref(a), C.byref(b), C.byref(c)), "tree_size")
+    return {"nnodes": a.value, "depth": b.value, "nleaves": c.value}
+
+
+def dump_tree(n_blobs: int, nterms: int) -> dict:
+    """Nodes in ascending reference global id, leaf members, coefficients."""
+    m = tree_size()["nnodes"]
+    d = dict(gid=np.zeros(m, np.uint64), level=np.zeros(m, np.int32), cx=np.zeros(m), cy=np.zeros(m),
+             region=np.zeros(m), count=np.zeros(m, np.int32), is_leaf=np.zeros(m, np.int32))
+    lo = np.zeros(m + 1, np.int32)
+    mem = np.zeros(max(n_blobs, 1), np.int32)
+    co = np.zeros(max(m * nterms * 2, 1))
+    if m:
+        _check(lib.fmm_dump_tree(d["gid"], d["level"], d["cx"], d["cy"], d["region"], d["count"],
+                                 d["is_leaf"], lo, mem, co), "dump_tree")
+    d["leaf_lo"], d["leaf_members"] = lo, mem[:n_blobs]
+    d["coeffs"] = co[:m * nterms * 2].view(np.complex128).reshape(m, nterms)
+    return d
+
+
+def dump_lists(txy, cap: int = 4096):
+    """Per target: (gids, kinds) in visit order (0 = P2P leaf, 1 = M2S cell)."""
+    txy = _f64(txy).reshape(-1, 2)
+    nt = txy.shape[0]
+    gid = np.zeros(nt * cap, np.uint64)
+    kind = np.zeros(nt * cap, np.int32)
+    ln = np.zeros(max(nt, 1), np.int32)
+    _check(lib.fmm_dump_lists(nt, txy.ravel(), cap, gid, kind, ln), "dump_lists")
+    if nt and ln[:nt].max(initial=0) > cap:
+        return dump_lists(txy, cap=int(ln.max()))
+    out = []
+    for t in range(nt):
+        L = ln[t]
+        out.append((gid[t * cap:t * cap + L].copy(), kind[t * cap:t * cap + L].copy()))
+    return out
+
+
+def direct(xy, cir, cor, txy, style: int = 1) -> np.ndarray:
+    """FP64 direct sum on the GPU (style 0: biot_b rule; 1: direct_evaluate rule)."""
+    xy = _f64(xy).reshape(-1, 2)
+    txy = _f64(txy).reshape(-1, 2)
+    ns, nt = xy.shape[0], txy.shape[0]
+    uv = np.zeros((max(nt, 1), 2))
+    if nt:
+        _check(lib.fmm_direct(ns, xy.ravel() if ns else np.zeros(2), _f64(cir).ravel() if ns else np.zeros(1),
+                              _f64(cor).ravel() if ns else np.zeros(1), nt, txy.ravel(), uv.ravel(),
+                              int(style)), "direct")
+    return uv[:nt]
+
+
+def direct_device(ns, xy_ptr, cir_ptr, cor_ptr, nt, txy_ptr, uv_ptr, style: int = 1) -> None:
+    _check(lib.fmm_direct_device(int(ns), xy_ptr, cir_ptr, cor_ptr, int(nt), txy_ptr, uv_ptr,
+                                 int(style)), "direct_device")
+
+
+PHASES = ("bbox_keys", "radix_sort", "tree_levels", "upward", "evaluate")
+
+
+def set_timing(on: bool) -> None:
+    _check(lib.fmm_set_timing(1 if on else 0), "set_timing")
+
+
+def get_timing() -> dict:
+    """Milliseconds of the phases of the last build and evaluation."""
+    ms = np.zeros(len(PHASES))
+    _check(lib.fmm_get_timing(ms, len(PHASES)), "get_timing")
+    return dict(zip(PHASES, ms.tolist()))
+
+
+def count_interactions():
+    """(far cells evaluated by M2S, near pairs) over all blobs as targets."""
+    a, b = C.c_longlong(), C.c_longlong()
+    _check(lib.fmm_count_interactions(C.byref(a), C.byref(b)), "count_interactions")
+    return a.value, b.value
+
+
+def synchronize() -> None:
+    _check(lib.fmm_synchronize(), "synchronize")
+
+
+def release() -> None:
+    lib.fmm_release()

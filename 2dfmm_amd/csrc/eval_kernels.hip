@@ -1,0 +1,278 @@
+// eval_kernels.hip -- velocity evaluation on the GPU (gfx950).
+//
+// Treecode (parity) traversal, one lane per target, following fmm.c:69-118:
+// a node is expanded by testing its four children in quadrant order 0..3;
+// child c is opened iff region_c > theta * |z - z_c| (the same FP64
+// operations: two products, a sum, a correctly rounded sqrt, a product), an
+// unopened child is evaluated at once by M2S (fmm.c:60-67), and the opened
+// children are then visited in quadrant order 3..0, depth first, which is the
+// order the reference's LIFO pops them.  Instead of a stack the lane keeps one
+// 4-bit mask of not-yet-visited opened children per level (128 bits for 31
+// levels, in registers) and climbs through the group parent array, so no
+// scratch memory is touched.  Opened leaves are summed directly with the
+// regularised blob kernel of fmm.c:121-142.  The complex potential and the
+// direct part are combined as fmm.c:115-117.
+//
+// Direct judge: all sources for every target, the rule of d_biot.f90:21-60
+// (style 0) or of fmm.c:121-142 (style 1); LDS tiles of 256 sources.
+#include <hip/hip_runtime.h>
+
+#include "fmm2d_internal.h"
+
+namespace fmm2d {
+
+struct Slots {
+  uint64_t lo = 0, hi = 0;
+  __device__ inline uint32_t get(int d) const {
+    const int i = d - 1;
+    return i < 16 ? (uint32_t)(lo >> (4 * i)) & 15u : (uint32_t)(hi >> (4 * (i - 16))) & 15u;
+  }
+  __device__ inline void set(int d, uint32_t m) {
+    const int i = d - 1;
+    if (i < 16) {
+      lo = (lo & ~(15ull << (4 * i))) | ((uint64_t)m << (4 * i));
+    } else {
+      const int s = 4 * (i - 16);
+      hi = (hi & ~(15ull << s)) | ((uint64_t)m << s);
+    }
+  }
+};
+
+// phi += sum_{k=0}^{p-2} a_k (z - z_c)^{-(k+1)}, Horner in w = 1 / (z - z_c)
+__device__ inline void m2s_horner(const double2 *__restrict__ a, int p, double dx, double dy,
+                                  double &phr, double &phi) {
+  if (p < 2) return;
+  const double inv = 1.0 / (dx * dx + dy * dy);
+  const double wr = dx * inv, wi = -dy * inv;
+  double2 t = a[p - 2];
+  double sr = t.x, si = t.y;
+  for (int k = p - 3; k >= 0; k--) {
+    const double2 ak = a[k];
+    const double nr = sr * wr - si * wi + ak.x;
+    const double ni = sr * wi + si * wr + ak.y;
+    sr = nr;
+    si = ni;
+  }
+  phr += sr * wr - si * wi;
+  phi += sr * wi + si * wr;
+}
+
+// fmm.c:121-142 for one leaf
+__device__ inline void p2p_leaf(const TreeView &tv, int lo, int cnt, double x, double y,
+                                double &ud, double &vd) {
+  double ax = 0.0, ay = 0.0;
+  for (int j = lo; j < lo + cnt; j++) {
+    const double rx = x - tv.sx[j];
+    const double ry = y - tv.sy[j];
+    const double r = sqrt(rx * rx + ry * ry);
+    if (r > 1.0e-10) {
+      const double rv2 = 1.0 / (r * r);
+      const double xai = r / tv.ss[j];
+      const double rv = tv.sg[j] * (1.0 - exp(-xai * xai));
+      ax -= rv * ry * rv2;
+      ay += rv * rx * rv2;
+    }
+  }
+  ud += 0.5 * ax / M_PI;
+  vd += 0.5 * ay / M_PI;
+}
+
+// MODE 0: velocities; 1: interaction lists; 2: interaction counts only
+// (far cells with blobs, near source-target pairs) for the flop count.
+template <int MODE>
+__global__ __launch_bounds__(256) void parity_eval_kernel(TreeView tv, int nt,
+                                                          const double2 *__restrict__ txy,
+                                                          const uint32_t *__restrict__ perm,
+                                                          double2 *__restrict__ uv, int cap,
+                                                          int32_t *__restrict__ list,
+                                                          int32_t *__restrict__ len) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nt) return;
+  double x, y;
+  if (txy) {
+    const double2 p = txy[t];
+    x = p.x;
+    y = p.y;
+  } else {
+    x = tv.sx[t];
+    y = tv.sy[t];
+  }
+  const NodeRec *__restrict__ nodes = tv.nodes;
+  const int p = tv.nterms;
+  const double theta = tv.theta;
+  double phr = 0.0, phi = 0.0, ud = 0.0, vd = 0.0;
+  Slots sl;
+  int d = 0, cur = 0, node = 0, nl = 0;
+  bool expand = true;
+  constexpr bool RECORD = MODE == 1, COUNT = MODE == 2;
+  int32_t *my_list = RECORD ? list + (size_t)t * cap * 2 : nullptr;
+  int32_t n_m2s = 0, n_pairs = 0;
+  for (;;) {
+    if (expand) {
+      const int base = nodes[node].first;
+      uint32_t m = 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int c = base + q;
+        const NodeRec C = nodes[c];
+        const double dx = x - C.cx, dy = y - C.cy;
+        const double r = sqrt(dx * dx + dy * dy);
+        if (C.region > theta * r) {
+          m |= 1u << q;
+        } else {
+          if (RECORD) {
+            if (nl < cap) { my_list[2 * nl] = c; my_list[2 * nl + 1] = 1; }
+            nl++;
+          }
+          if (COUNT) {
+            n_m2s += node_count(C.cnt) > 0 ? 1 : 0;
+          } else if (node_count(C.cnt) > 0) {
+            m2s_horner(tv.coef + (size_t)c * p, p, dx, dy, phr, phi);
+          }
+        }
+      }
+      d++;
+      sl.set(d, m);
+      cur = base;
+    }
+    uint32_t m;
+    bool done = false;
+    for (;;) {
+      m = sl.get(d);
+      if (m) break;
+      if (d == 1) { done = true; break; }
+      cur = tv.gparent[cur >> 2];
+      d--;
+    }
+    if (done) break;
+    const int q = 31 - __clz(m);
+    sl.set(d, m & ~(1u << q));
+    node = (cur & ~3) + q;
+    const NodeRec N = nodes[node];
+    if (node_internal(N.cnt)) {
+      expand = true;
+    } else {
+      if (RECORD) {
+        if (nl < cap) { my_list[2 * nl] = node; my_list[2 * nl + 1] = 0; }
+        nl++;
+      }
+      if (COUNT) n_pairs += node_count(N.cnt);
+      else p2p_leaf(tv, N.first, node_count(N.cnt), x, y, ud, vd);
+      expand = false;
+    }
+  }
+  // fmm.c:115-117
+  const double u = 0.5 * phi / M_PI + ud;
+  const double v = 0.5 * phr / M_PI + vd;
+  if (RECORD) len[t] = nl;
+  if (COUNT) {
+    len[2 * t] = n_m2s;
+    len[2 * t + 1] = n_pairs;
+    return;
+  }
+  if (uv) {
+    const int o = perm ? (int)perm[t] : t;
+    uv[o] = make_double2(u, v);
+  }
+}
+
+int launch_parity_eval(const TreeView &tv, int nt, const double *d_txy, const uint32_t *d_perm,
+                       double *d_uv, hipStream_t s) {
+  if (nt <= 0) return 0;
+  parity_eval_kernel<0><<<(nt + 255) / 256, 256, 0, s>>>(
+      tv, nt, reinterpret_cast<const double2 *>(d_txy), d_txy ? nullptr : d_perm,
+      reinterpret_cast<double2 *>(d_uv), 0, nullptr, nullptr);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int launch_parity_lists(const TreeView &tv, int nt, const double *d_txy, int cap, int32_t *d_list,
+                        int32_t *d_len, hipStream_t s) {
+  if (nt <= 0) return 0;
+  parity_eval_kernel<1><<<(nt + 255) / 256, 256, 0, s>>>(
+      tv, nt, reinterpret_cast<const double2 *>(d_txy), nullptr, nullptr, cap, d_list, d_len);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int launch_parity_counts(const TreeView &tv, int nt, const double *d_txy, int32_t *d_counts,
+                         hipStream_t s) {
+  if (nt <= 0) return 0;
+  parity_eval_kernel<2><<<(nt + 255) / 256, 256, 0, s>>>(
+      tv, nt, reinterpret_cast<const double2 *>(d_txy), nullptr, nullptr, 0, nullptr, d_counts);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 direct sum (judge)
+// ---------------------------------------------------------------------------
+constexpr int kDirTile = 256;
+
+__global__ __launch_bounds__(256) void direct_kernel(int ns, const double2 *__restrict__ sxy,
+                                                     const double *__restrict__ cir,
+                                                     const double *__restrict__ cor, int nt,
+                                                     const double2 *__restrict__ txy,
+                                                     double2 *__restrict__ uv, int style) {
+  __shared__ double lx[kDirTile], ly[kDirTile], lg[kDirTile], ls[kDirTile];
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  double x = 0.0, y = 0.0;
+  if (t < nt) {
+    const double2 p = txy[t];
+    x = p.x;
+    y = p.y;
+  }
+  const double cut = style == 0 ? 1.0e-6 : 1.0e-10;
+  double ax = 0.0, ay = 0.0;
+  for (int b = 0; b < ns; b += kDirTile) {
+    __syncthreads();
+    const int j = b + threadIdx.x;
+    if (j < ns) {
+      const double2 q = sxy[j];
+      lx[threadIdx.x] = q.x;
+      ly[threadIdx.x] = q.y;
+      lg[threadIdx.x] = cir[j];
+      ls[threadIdx.x] = cor[j];
+    }
+    __syncthreads();
+    const int m = min(kDirTile, ns - b);
+    for (int k = 0; k < m; k++) {
+      const double rx = x - lx[k];
+      const double ry = y - ly[k];
+      const double r = sqrt(rx * rx + ry * ry);
+      if (r > cut) {
+        const double rv2 = 1.0 / (r * r);
+        const double xai = r / ls[k];
+        const double rv = lg[k] * (1.0 - exp(-xai * xai));
+        if (style == 0) {
+          ax = ax - ry * rv * rv2;
+          ay = ay + rx * rv * rv2;
+        } else {
+          ax -= rv * ry * rv2;
+          ay += rv * rx * rv2;
+        }
+      }
+    }
+  }
+  if (t < nt) {
+    if (style == 0) {
+      const double pitwo = 0.50 / 3.141592653589793238;
+      uv[t] = make_double2(ax * pitwo, ay * pitwo);
+    } else {
+      uv[t] = make_double2(0.5 * ax / M_PI, 0.5 * ay / M_PI);
+    }
+  }
+}
+
+int launch_direct(int ns, const double *d_sxy, const double *d_cir, const double *d_cor, int nt,
+                  const double *d_txy, double *d_uv, int style, hipStream_t s) {
+  if (nt <= 0) return 0;
+  direct_kernel<<<(nt + 255) / 256, 256, 0, s>>>(ns, reinterpret_cast<const double2 *>(d_sxy),
+                                                 d_cir, d_cor, nt,
+                                                 reinterpret_cast<const double2 *>(d_txy),
+                                                 reinterpret_cast<double2 *>(d_uv), style);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+}  // namespace fmm2d

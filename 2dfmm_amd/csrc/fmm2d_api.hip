@@ -1,0 +1,812 @@
+// fmm2d_api.hip -- host side of the library: one context (what the reference
+// keeps in the globals qtree/vors, quadtree.h:49-50), the parameter file
+// reader (quadtree.c:12-27), the build sequence of make_tree
+// (quadtree.c:102-171) on the GPU, and the C ABI of include/fmm2d.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fmm2d.h"
+#include "fmm2d_internal.h"
+
+namespace fmm2d {
+
+static char g_err[1024];
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+// Grow-only device buffer.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  // contents are not kept
+  int ensure(size_t n, hipStream_t s) {
+    if (n <= cap) return 0;
+    if (p) {
+      FMM_HIP_TRY(hipStreamSynchronize(s));
+      FMM_HIP_TRY(hipFree(p));
+      p = nullptr;
+      cap = 0;
+    }
+    size_t c = std::max<size_t>(n + n / 4, 64);
+    FMM_HIP_TRY(hipMalloc(&p, c * sizeof(T)));
+    cap = c;
+    return 0;
+  }
+  // contents [0, keep) are kept
+  int ensure_keep(size_t n, size_t keep, hipStream_t s) {
+    if (n <= cap) return 0;
+    size_t c = std::max<size_t>(n + n / 2, 64);
+    T *q = nullptr;
+    FMM_HIP_TRY(hipMalloc(&q, c * sizeof(T)));
+    if (p && keep) FMM_HIP_TRY(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
+    FMM_HIP_TRY(hipStreamSynchronize(s));
+    if (p) FMM_HIP_TRY(hipFree(p));
+    p = q;
+    cap = c;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Context {
+  // parameters (fmm_param.dat)
+  int nterms = 0, num_p = 0;
+  double theta = 0.0, r_min = 0.0;
+  bool have_params = false;
+  bool fixed_params = false;
+  std::string param_path = "../cal_cond/fmm_param.dat";
+  int mode = FMM2D_MODE_TREECODE;
+  bool prefetch = true;
+  hipStream_t stream = nullptr;
+  bool init = false;
+  // tree
+  bool valid = false;
+  int n = 0, nnodes = 0, depth = 0, nleaves = 0;
+  int key_depth = kMaxDepth;
+  int t_nterms = 0;  // nterms the tree was built with
+  double t_theta = 0.0;
+  DevBuf<double> xy, cir, cor;
+  DevBuf<uint64_t> key, key_alt;
+  DevBuf<uint32_t> idx, idx_alt, hist, hist_scan, perm, flag, pos, total;
+  DevBuf<double> sx, sy, sg, ss, bbox_part, uv_dev, tmp_in, tmp_out;
+  DevBuf<NodeRec> nodes;
+  DevBuf<int32_t> gparent, split;
+  DevBuf<double2> coef;
+  DevBuf<RootBox> root;
+  std::vector<int> split_off;
+  // host side
+  uint32_t *h_total = nullptr;
+  double *h_uv = nullptr;
+  size_t h_uv_cap = 0;
+  hipEvent_t ev_uv = nullptr;
+  bool uv_pending = false, uv_ready = false;
+  std::vector<uint64_t> h_src;  // blob positions (bit patterns), caller order
+  std::vector<int32_t> hash;
+  size_t cursor = 0;
+  // phase timing with events on the library stream:
+  // 0 start, 1 keys, 2 sorted, 3 tree, 4 expansions | 5, 6 around the last evaluation
+  bool timing = false;
+  hipEvent_t tev[7] = {};
+};
+
+static Context g;
+static std::mutex g_mu;
+
+static int ctx_init() {
+  if (g.init) return 0;
+  int ndev = 0;
+  FMM_HIP_TRY(hipGetDeviceCount(&ndev));
+  if (ndev < 1) {
+    set_error("no HIP device available");
+    return -1;
+  }
+  FMM_HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  FMM_HIP_TRY(hipHostMalloc((void **)&g.h_total, 64, hipHostMallocDefault));
+  FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_uv, hipEventDisableTiming));
+  for (auto &e : g.tev) FMM_HIP_TRY(hipEventCreate(&e));
+  g.init = true;
+  return 0;
+}
+
+static inline void mark(int i) {
+  if (g.timing) (void)hipEventRecord(g.tev[i], g.stream);
+}
+
+// quadtree.c:12-27 (the same fscanf formats, so "0.5d0" reads as 0.5 and the
+// r_min line of the shipped file does not parse, which is harmless).
+static int read_param_file() {
+  FILE *f = fopen(g.param_path.c_str(), "r");
+  if (!f) {
+    printf("Error: Could not read FMM parameters.");
+    fflush(stdout);
+    if (g.have_params) return 0;
+    set_error("cannot open parameter file %s and no parameters were set", g.param_path.c_str());
+    return -1;
+  }
+  int nt = g.nterms, np = g.num_p;
+  double th = g.theta, rm = g.r_min;
+  int got = 0;
+  if (fscanf(f, "%i    nterms", &nt) == 1) got++;
+  if (fscanf(f, "%i   num_p", &np) == 1) got++;
+  if (fscanf(f, "%lf theta", &th) == 1) got++;
+  if (fscanf(f, "%lf  r_min ! for Tree", &rm) == 1) got++;
+  fclose(f);
+  if (got < 3) {
+    if (g.have_params) return 0;
+    set_error("parameter file %s: could not read nterms, num_p, theta", g.param_path.c_str());
+    return -1;
+  }
+  if (nt < 1 || nt > kMaxTerms || np < 1 || !(th > 0.0)) {
+    set_error("parameter file %s: nterms=%d (1..%d), num_p=%d (>=1), theta=%g (>0)",
+              g.param_path.c_str(), nt, kMaxTerms, np, th);
+    return -1;
+  }
+  g.nterms = nt;
+  g.num_p = np;
+  g.theta = th;
+  g.r_min = rm;
+  g.have_params = true;
+  return 0;
+}
+
+static int resolve_params() {
+  if (g.fixed_params) return 0;
+  return read_param_file();
+}
+
+// The tree build of make_tree on device inputs (caller order).
+static int build(int n, const double *dxy, const double *dcir, const double *dcor) {
+  hipStream_t s = g.stream;
+  g.valid = false;
+  g.uv_pending = g.uv_ready = false;
+  g.hash.clear();
+  g.cursor = 0;
+  g.n = n;
+  g.t_nterms = g.nterms;
+  g.t_theta = g.theta;
+  g.nnodes = g.depth = g.nleaves = 0;
+  if (n == 0) {
+    g.valid = true;
+    return 0;
+  }
+  const int D = g.key_depth;
+  if (g.key.ensure(n, s) || g.key_alt.ensure(n, s) || g.idx.ensure(n, s) || g.idx_alt.ensure(n, s) ||
+      g.perm.ensure(n, s) || g.sx.ensure(n, s) || g.sy.ensure(n, s) || g.sg.ensure(n, s) ||
+      g.ss.ensure(n, s) || g.root.ensure(1, s) || g.bbox_part.ensure(4 * 512, s) ||
+      g.total.ensure(4, s) || g.hist.ensure(radix_sort_hist_words(n), s) ||
+      g.hist_scan.ensure(radix_sort_hist_words(n), s))
+    return -1;
+  mark(0);
+  if (launch_bbox(dxy, n, g.root.p, g.bbox_part.p, s)) return -1;
+  if (launch_keygen(dxy, n, g.root.p, D, g.key.p, g.idx.p, s)) return -1;
+  mark(1);
+  if (radix_sort_pairs(&g.key.p, &g.idx.p, &g.key_alt.p, &g.idx_alt.p, n, 2 * D, g.hist.p,
+                       g.hist_scan.p, s))
+    return -1;
+  mark(2);
+  // (the sort may swap key/key_alt and idx/idx_alt; each pair is always sized
+  // together, so their capacities are equal and need no fix-up)
+  // level-by-level build
+  size_t est = 4 + 16 * ((size_t)n / (size_t)g.num_p + 1);
+  if (g.nodes.ensure_keep(est, 0, s) || g.gparent.ensure_keep(est / 4 + 1, 0, s) ||
+      g.split.ensure_keep(est / 4 + 1, 0, s))
+    return -1;
+  if (launch_init_root(g.nodes.p, g.gparent.p, g.root.p, n, g.split.p, s)) return -1;
+  g.split_off.assign({0, 1});
+  int child_base = 4, nsplit_prev = 1, level = 1;
+  while (nsplit_prev > 0) {
+    if (level > D) {
+      set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D, g.num_p);
+      return -1;
+    }
+    const int nchild = 4 * nsplit_prev;
+    if (g.nodes.ensure_keep((size_t)child_base + nchild, child_base, s) ||
+        g.gparent.ensure_keep(((size_t)child_base + nchild) / 4, child_base / 4, s) ||
+        g.flag.ensure(nchild, s) || g.pos.ensure(nchild, s))
+      return -1;
+    if (launch_build_level(g.key.p, 2 * D, level, g.split.p + g.split_off[level - 1], nsplit_prev,
+                           child_base, g.num_p, g.nodes.p, g.gparent.p, g.flag.p, s))
+      return -1;
+    if (scan_u32(g.flag.p, g.pos.p, nchild, g.total.p, s)) return -1;
+    FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FMM_HIP_TRY(hipStreamSynchronize(s));
+    const int nsplit = (int)g.h_total[0];
+    if (nsplit > 0 && level == D) {
+      set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D, g.num_p);
+      return -1;
+    }
+    const int off = g.split_off[level];
+    if (g.split.ensure_keep((size_t)off + nsplit + 1, off, s)) return -1;
+    if (nsplit > 0 &&
+        launch_compact_split(g.flag.p, g.pos.p, nchild, child_base, g.split.p + off, s))
+      return -1;
+    g.split_off.push_back(off + nsplit);
+    child_base += nchild;
+    g.depth = level;
+    level++;
+    nsplit_prev = nsplit;
+  }
+  g.nnodes = child_base;
+  g.nleaves = (child_base - 4 + 1) - g.split_off.back();
+  if (launch_leaf_order(g.nodes.p, g.nnodes, g.idx.p, g.perm.p, g.num_p, s)) return -1;
+  if (launch_gather(g.perm.p, n, dxy, dcir, dcor, g.sx.p, g.sy.p, g.sg.p, g.ss.p, s)) return -1;
+  mark(3);
+  const size_t ncoef = (size_t)g.nnodes * g.nterms;
+  if (g.coef.ensure(ncoef, s)) return -1;
+  FMM_HIP_TRY(hipMemsetAsync(g.coef.p, 0, ncoef * sizeof(double2), s));
+  if (launch_s2m(g.nodes.p, g.nnodes, g.sx.p, g.sy.p, g.sg.p, g.nterms, g.coef.p, s)) return -1;
+  for (int l = g.depth - 1; l >= 1; l--) {
+    const int a = g.split_off[l], b = g.split_off[l + 1];
+    if (launch_m2m_level(g.nodes.p, g.split.p + a, b - a, g.nterms, g.coef.p, s)) return -1;
+  }
+  mark(4);
+  g.valid = true;
+  return 0;
+}
+
+static TreeView view() {
+  TreeView tv;
+  tv.nodes = g.nodes.p;
+  tv.gparent = g.gparent.p;
+  tv.coef = g.coef.p;
+  tv.sx = g.sx.p;
+  tv.sy = g.sy.p;
+  tv.sg = g.sg.p;
+  tv.ss = g.ss.p;
+  tv.nterms = g.t_nterms;
+  tv.theta = g.t_theta;
+  return tv;
+}
+
+static int need_tree() {
+  if (!g.init || !g.valid) {
+    set_error("no valid tree: call make_tree first");
+    return -1;
+  }
+  return 0;
+}
+
+static int eval_sources_dev(double *d_uv) {
+  if (need_tree()) return -1;
+  if (g.n == 0) return 0;
+  mark(5);
+  const int rc = launch_parity_eval(view(), g.n, nullptr, g.perm.p, d_uv, g.stream);
+  mark(6);
+  return rc;
+}
+
+static int eval_batch_dev(int nt, const double *d_txy, double *d_uv) {
+  if (need_tree()) return -1;
+  if (nt <= 0) return 0;
+  if (g.n == 0) {
+    FMM_HIP_TRY(hipMemsetAsync(d_uv, 0, sizeof(double) * 2 * (size_t)nt, g.stream));
+    return 0;
+  }
+  return launch_parity_eval(view(), nt, d_txy, nullptr, d_uv, g.stream);
+}
+
+static int host_eval_batch(int nt, const double *xy, double *uv) {
+  if (nt <= 0) return 0;
+  hipStream_t s = g.stream;
+  if (g.tmp_in.ensure(2 * (size_t)nt, s) || g.tmp_out.ensure(2 * (size_t)nt, s)) return -1;
+  FMM_HIP_TRY(hipMemcpyAsync(g.tmp_in.p, xy, sizeof(double) * 2 * nt, hipMemcpyHostToDevice, s));
+  if (eval_batch_dev(nt, g.tmp_in.p, g.tmp_out.p)) return -1;
+  FMM_HIP_TRY(hipMemcpyAsync(uv, g.tmp_out.p, sizeof(double) * 2 * nt, hipMemcpyDeviceToHost, s));
+  FMM_HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+static inline uint64_t mix64(uint64_t a, uint64_t b) {
+  uint64_t z = a * 0x9E3779B97F4A7C15ull ^ (b + 0x632BE59BD9B4E019ull + (a << 6) + (a >> 2));
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static void build_hash() {
+  size_t cap = 16;
+  while (cap < 2 * (size_t)g.n) cap <<= 1;
+  g.hash.assign(cap, -1);
+  for (int i = 0; i < g.n; i++) {
+    size_t h = mix64(g.h_src[2 * i], g.h_src[2 * i + 1]) & (cap - 1);
+    while (g.hash[h] >= 0) {
+      const int j = g.hash[h];
+      if (g.h_src[2 * j] == g.h_src[2 * i] && g.h_src[2 * j + 1] == g.h_src[2 * i + 1]) break;
+      h = (h + 1) & (cap - 1);
+    }
+    if (g.hash[h] < 0) g.hash[h] = i;
+  }
+}
+
+static int lookup(uint64_t bx, uint64_t by) {
+  if (g.cursor < (size_t)g.n && g.h_src[2 * g.cursor] == bx && g.h_src[2 * g.cursor + 1] == by)
+    return (int)g.cursor;
+  if (g.hash.empty()) build_hash();
+  const size_t cap = g.hash.size();
+  size_t h = mix64(bx, by) & (cap - 1);
+  while (g.hash[h] >= 0) {
+    const int j = g.hash[h];
+    if (g.h_src[2 * j] == bx && g.h_src[2 * j + 1] == by) return j;
+    h = (h + 1) & (cap - 1);
+  }
+  return -1;
+}
+
+// Host nodes in gid order, with gids and levels.
+struct HostTree {
+  std::vector<NodeRec> nodes;
+  std::vector<int32_t> gparent;
+  std::vector<uint64_t> local, gid;
+  std::vector<int32_t> level;
+};
+
+static int fetch_tree(HostTree &h) {
+  hipStream_t s = g.stream;
+  h.nodes.resize(g.nnodes);
+  h.gparent.resize(g.nnodes / 4);
+  FMM_HIP_TRY(hipMemcpyAsync(h.nodes.data(), g.nodes.p, sizeof(NodeRec) * g.nnodes,
+                             hipMemcpyDeviceToHost, s));
+  FMM_HIP_TRY(hipMemcpyAsync(h.gparent.data(), g.gparent.p, sizeof(int32_t) * (g.nnodes / 4),
+                             hipMemcpyDeviceToHost, s));
+  FMM_HIP_TRY(hipStreamSynchronize(s));
+  h.local.assign(g.nnodes, 0);
+  h.gid.assign(g.nnodes, 0);
+  h.level.assign(g.nnodes, 0);
+  for (int i = 4; i < g.nnodes; i++) {
+    const int par = h.gparent[i >> 2];
+    h.local[i] = (h.local[par] << 2) + (uint64_t)(i & 3);
+    h.level[i] = h.level[par] + 1;
+    h.gid[i] = h.local[i] + ((1ull << (2 * h.level[i])) - 1) / 3;
+  }
+  return 0;
+}
+
+}  // namespace fmm2d
+
+using namespace fmm2d;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void make_tree(int nvor_b, const double *vor_b, const double *cir_b, const double *cor_b) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc = ctx_init();
+  if (!rc) rc = resolve_params();
+  if (!rc && nvor_b < 0) {
+    set_error("nvor_b < 0");
+    rc = -1;
+  }
+  if (!rc) {
+    const int n = nvor_b;
+    hipStream_t s = g.stream;
+    g.valid = false;
+    if (n > 0) {
+      if (g.xy.ensure(2 * (size_t)n, s) || g.cir.ensure(n, s) || g.cor.ensure(n, s)) rc = -1;
+      if (!rc) {
+        if (hipMemcpyAsync(g.xy.p, vor_b, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipMemcpyAsync(g.cir.p, cir_b, sizeof(double) * n, hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipMemcpyAsync(g.cor.p, cor_b, sizeof(double) * n, hipMemcpyHostToDevice, s) !=
+                hipSuccess) {
+          set_error("host to device copy of the blobs failed");
+          rc = -1;
+        }
+      }
+    }
+    if (!rc) rc = build(n, g.xy.p, g.cir.p, g.cor.p);
+    if (!rc) {
+      g.h_src.resize(2 * (size_t)n);
+      if (n > 0) memcpy(g.h_src.data(), vor_b, sizeof(double) * 2 * n);
+      if (g.prefetch && n > 0) {
+        if (g.uv_dev.ensure(2 * (size_t)n, s)) rc = -1;
+        if (!rc && 2 * (size_t)n > g.h_uv_cap) {
+          if (g.h_uv) (void)hipHostFree(g.h_uv);
+          g.h_uv = nullptr;
+          g.h_uv_cap = 0;
+          if (hipHostMalloc((void **)&g.h_uv, sizeof(double) * 2 * (size_t)n,
+                            hipHostMallocDefault) != hipSuccess) {
+            set_error("pinned host allocation failed");
+            rc = -1;
+          } else {
+            g.h_uv_cap = 2 * (size_t)n;
+          }
+        }
+        if (!rc) rc = eval_sources_dev(g.uv_dev.p);
+        if (!rc && (hipMemcpyAsync(g.h_uv, g.uv_dev.p, sizeof(double) * 2 * n,
+                                   hipMemcpyDeviceToHost, s) != hipSuccess ||
+                    hipEventRecord(g.ev_uv, s) != hipSuccess)) {
+          set_error("velocity download failed");
+          rc = -1;
+        }
+        if (!rc) g.uv_pending = true;
+      }
+    }
+  }
+  if (rc) {
+    g.valid = false;
+    fprintf(stderr, "make_tree: %s\n", g_err);
+  }
+}
+
+void biot_tree(double x, double y, double *u, double *v) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  *u = 0.0;
+  *v = 0.0;
+  if (need_tree()) {
+    fprintf(stderr, "biot_tree: %s\n", g_err);
+    return;
+  }
+  if (g.n == 0) return;
+  if (g.uv_pending) {
+    if (hipEventSynchronize(g.ev_uv) != hipSuccess) {
+      fprintf(stderr, "biot_tree: velocity prefetch failed\n");
+      return;
+    }
+    g.uv_pending = false;
+    g.uv_ready = true;
+  }
+  if (g.uv_ready) {
+    uint64_t bx, by;
+    memcpy(&bx, &x, 8);
+    memcpy(&by, &y, 8);
+    const int j = lookup(bx, by);
+    if (j >= 0) {
+      *u = g.h_uv[2 * j];
+      *v = g.h_uv[2 * j + 1];
+      g.cursor = (size_t)j + 1;
+      return;
+    }
+  }
+  double xy[2] = {x, y}, uv[2] = {0.0, 0.0};
+  if (host_eval_batch(1, xy, uv)) {
+    fprintf(stderr, "biot_tree: %s\n", g_err);
+    return;
+  }
+  *u = uv[0];
+  *v = uv[1];
+}
+
+int fmm_set_params(int nterms, int num_p, double theta, double r_min) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (nterms < 1 || nterms > kMaxTerms || num_p < 1 || !(theta > 0.0)) {
+    set_error("fmm_set_params: nterms=%d (1..%d), num_p=%d (>=1), theta=%g (>0)", nterms,
+              kMaxTerms, num_p, theta);
+    return -1;
+  }
+  g.nterms = nterms;
+  g.num_p = num_p;
+  g.theta = theta;
+  g.r_min = r_min;
+  g.have_params = true;
+  g.fixed_params = true;
+  return 0;
+}
+
+int fmm_use_param_file(const char *path) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g.fixed_params = false;
+  g.param_path = path ? path : "../cal_cond/fmm_param.dat";
+  return 0;
+}
+
+int fmm_get_params(int *nterms, int *num_p, double *theta, double *r_min) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.have_params) {
+    set_error("no parameters yet");
+    return -1;
+  }
+  if (nterms) *nterms = g.nterms;
+  if (num_p) *num_p = g.num_p;
+  if (theta) *theta = g.theta;
+  if (r_min) *r_min = g.r_min;
+  return 0;
+}
+
+int fmm_set_mode(int mode) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (mode != FMM2D_MODE_TREECODE) {
+    set_error("fmm_set_mode: unknown mode %d", mode);
+    return -1;
+  }
+  g.mode = mode;
+  return 0;
+}
+
+int fmm_set_prefetch(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g.prefetch = on != 0;
+  return 0;
+}
+
+const char *fmm_last_error(void) { return g_err; }
+
+int fmm_make_tree_device(int n, const double *d_xy, const double *d_cir, const double *d_cor) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init() || resolve_params()) return -1;
+  if (n < 0) {
+    set_error("n < 0");
+    return -1;
+  }
+  g.h_src.clear();
+  return build(n, d_xy, d_cir, d_cor);
+}
+
+int fmm_eval_sources_device(double *d_uv) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return eval_sources_dev(d_uv);
+}
+
+int fmm_eval_sources(double *uv) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_tree()) return -1;
+  if (g.n == 0) return 0;
+  hipStream_t s = g.stream;
+  if (g.uv_dev.ensure(2 * (size_t)g.n, s)) return -1;
+  if (eval_sources_dev(g.uv_dev.p)) return -1;
+  FMM_HIP_TRY(hipMemcpyAsync(uv, g.uv_dev.p, sizeof(double) * 2 * g.n, hipMemcpyDeviceToHost, s));
+  FMM_HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fmm_eval_batch(int nt, const double *xy, double *uv) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_tree()) return -1;
+  return host_eval_batch(nt, xy, uv);
+}
+
+int fmm_eval_batch_device(int nt, const double *d_xy, double *d_uv) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return eval_batch_dev(nt, d_xy, d_uv);
+}
+
+int fmm_tree_size(int *nnodes, int *depth, int *nleaves) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_tree()) return -1;
+  if (nnodes) *nnodes = g.n ? g.nnodes - 3 : 0;
+  if (depth) *depth = g.depth;
+  if (nleaves) *nleaves = g.nleaves;
+  return 0;
+}
+
+int fmm_dump_tree(uint64_t *gid, int32_t *level, double *cx, double *cy, double *region,
+                  int32_t *count, int32_t *is_leaf, int32_t *lo, int32_t *members, double *coeffs) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_tree()) return -1;
+  if (g.n == 0) return 0;
+  HostTree h;
+  if (fetch_tree(h)) return -1;
+  hipStream_t s = g.stream;
+  std::vector<uint32_t> perm(g.n);
+  std::vector<double2> co((size_t)g.nnodes * g.t_nterms);
+  FMM_HIP_TRY(hipMemcpyAsync(perm.data(), g.perm.p, sizeof(uint32_t) * g.n, hipMemcpyDeviceToHost, s));
+  FMM_HIP_TRY(hipMemcpyAsync(co.data(), g.coef.p, sizeof(double2) * co.size(), hipMemcpyDeviceToHost, s));
+  FMM_HIP_TRY(hipStreamSynchronize(s));
+  // node index order is level-major and Morton inside a level, i.e. ascending gid
+  int k = 0, off = 0;
+  for (int i = 0; i < g.nnodes; i++) {
+    if (i >= 1 && i <= 3) continue;
+    const NodeRec &r = h.nodes[i];
+    const bool leaf = i != 0 && !node_internal(r.cnt);
+    if (gid) gid[k] = h.gid[i];
+    if (level) level[k] = h.level[i];
+    if (cx) cx[k] = r.cx;
+    if (cy) cy[k] = r.cy;
+    if (region) region[k] = r.region;
+    if (count) count[k] = node_count(r.cnt);
+    if (is_leaf) is_leaf[k] = leaf ? 1 : 0;
+    if (lo) lo[k] = off;
+    if (leaf) {
+      for (int j = 0; j < node_count(r.cnt); j++)
+        if (members) members[off + j] = (int32_t)perm[r.first + j];
+      off += node_count(r.cnt);
+    }
+    if (coeffs)
+      for (int j = 0; j < g.t_nterms; j++) {
+        coeffs[((size_t)k * g.t_nterms + j) * 2] = co[(size_t)i * g.t_nterms + j].x;
+        coeffs[((size_t)k * g.t_nterms + j) * 2 + 1] = co[(size_t)i * g.t_nterms + j].y;
+      }
+    k++;
+  }
+  if (lo) lo[k] = off;
+  return 0;
+}
+
+int fmm_dump_lists(int nt, const double *xy, int cap, uint64_t *gid, int32_t *kind, int32_t *len) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_tree()) return -1;
+  if (nt <= 0) return 0;
+  if (cap < 1) {
+    set_error("cap must be >= 1");
+    return -1;
+  }
+  if (g.n == 0) {
+    for (int t = 0; t < nt; t++) len[t] = 0;
+    return 0;
+  }
+  hipStream_t s = g.stream;
+  HostTree h;
+  if (fetch_tree(h)) return -1;
+  DevBuf<double> dxy;
+  DevBuf<int32_t> dl, dn;
+  int rc = 0;
+  if (dxy.ensure(2 * (size_t)nt, s) || dl.ensure((size_t)nt * cap * 2, s) || dn.ensure(nt, s)) rc = -1;
+  std::vector<int32_t> hl;
+  if (!rc) {
+    if (hipMemcpyAsync(dxy.p, xy, sizeof(double) * 2 * nt, hipMemcpyHostToDevice, s) != hipSuccess) {
+      set_error("copy failed");
+      rc = -1;
+    }
+  }
+  if (!rc) rc = launch_parity_lists(view(), nt, dxy.p, cap, dl.p, dn.p, s);
+  if (!rc) {
+    hl.resize((size_t)nt * cap * 2);
+    if (hipMemcpyAsync(hl.data(), dl.p, sizeof(int32_t) * hl.size(), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(len, dn.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("list download failed");
+      rc = -1;
+    }
+  }
+  if (!rc) {
+    for (int t = 0; t < nt; t++) {
+      const int m = std::min(len[t], cap);
+      for (int i = 0; i < m; i++) {
+        const size_t o = (size_t)t * cap + i;
+        gid[o] = h.gid[hl[2 * o]];
+        kind[o] = hl[2 * o + 1];
+      }
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  dxy.release();
+  dl.release();
+  dn.release();
+  return rc;
+}
+
+int fmm_direct_device(int ns, const double *d_xy, const double *d_cir, const double *d_cor, int nt,
+                      const double *d_txy, double *d_uv, int style) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  if (style != 0 && style != 1) {
+    set_error("style must be 0 or 1");
+    return -1;
+  }
+  return launch_direct(ns, d_xy, d_cir, d_cor, nt, d_txy, d_uv, style, g.stream);
+}
+
+int fmm_direct(int ns, const double *xy, const double *cir, const double *cor, int nt,
+               const double *txy, double *uv, int style) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  if (style != 0 && style != 1) {
+    set_error("style must be 0 or 1");
+    return -1;
+  }
+  if (nt <= 0) return 0;
+  hipStream_t s = g.stream;
+  DevBuf<double> a, b, c, t, o;
+  int rc = 0;
+  const size_t nsz = ns > 0 ? (size_t)ns : 1;
+  if (a.ensure(2 * nsz, s) || b.ensure(nsz, s) || c.ensure(nsz, s) || t.ensure(2 * (size_t)nt, s) ||
+      o.ensure(2 * (size_t)nt, s))
+    rc = -1;
+  if (!rc && ns > 0 &&
+      (hipMemcpyAsync(a.p, xy, sizeof(double) * 2 * ns, hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(b.p, cir, sizeof(double) * ns, hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(c.p, cor, sizeof(double) * ns, hipMemcpyHostToDevice, s) != hipSuccess)) {
+    set_error("copy failed");
+    rc = -1;
+  }
+  if (!rc && hipMemcpyAsync(t.p, txy, sizeof(double) * 2 * nt, hipMemcpyHostToDevice, s) != hipSuccess) {
+    set_error("copy failed");
+    rc = -1;
+  }
+  if (!rc) rc = launch_direct(ns, a.p, b.p, c.p, nt, t.p, o.p, style, s);
+  if (!rc && (hipMemcpyAsync(uv, o.p, sizeof(double) * 2 * nt, hipMemcpyDeviceToHost, s) != hipSuccess ||
+              hipStreamSynchronize(s) != hipSuccess)) {
+    set_error("copy failed");
+    rc = -1;
+  }
+  (void)hipStreamSynchronize(s);
+  a.release(); b.release(); c.release(); t.release(); o.release();
+  return rc;
+}
+
+int fmm_set_timing(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  g.timing = on != 0;
+  return 0;
+}
+
+int fmm_get_timing(double *ms, int n) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.init || !g.timing) {
+    set_error("timing is off");
+    return -1;
+  }
+  FMM_HIP_TRY(hipStreamSynchronize(g.stream));
+  const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {5, 6}};
+  for (int i = 0; i < n && i < 5; i++) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, g.tev[pairs[i][0]], g.tev[pairs[i][1]]) != hipSuccess) t = -1.f;
+    ms[i] = t;
+  }
+  return 0;
+}
+
+int fmm_count_interactions(long long *m2s, long long *pairs) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_tree()) return -1;
+  *m2s = 0;
+  *pairs = 0;
+  if (g.n == 0) return 0;
+  DevBuf<int32_t> c;
+  if (c.ensure(2 * (size_t)g.n, g.stream)) return -1;
+  std::vector<int32_t> h(2 * (size_t)g.n);
+  int rc = launch_parity_counts(view(), g.n, nullptr, c.p, g.stream);
+  if (!rc && (hipMemcpyAsync(h.data(), c.p, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost,
+                             g.stream) != hipSuccess ||
+              hipStreamSynchronize(g.stream) != hipSuccess)) {
+    set_error("count download failed");
+    rc = -1;
+  }
+  c.release();
+  if (rc) return rc;
+  long long a = 0, b = 0;
+  for (int i = 0; i < g.n; i++) {
+    a += h[2 * i];
+    b += h[2 * i + 1];
+  }
+  *m2s = a;
+  *pairs = b;
+  return 0;
+}
+
+int fmm_synchronize(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.init) return 0;
+  FMM_HIP_TRY(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+void fmm_release(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.init) return;
+  (void)hipStreamSynchronize(g.stream);
+  g.xy.release(); g.cir.release(); g.cor.release();
+  g.key.release(); g.key_alt.release(); g.idx.release(); g.idx_alt.release();
+  g.hist.release(); g.hist_scan.release(); g.perm.release(); g.flag.release(); g.pos.release();
+  g.total.release(); g.sx.release(); g.sy.release(); g.sg.release(); g.ss.release();
+  g.bbox_part.release(); g.uv_dev.release(); g.tmp_in.release(); g.tmp_out.release();
+  g.nodes.release(); g.gparent.release(); g.split.release(); g.coef.release(); g.root.release();
+  if (g.h_uv) (void)hipHostFree(g.h_uv);
+  g.h_uv = nullptr;
+  g.h_uv_cap = 0;
+  g.valid = false;
+}
+
+}  // extern "C"

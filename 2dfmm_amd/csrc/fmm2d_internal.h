@@ -1,0 +1,108 @@
+// fmm2d_internal.h -- shared types of the HIP quadtree / treecode library.
+//
+// Device data layout (HBM), for N blobs and M tree nodes:
+//   xy[N][2], cir[N], cor[N]          caller's order (input, read once)
+//   key[N] u64, idx[N] u32            Morton path keys + blob ids (radix sorted)
+//   perm[N] u32                       sorted position -> blob id (leaf order,
+//                                     ascending blob id inside each leaf)
+//   sx, sy, sg, ss [N] f64            blobs gathered into leaf order (SoA)
+//   nodes[M] NodeRec (32 B)           level by level; the four children of a
+//                                     node are consecutive, first index % 4 == 0
+//   gparent[M/4] i32                  parent of each group of four siblings
+//   coef[M][P] double2                multipole coefficients a_k, k < nterms
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+
+namespace fmm2d {
+
+// Maximum tree depth: keys hold 2 bits per level in a u64.
+constexpr int kMaxDepth = 31;
+// Largest supported expansion length (nterms).
+constexpr int kMaxTerms = 32;
+
+// One tree node.  `first` is the index of child q=0 for an internal node and
+// the start of the node's blobs in the leaf-ordered arrays for a leaf; `cnt`
+// is the blob count with kInternalBit set for internal (split) nodes.
+struct __align__(32) NodeRec {
+  double cx, cy, region;
+  int32_t first;
+  int32_t cnt;
+};
+constexpr int32_t kInternalBit = int32_t(0x80000000u);
+__host__ __device__ inline int32_t node_count(int32_t c) { return c & 0x7fffffff; }
+__host__ __device__ inline bool node_internal(int32_t c) { return (c & kInternalBit) != 0; }
+
+// Root square, computed on the device (quadtree.c:81-99).
+struct RootBox {
+  double cx, cy, region;
+  double pad;
+};
+
+#define FMM_HIP_TRY(expr)                                                         \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      ::fmm2d::set_error("%s:%d: %s: %s", __FILE__, __LINE__, #expr,              \
+                         hipGetErrorString(e_));                                  \
+      return -1;                                                                  \
+    }                                                                             \
+  } while (0)
+
+void set_error(const char *fmt, ...);
+
+// ---- tree construction (tree_kernels.hip) --------------------------------
+// Exclusive scan of n u32 values; total written to *d_total when non-null.
+int scan_u32(const uint32_t *d_in, uint32_t *d_out, int n, uint32_t *d_total, hipStream_t s);
+int launch_bbox(const double *d_xy, int n, RootBox *d_root, double *d_partial, hipStream_t s);
+int launch_keygen(const double *d_xy, int n, const RootBox *d_root, int depth, uint64_t *d_key,
+                  uint32_t *d_idx, hipStream_t s);
+// Stable LSD radix sort of (key, idx) pairs on the low `bits` bits.  Uses the
+// alternate buffers; the sorted result ends in (*key, *idx) (pointers swapped).
+int radix_sort_pairs(uint64_t **key, uint32_t **idx, uint64_t **key_alt, uint32_t **idx_alt,
+                     int n, int bits, uint32_t *d_hist, uint32_t *d_hist_scan, hipStream_t s);
+size_t radix_sort_hist_words(int n);
+int launch_init_root(NodeRec *d_nodes, int32_t *d_gparent, const RootBox *d_root, int n,
+                     int32_t *d_split, hipStream_t s);
+int launch_build_level(const uint64_t *d_key, int depth_bits, int level, const int32_t *d_split_prev,
+                       int nsplit_prev, int child_base, int num_p, NodeRec *d_nodes,
+                       int32_t *d_gparent, uint32_t *d_flag, hipStream_t s);
+int launch_compact_split(const uint32_t *d_flag, const uint32_t *d_pos, int nchild, int child_base,
+                         int32_t *d_split_next, hipStream_t s);
+int launch_leaf_order(const NodeRec *d_nodes, int nnodes, const uint32_t *d_idx_sorted,
+                      uint32_t *d_perm, int num_p, hipStream_t s);
+int launch_gather(const uint32_t *d_perm, int n, const double *d_xy, const double *d_cir,
+                  const double *d_cor, double *sx, double *sy, double *sg, double *ss, hipStream_t s);
+
+// ---- expansions (expansion_kernels.hip) ----------------------------------
+int launch_s2m(const NodeRec *d_nodes, int nnodes, const double *sx, const double *sy,
+               const double *sg, int nterms, double2 *d_coef, hipStream_t s);
+int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int nparents, int nterms,
+                     double2 *d_coef, hipStream_t s);
+
+// ---- evaluation (eval_kernels.hip) ---------------------------------------
+struct TreeView {
+  const NodeRec *nodes;
+  const int32_t *gparent;
+  const double2 *coef;
+  const double *sx, *sy, *sg, *ss;
+  int nterms;
+  double theta;
+};
+// Treecode velocities at targets (txy interleaved; or the sorted blobs
+// themselves when txy is null, then results are scattered through perm).
+int launch_parity_eval(const TreeView &tv, int nt, const double *d_txy, const uint32_t *d_perm,
+                       double *d_uv, hipStream_t s);
+// Interaction lists of targets: node index (>= 0) and kind (0 P2P, 1 M2S).
+int launch_parity_lists(const TreeView &tv, int nt, const double *d_txy, int cap, int32_t *d_list,
+                        int32_t *d_len, hipStream_t s);
+// Per target: (far cells with blobs, near pairs); txy null = the blobs themselves.
+int launch_parity_counts(const TreeView &tv, int nt, const double *d_txy, int32_t *d_counts,
+                         hipStream_t s);
+// FP64 direct sum over all sources (style 0: biot_b rule, 1: direct_evaluate rule).
+int launch_direct(int ns, const double *d_sxy, const double *d_cir, const double *d_cor, int nt,
+                  const double *d_txy, double *d_uv, int style, hipStream_t s);
+
+}  // namespace fmm2d

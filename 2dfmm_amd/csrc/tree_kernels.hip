@@ -1,0 +1,531 @@
+// tree_kernels.hip -- quadtree construction on the GPU (gfx950).
+//
+//   bbox      root square from exact min/max (quadtree.c:81-99)
+//   keygen    per blob, the quadrant path from the root, 2 bits per level,
+//             with the parent->child centre recurrence of quadtree.c:61-79 and
+//             the tie rule x >= cx, y <= cy of quadtree.c:29-35
+//   radix     stable LSD sort of (key, blob id), 8-bit digits, LDS ranking
+//   levels    one launch per level: children of every split node found by
+//             binary search on the sorted keys; leaf iff count < num_p
+//             (quadtree.c:197-245)
+//   leaf      within every leaf, blobs in ascending id (the stable partition
+//             order of quadtree.c:219-225)
+#include <hip/hip_runtime.h>
+
+#include "fmm2d_internal.h"
+
+namespace fmm2d {
+
+// ---------------------------------------------------------------------------
+// wave / block helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread across a 256-thread block.
+__device__ inline uint32_t block_excl_scan_256(uint32_t v, uint32_t *lds4, uint32_t *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) lds4[w] = inc;
+  __syncthreads();
+  uint32_t off = 0;
+  for (int k = 0; k < w; k++) off += lds4[k];
+  if (total) *total = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+  __syncthreads();
+  return off + inc - v;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of u32 (1024 values per block, recursive over block sums)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void scan_block_kernel(const uint32_t *__restrict__ in,
+                                                         uint32_t *__restrict__ out,
+                                                         uint32_t *__restrict__ sums, int n) {
+  __shared__ uint32_t lds4[4];
+  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+  uint32_t a[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) a[j] = (base + j < n) ? in[base + j] : 0u;
+  uint32_t t = a[0] + a[1] + a[2] + a[3];
+  uint32_t tot;
+  uint32_t ex = block_excl_scan_256(t, lds4, &tot);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (base + j < n) out[base + j] = ex;
+    ex += a[j];
+  }
+  if (threadIdx.x == 0 && sums) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void scan_add_kernel(uint32_t *__restrict__ out,
+                                                       const uint32_t *__restrict__ offs, int n) {
+  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+  const uint32_t o = offs[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (base + j < n) out[base + j] += o;
+}
+
+// Workspace: the scan needs block-sum arrays for each recursion level; they
+// live in a grow-only device buffer owned by this file.
+static uint32_t *g_scan_ws = nullptr;
+static size_t g_scan_ws_words = 0;
+
+static int scan_rec(const uint32_t *d_in, uint32_t *d_out, int n, uint32_t *d_total, uint32_t *ws,
+                    hipStream_t s) {
+  const int nb = (n + 1023) / 1024;
+  if (nb <= 1) {
+    scan_block_kernel<<<1, 256, 0, s>>>(d_in, d_out, d_total, n);
+    FMM_HIP_TRY(hipGetLastError());
+    return 0;
+  }
+  uint32_t *sums = ws, *offs = ws + nb;
+  scan_block_kernel<<<nb, 256, 0, s>>>(d_in, d_out, sums, n);
+  FMM_HIP_TRY(hipGetLastError());
+  if (scan_rec(sums, offs, nb, d_total, ws + 2 * nb, s) != 0) return -1;
+  scan_add_kernel<<<nb, 256, 0, s>>>(d_out, offs, n);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int scan_u32(const uint32_t *d_in, uint32_t *d_out, int n, uint32_t *d_total, hipStream_t s) {
+  if (n <= 0) {
+    if (d_total) FMM_HIP_TRY(hipMemsetAsync(d_total, 0, sizeof(uint32_t), s));
+    return 0;
+  }
+  size_t need = 64;
+  for (size_t m = (size_t)n; m > 1; m = (m + 1023) / 1024) need += 2 * ((m + 1023) / 1024) + 2;
+  if (need > g_scan_ws_words) {
+    if (g_scan_ws) {
+      FMM_HIP_TRY(hipStreamSynchronize(s));
+      FMM_HIP_TRY(hipFree(g_scan_ws));
+      g_scan_ws = nullptr;
+    }
+    size_t words = need * 2;
+    FMM_HIP_TRY(hipMalloc(&g_scan_ws, words * sizeof(uint32_t)));
+    g_scan_ws_words = words;
+  }
+  return scan_rec(d_in, d_out, n, d_total, g_scan_ws, s);
+}
+
+// ---------------------------------------------------------------------------
+// bbox: exact min / max of x and y, then the root square
+// ---------------------------------------------------------------------------
+constexpr int kBboxBlocks = 512;
+
+__global__ __launch_bounds__(256) void bbox_partial_kernel(const double2 *__restrict__ xy, int n,
+                                                           double *__restrict__ part) {
+  double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    double2 p = xy[i];
+    mnx = fmin(mnx, p.x);
+    mxx = fmax(mxx, p.x);
+    mny = fmin(mny, p.y);
+    mxy = fmax(mxy, p.y);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mnx = fmin(mnx, __shfl_xor(mnx, o, 64));
+    mxx = fmax(mxx, __shfl_xor(mxx, o, 64));
+    mny = fmin(mny, __shfl_xor(mny, o, 64));
+    mxy = fmax(mxy, __shfl_xor(mxy, o, 64));
+  }
+  __shared__ double s[4][4];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s[w][0] = mnx; s[w][1] = mxx; s[w][2] = mny; s[w][3] = mxy;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) {
+      s[0][0] = fmin(s[0][0], s[k][0]);
+      s[0][1] = fmax(s[0][1], s[k][1]);
+      s[0][2] = fmin(s[0][2], s[k][2]);
+      s[0][3] = fmax(s[0][3], s[k][3]);
+    }
+    for (int k = 0; k < 4; k++) part[4 * blockIdx.x + k] = s[0][k];
+  }
+}
+
+__global__ __launch_bounds__(64) void bbox_final_kernel(const double *__restrict__ part, int nb,
+                                                        RootBox *__restrict__ root) {
+  double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+  for (int b = threadIdx.x; b < nb; b += 64) {
+    mnx = fmin(mnx, part[4 * b + 0]);
+    mxx = fmax(mxx, part[4 * b + 1]);
+    mny = fmin(mny, part[4 * b + 2]);
+    mxy = fmax(mxy, part[4 * b + 3]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mnx = fmin(mnx, __shfl_xor(mnx, o, 64));
+    mxx = fmax(mxx, __shfl_xor(mxx, o, 64));
+    mny = fmin(mny, __shfl_xor(mny, o, 64));
+    mxy = fmax(mxy, __shfl_xor(mxy, o, 64));
+  }
+  if (threadIdx.x == 0) {
+    // quadtree.c:87-98: side = max(|dx| + 1e-4, |dy| + 1e-4), x side wins only if strictly larger
+    double rx = fabs(mxx - mnx) + 0.0001;
+    double ry = fabs(mxy - mny) + 0.0001;
+    root->region = rx > ry ? rx : ry;
+    root->cx = 0.5 * (mxx + mnx);
+    root->cy = 0.5 * (mxy + mny);
+    root->pad = 0.0;
+  }
+}
+
+int launch_bbox(const double *d_xy, int n, RootBox *d_root, double *d_partial, hipStream_t s) {
+  int nb = (n + 255) / 256;
+  if (nb > kBboxBlocks) nb = kBboxBlocks;
+  if (nb < 1) nb = 1;
+  bbox_partial_kernel<<<nb, 256, 0, s>>>(reinterpret_cast<const double2 *>(d_xy), n, d_partial);
+  FMM_HIP_TRY(hipGetLastError());
+  bbox_final_kernel<<<1, 64, 0, s>>>(d_partial, nb, d_root);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// keygen: quadrant path of each blob, most significant digit = level 1
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void keygen_kernel(const double2 *__restrict__ xy, int n,
+                                                     const RootBox *__restrict__ root, int depth,
+                                                     uint64_t *__restrict__ key,
+                                                     uint32_t *__restrict__ idx) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double2 p = xy[i];
+  double cx = root->cx, cy = root->cy, reg = root->region;
+  uint64_t k = 0;
+  for (int l = 0; l < depth; l++) {
+    const bool right = p.x >= cx;
+    const bool down = p.y <= cy;
+    k = (k << 2) | (uint64_t)((right ? 1 : 0) + (down ? 2 : 0));
+    const double h = 0.25 * reg;
+    cx = right ? cx + h : cx - h;
+    cy = down ? cy - h : cy + h;
+    reg = 0.5 * reg;
+  }
+  key[i] = k;
+  idx[i] = (uint32_t)i;
+}
+
+int launch_keygen(const double *d_xy, int n, const RootBox *d_root, int depth, uint64_t *d_key,
+                  uint32_t *d_idx, hipStream_t s) {
+  keygen_kernel<<<(n + 255) / 256, 256, 0, s>>>(reinterpret_cast<const double2 *>(d_xy), n, d_root,
+                                                 depth, d_key, d_idx);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort, 8-bit digits.  A tile is 4096 pairs, 1024 per wave: item j
+// of lane l of wave w is pair w*1024 + j*64 + l, so the tile order that must
+// be preserved is (wave, item, lane).  Lanes holding the same digit are
+// found with eight ballots; the leader advances a per-wave digit counter in
+// LDS.  Pairs are staged in LDS in tile-sorted order and written out in runs.
+// ---------------------------------------------------------------------------
+constexpr int kRsThreads = 256, kRsItems = 16, kRsTile = kRsThreads * kRsItems;
+
+__device__ inline uint64_t peers_of(uint32_t d, uint64_t valid) {
+  uint64_t m = valid;
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(256) void rs_hist_kernel(const uint64_t *__restrict__ key, int n,
+                                                      int shift, uint32_t *__restrict__ hist,
+                                                      int ntiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wbase = blockIdx.x * kRsTile + w * (kRsItems * 64);
+  for (int j = 0; j < kRsItems; j++) {
+    const int i = wbase + j * 64 + lane;
+    const bool ok = i < n;
+    const uint32_t d = ok ? (uint32_t)(key[i] >> shift) & 255u : 0u;
+    const uint64_t valid = __ballot(ok);
+    if (ok) {
+      const uint64_t pm = peers_of(d, valid);
+      if (lane == __ffsll((unsigned long long)pm) - 1) atomicAdd(&h[d], (uint32_t)__popcll(pm));
+    }
+  }
+  __syncthreads();
+  hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void rs_scatter_kernel(const uint64_t *__restrict__ kin,
+                                                         const uint32_t *__restrict__ vin,
+                                                         uint64_t *__restrict__ kout,
+                                                         uint32_t *__restrict__ vout, int n,
+                                                         int shift,
+                                                         const uint32_t *__restrict__ hscan,
+                                                         int ntiles) {
+  __shared__ uint32_t wcnt[4][256];
+  __shared__ uint32_t dstart[256];
+  __shared__ uint32_t gbase[256];
+  __shared__ uint32_t lds4[4];
+  __shared__ uint64_t skey[kRsTile];
+  __shared__ uint32_t sval[kRsTile];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int k = tid; k < 4 * 256; k += 256) (&wcnt[0][0])[k] = 0;
+  __syncthreads();
+  const int tbase = blockIdx.x * kRsTile;
+  const int wbase = tbase + w * (kRsItems * 64);
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint64_t k[kRsItems];
+  uint32_t v[kRsItems], r[kRsItems];
+#pragma unroll
+  for (int j = 0; j < kRsItems; j++) {
+    const int i = wbase + j * 64 + lane;
+    const bool ok = i < n;
+    k[j] = ok ? kin[i] : 0ull;
+    v[j] = ok ? vin[i] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kRsItems; j++) {
+    const int i = wbase + j * 64 + lane;
+    const bool ok = i < n;
+    const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+    const uint64_t valid = __ballot(ok);
+    r[j] = 0;
+    if (ok) {
+      const uint64_t pm = peers_of(d, valid);
+      const uint32_t prior = wcnt[w][d];
+      r[j] = prior + (uint32_t)__popcll(pm & lt);
+      if (lane == __ffsll((unsigned long long)pm) - 1) wcnt[w][d] = prior + (uint32_t)__popcll(pm);
+    }
+  }
+  __syncthreads();
+  // per digit: exclusive prefix over waves, then over digits
+  uint32_t tot = 0;
+  {
+    const int d = tid;
+    uint32_t run = 0;
+    for (int ww = 0; ww < 4; ww++) {
+      const uint32_t t = wcnt[ww][d];
+      wcnt[ww][d] = run;
+      run += t;
+    }
+    tot = run;
+  }
+  const uint32_t ds = block_excl_scan_256(tot, lds4, nullptr);
+  dstart[tid] = ds;
+  gbase[tid] = hscan[tid * ntiles + blockIdx.x];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kRsItems; j++) {
+    const int i = wbase + j * 64 + lane;
+    if (i < n) {
+      const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+      const uint32_t pos = dstart[d] + wcnt[w][d] + r[j];
+      skey[pos] = k[j];
+      sval[pos] = v[j];
+    }
+  }
+  __syncthreads();
+  const int cnt = min(kRsTile, n - tbase);
+  for (int e = tid; e < cnt; e += 256) {
+    const uint64_t kk = skey[e];
+    const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+    const uint32_t dst = gbase[d] + (uint32_t)e - dstart[d];
+    kout[dst] = kk;
+    vout[dst] = sval[e];
+  }
+}
+
+size_t radix_sort_hist_words(int n) {
+  const size_t ntiles = (size_t)((n + kRsTile - 1) / kRsTile);
+  return 256 * (ntiles > 0 ? ntiles : 1);
+}
+
+int radix_sort_pairs(uint64_t **key, uint32_t **idx, uint64_t **key_alt, uint32_t **idx_alt, int n,
+                     int bits, uint32_t *d_hist, uint32_t *d_hist_scan, hipStream_t s) {
+  if (n <= 1) return 0;
+  const int ntiles = (n + kRsTile - 1) / kRsTile;
+  for (int shift = 0; shift < bits; shift += 8) {
+    rs_hist_kernel<<<ntiles, 256, 0, s>>>(*key, n, shift, d_hist, ntiles);
+    FMM_HIP_TRY(hipGetLastError());
+    if (scan_u32(d_hist, d_hist_scan, 256 * ntiles, nullptr, s) != 0) return -1;
+    rs_scatter_kernel<<<ntiles, 256, 0, s>>>(*key, *idx, *key_alt, *idx_alt, n, shift, d_hist_scan,
+                                             ntiles);
+    FMM_HIP_TRY(hipGetLastError());
+    uint64_t *tk = *key; *key = *key_alt; *key_alt = tk;
+    uint32_t *ti = *idx; *idx = *idx_alt; *idx_alt = ti;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// level-by-level tree build
+// ---------------------------------------------------------------------------
+__global__ void init_root_kernel(NodeRec *__restrict__ nodes, int32_t *__restrict__ gparent,
+                                 const RootBox *__restrict__ root, int n,
+                                 int32_t *__restrict__ split) {
+  const int t = threadIdx.x;
+  if (t < 4) {
+    NodeRec r;
+    if (t == 0) {
+      r.cx = root->cx; r.cy = root->cy; r.region = root->region;
+      r.first = 0;  // the range start; replaced by the child index when level 1 is built
+      r.cnt = n | kInternalBit;
+    } else {
+      r.cx = 0.0; r.cy = 0.0; r.region = 0.0; r.first = 0; r.cnt = 0;
+    }
+    nodes[t] = r;
+  }
+  if (t == 0) {
+    gparent[0] = -1;
+    split[0] = 0;
+  }
+}
+
+int launch_init_root(NodeRec *d_nodes, int32_t *d_gparent, const RootBox *d_root, int n,
+                     int32_t *d_split, hipStream_t s) {
+  init_root_kernel<<<1, 64, 0, s>>>(d_nodes, d_gparent, d_root, n, d_split);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+__device__ inline int lower_bound_digit(const uint64_t *__restrict__ key, int lo, int hi, int shift,
+                                        uint32_t q) {
+  // first index in [lo, hi) whose digit >= q (digits are non-decreasing there)
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    if ((uint32_t)((key[mid] >> shift) & 3u) < q) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// The range start of an internal node is kept in `first` until its own
+// children are built (then `first` becomes the child index).
+__global__ __launch_bounds__(256) void build_level_kernel(const uint64_t *__restrict__ key,
+                                                          int shift, const int32_t *__restrict__ split_prev,
+                                                          int nsplit, int child_base, int num_p,
+                                                          NodeRec *__restrict__ nodes,
+                                                          int32_t *__restrict__ gparent,
+                                                          uint32_t *__restrict__ flag) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= nsplit) return;
+  const int par = split_prev[j];
+  NodeRec P = nodes[par];
+  const int lo = P.first, hi = lo + node_count(P.cnt);
+  int b[5];
+  b[0] = lo;
+  b[4] = hi;
+  b[2] = lower_bound_digit(key, lo, hi, shift, 2u);
+  b[1] = lower_bound_digit(key, lo, b[2], shift, 1u);
+  b[3] = lower_bound_digit(key, b[2], hi, shift, 3u);
+  const int cb = child_base + 4 * j;
+  const double h = 0.25 * P.region;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    NodeRec c;
+    // quadtree.c:61-79
+    c.cx = (q & 1) ? P.cx + h : P.cx - h;
+    c.cy = (q < 2) ? P.cy + h : P.cy - h;
+    c.region = 0.5 * P.region;
+    const int cnt = b[q + 1] - b[q];
+    c.first = b[q];
+    const bool split = cnt >= num_p;
+    c.cnt = split ? (cnt | kInternalBit) : cnt;
+    nodes[cb + q] = c;
+    flag[4 * j + q] = split ? 1u : 0u;
+  }
+  gparent[cb >> 2] = par;
+  nodes[par].first = cb;
+}
+
+int launch_build_level(const uint64_t *d_key, int depth_bits, int level, const int32_t *d_split_prev,
+                       int nsplit_prev, int child_base, int num_p, NodeRec *d_nodes,
+                       int32_t *d_gparent, uint32_t *d_flag, hipStream_t s) {
+  const int shift = depth_bits - 2 * level;
+  build_level_kernel<<<(nsplit_prev + 255) / 256, 256, 0, s>>>(
+      d_key, shift, d_split_prev, nsplit_prev, child_base, num_p, d_nodes, d_gparent, d_flag);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+__global__ __launch_bounds__(256) void compact_split_kernel(const uint32_t *__restrict__ flag,
+                                                            const uint32_t *__restrict__ pos,
+                                                            int nchild, int child_base,
+                                                            int32_t *__restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nchild && flag[i]) out[pos[i]] = child_base + i;
+}
+
+int launch_compact_split(const uint32_t *d_flag, const uint32_t *d_pos, int nchild, int child_base,
+                         int32_t *d_split_next, hipStream_t s) {
+  compact_split_kernel<<<(nchild + 255) / 256, 256, 0, s>>>(d_flag, d_pos, nchild, child_base,
+                                                            d_split_next);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// One wave per node: a leaf's blob ids are reordered ascending by ranking.
+__global__ __launch_bounds__(256) void leaf_order_kernel(const NodeRec *__restrict__ nodes,
+                                                         int nnodes,
+                                                         const uint32_t *__restrict__ idx,
+                                                         uint32_t *__restrict__ perm) {
+  const int node = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (node >= nnodes || node == 0) return;
+  const NodeRec r = nodes[node];
+  if (node_internal(r.cnt)) return;
+  const int c = node_count(r.cnt), lo = r.first;
+  for (int e = lane; e < c; e += 64) {
+    const uint32_t my = idx[lo + e];
+    int rank = 0;
+    for (int f = 0; f < c; f++) rank += idx[lo + f] < my ? 1 : 0;
+    perm[lo + rank] = my;
+  }
+}
+
+int launch_leaf_order(const NodeRec *d_nodes, int nnodes, const uint32_t *d_idx_sorted,
+                      uint32_t *d_perm, int num_p, hipStream_t s) {
+  (void)num_p;
+  leaf_order_kernel<<<(nnodes + 3) / 4, 256, 0, s>>>(d_nodes, nnodes, d_idx_sorted, d_perm);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+__global__ __launch_bounds__(256) void gather_kernel(const uint32_t *__restrict__ perm, int n,
+                                                     const double2 *__restrict__ xy,
+                                                     const double *__restrict__ cir,
+                                                     const double *__restrict__ cor,
+                                                     double *__restrict__ sx, double *__restrict__ sy,
+                                                     double *__restrict__ sg,
+                                                     double *__restrict__ ss) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = perm[i];
+  const double2 q = xy[p];
+  sx[i] = q.x;
+  sy[i] = q.y;
+  sg[i] = cir[p];
+  ss[i] = cor[p];
+}
+
+int launch_gather(const uint32_t *d_perm, int n, const double *d_xy, const double *d_cir,
+                  const double *d_cor, double *sx, double *sy, double *sg, double *ss,
+                  hipStream_t s) {
+  gather_kernel<<<(n + 255) / 256, 256, 0, s>>>(d_perm, n, reinterpret_cast<const double2 *>(d_xy),
+                                                d_cir, d_cor, sx, sy, sg, ss);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+}  // namespace fmm2d

@@ -1,0 +1,127 @@
+/*
+ * fmm2d.h -- C ABI of the MI355X (gfx950) 2-D vortex-blob tree library.
+ *
+ * Drop-in for the reference treecode's two entry points, which the vortex
+ * solver binds through ver-c/src/m_tree_bindc.f90 (bind(c), no name mangling):
+ *
+ *   make_tree  replaces ver-c/src/quadtree.c:102-171 (declared quadtree.h:38-41,
+ *              bound at m_tree_bindc.f90:4-8)
+ *   biot_tree  replaces ver-c/src/fmm.c:69-118 (declared fmm.h:7, bound at
+ *              m_tree_bindc.f90:10-14)
+ *
+ * Everything else is an extension with no counterpart in the reference; each
+ * says which reference behaviour it exposes or which caller it serves.
+ *
+ * Threading: make_tree and every fmm_* call that builds or evaluates must not
+ * run concurrently with each other (as in the reference, whose state is
+ * global).  biot_tree may be called from several threads after make_tree.
+ * Errors: functions returning int give 0 on success and -1 on failure, with
+ * fmm_last_error() describing it; the two reference entry points return
+ * nothing, so they print the error to stderr, make_tree leaves an empty tree
+ * and biot_tree returns u = v = 0 (the reference has no error path at all).
+ */
+#ifndef FMM2D_H
+#define FMM2D_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- the reference ABI ------------------------------------------------- */
+
+/* Builds the quadtree over nvor_b blobs and its multipole expansions.
+ * vor_b is x0,y0,x1,y1,... (Fortran vor_b(2,nvor_b), ver-c/memo:1-2); cir_b
+ * and cor_b are the circulations and core radii.  Inputs are read during the
+ * call only.  Parameters come from "../cal_cond/fmm_param.dat" relative to
+ * the working directory, re-read on every call, as quadtree.c:12-27 does,
+ * unless fmm_set_params() fixed them.  nvor_b == 0 gives an empty tree
+ * (velocities 0).  The velocities at all blobs are computed on the GPU right
+ * away (the solver asks for exactly those, e_velo_mpi.f90:37-48), so that
+ * biot_tree at a blob position is a lookup. */
+void make_tree(int nvor_b, const double *vor_b, const double *cir_b, const double *cor_b);
+
+/* Velocity (u, v) induced at (x, y) by the blobs of the last make_tree, by
+ * the reference's treecode rule (fmm.c:69-118).  A blob position returns the
+ * velocity computed during make_tree; any other point is evaluated on the GPU
+ * one at a time (use fmm_eval_batch for many points). */
+void biot_tree(double x, double y, double *u, double *v);
+
+/* ---- extensions --------------------------------------------------------- */
+
+/* Fix nterms, num_p, theta, r_min (the four fields of fmm_param.dat) for all
+ * later make_tree calls; the parameter file is then no longer read.
+ * nterms in [1, 32], num_p >= 1, theta > 0. */
+int fmm_set_params(int nterms, int num_p, double theta, double r_min);
+/* Go back to reading a parameter file on every make_tree; path NULL means the
+ * reference's "../cal_cond/fmm_param.dat". */
+int fmm_use_param_file(const char *path);
+int fmm_get_params(int *nterms, int *num_p, double *theta, double *r_min);
+
+/* Evaluation mode.  FMM2D_MODE_TREECODE is the reference's algorithm
+ * (per-target opening test, M2S far field, direct near field). */
+#define FMM2D_MODE_TREECODE 0
+int fmm_set_mode(int mode);
+
+/* 0: make_tree does not evaluate at the blobs (biot_tree then evaluates on
+ * demand); 1 (default): it does, asynchronously. */
+int fmm_set_prefetch(int on);
+
+const char *fmm_last_error(void);
+
+/* make_tree on device-resident inputs (device pointers, same layout); no
+ * velocity prefetch.  For callers whose blobs already live in HBM. */
+int fmm_make_tree_device(int n, const double *d_xy, const double *d_cir, const double *d_cor);
+
+/* Velocities at all blobs of the last tree, in the caller's blob order,
+ * interleaved u0,v0,u1,v1,...  Host or device output buffer. */
+int fmm_eval_sources(double *uv);
+int fmm_eval_sources_device(double *d_uv);
+
+/* Velocities at nt arbitrary points xy (interleaved); host or device buffers. */
+int fmm_eval_batch(int nt, const double *xy, double *uv);
+int fmm_eval_batch_device(int nt, const double *d_xy, double *d_uv);
+
+/* Tree introspection (for tests and tools).  Nodes are reported in ascending
+ * global id g = m + (4^l - 1)/3 of the reference's numbering
+ * (morton_order.c:4-14): level-major, Morton order inside a level.  Arrays
+ * hold nnodes entries (lo: nnodes + 1; members: nvor_b; coeffs: 2*nnodes*nterms
+ * doubles).  is_leaf is 0 for the root.  members lists, for each leaf in node
+ * order, its blob ids in ascending order; lo[k] is where node k's list starts
+ * (internal nodes have empty lists). */
+int fmm_tree_size(int *nnodes, int *depth, int *nleaves);
+int fmm_dump_tree(uint64_t *gid, int32_t *level, double *cx, double *cy, double *region,
+                  int32_t *count, int32_t *is_leaf, int32_t *lo, int32_t *members, double *coeffs);
+
+/* Interaction lists of nt points, in the order the treecode visits them:
+ * gid[t*cap + i] and kind[t*cap + i] (0 = direct sum over a leaf, 1 = M2S of
+ * a cell); len[t] is the full length (entries past cap are not stored). */
+int fmm_dump_lists(int nt, const double *xy, int cap, uint64_t *gid, int32_t *kind, int32_t *len);
+
+/* FP64 direct sum over all ns sources at nt targets on the GPU: style 0 is
+ * biot_b (ver-c/src/includes/d_biot.f90:2-64, cut-off 1e-6, mirror off),
+ * style 1 the near-field rule of fmm.c:121-142 (cut-off 1e-10). */
+int fmm_direct(int ns, const double *xy, const double *cir, const double *cor, int nt,
+               const double *txy, double *uv, int style);
+int fmm_direct_device(int ns, const double *d_xy, const double *d_cir, const double *d_cor, int nt,
+                      const double *d_txy, double *d_uv, int style);
+
+/* Measurement (no reference counterpart: the reference has no timers).
+ * With timing on, HIP events on the library's stream bracket the phases of
+ * the last build and evaluation; fmm_get_timing fills up to 5 values in ms:
+ * [0] bbox + keys, [1] radix sort, [2] tree levels + leaf order + gather,
+ * [3] S2M + M2M, [4] the last evaluation kernel over all blobs. */
+int fmm_set_timing(int on);
+int fmm_get_timing(double *ms, int n);
+/* Treecode interactions of the last tree with all blobs as targets: far cells
+ * evaluated by M2S (cells holding blobs) and near source-target pairs. */
+int fmm_count_interactions(long long *m2s, long long *pairs);
+
+/* Wait for all work of the library; release all device memory. */
+int fmm_synchronize(void);
+void fmm_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

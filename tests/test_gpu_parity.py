@@ -1,0 +1,284 @@
+"""Parity of the HIP treecode (through the C ABI) with the reference and the oracle.
+
+Bars: the tree (global ids, levels, centres, regions, counts, leaf flags,
+leaf member lists) and the interaction lists are bit-exact; coefficients and
+velocities agree to 1e-12 relative (normwise max over nodes / targets).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden_cases, load_golden, rel_err, workloads, fmm_module
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def fmm():
+    m = fmm_module()
+    m.set_prefetch(True)
+    return m
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def compare_tree(fmm, n, nterms, ref, coeffs=True):
+    t = fmm.dump_tree(n, nterms)
+    for k in ("gid", "level", "count", "is_leaf"):
+        assert np.array_equal(t[k], ref[k]), k
+    for k in ("cx", "cy", "region"):
+        assert np.array_equal(t[k].view(np.uint64), ref[k].view(np.uint64)), k
+    # leaf member lists, leaf by leaf
+    leaf = np.nonzero(ref["is_leaf"])[0]
+    rl, tl = ref["leaf_lo"], t["leaf_lo"]
+    for i in leaf:
+        a = t["leaf_members"][tl[i]:tl[i] + t["count"][i]]
+        b = ref["leaf_members"][rl[i]:rl[i] + ref["count"][i]]
+        assert np.array_equal(a, b), f"leaf {ref['gid'][i]}"
+    if coeffs:
+        ca, cb = t["coeffs"], ref["coeffs"]
+        scale = np.abs(cb).max(axis=1, keepdims=True)
+        scale[scale == 0] = 1.0
+        err = (np.abs(ca - cb) / scale).max()
+        assert err <= TOL, f"coefficients rel err {err:.3e}"
+    return t
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_golden_reference(fmm, case):
+    """Tree, coefficients, velocities and lists vs the reference's own run."""
+    g = load_golden(case)
+    nterms, num_p, theta = int(g["params"][0]), int(g["params"][1]), float(g["params"][2])
+    _say(f"[golden] {case} N={g['xy'].shape[0]} p={nterms} num_p={num_p} theta={theta}")
+    fmm.set_params(nterms, num_p, theta)
+    fmm.make_tree(g["xy"], g["cir"], g["cor"])
+    n = g["xy"].shape[0]
+    compare_tree(fmm, n, nterms, g)
+    # velocities at the targets: via the per-point reference ABI and in batch
+    uv = fmm.eval_sources() if "txy" not in g else fmm.eval_batch(g["txy"])
+    l2, mx = rel_err(uv, g["uv"])
+    _say(f"  velocities rel_L2={l2:.2e} rel_max={mx:.2e}")
+    assert mx <= TOL
+    k = min(n, 50)
+    for i in range(k):
+        u, v = fmm.biot_tree(*g["xy"][i])
+        assert (u, v) == (uv[i, 0], uv[i, 1])
+    # interaction lists, exact
+    lists = fmm.dump_lists(g["list_xy"])
+    off = g["list_off"]
+    for j, (gid, kind) in enumerate(lists):
+        assert np.array_equal(gid, g["list_gid"][off[j]:off[j + 1]]), f"list {j}"
+        assert np.array_equal(kind, g["list_kind"][off[j]:off[j + 1]]), f"list kinds {j}"
+    luv = fmm.eval_batch(g["list_xy"])
+    assert rel_err(luv, g["list_uv"])[1] <= TOL
+
+
+CASES = [
+    ("uniform", 20000, 11, 10, 32, 0.5),
+    ("uniform", 5000, 12, 4, 2, 0.7),
+    ("uniform", 30000, 13, 16, 64, 0.5),
+    ("sheet", 40000, 14, 16, 64, 0.5),
+    ("mixture", 60000, 15, 12, 64, 0.5),
+    ("mixture", 20000, 16, 32, 16, 0.3),
+    ("uniform", 3000, 17, 1, 5, 0.5),
+    ("sheet", 8000, 18, 20, 200, 0.6),
+]
+
+
+def _inputs(kind, n, seed):
+    wl = workloads()
+    if kind == "uniform":
+        return wl.uniform_square(n, seed=seed)
+    if kind == "sheet":
+        return wl.sheet(n, seed=seed)
+    return wl.gaussian_mixture(n, seed=seed)
+
+
+@pytest.mark.parametrize("kind,n,seed,nterms,num_p,theta", CASES)
+def test_oracle_parity(fmm, oracle, kind, n, seed, nterms, num_p, theta):
+    """Seeded inputs vs the CPU oracle (itself bit-exact with the reference)."""
+    _say(f"[oracle] {kind} N={n} seed={seed} p={nterms} num_p={num_p} theta={theta}")
+    xy, cir, cor = _inputs(kind, n, seed)
+    T = oracle.Tree(xy, cir, cor, nterms, num_p, theta)
+    nd = T.nodes()
+    ref = dict(nd)
+    ref["coeffs"] = T.coeffs()
+    lo = np.zeros(len(nd["gid"]) + 1, np.int32)
+    mem = []
+    off = 0
+    for i in range(len(nd["gid"])):
+        lo[i] = off
+        if nd["is_leaf"][i]:
+            mem.append(nd["members"][nd["lo"][i]:nd["lo"][i] + nd["count"][i]])
+            off += nd["count"][i]
+    lo[-1] = off
+    ref["leaf_lo"], ref["leaf_members"] = lo, np.concatenate(mem)
+    fmm.set_params(nterms, num_p, theta)
+    fmm.make_tree(xy, cir, cor)
+    compare_tree(fmm, n, nterms, ref)
+    idx = workloads().sample_targets(n, 2000, seed=seed)
+    uv = fmm.eval_sources()
+    want = T.eval(xy[idx])
+    l2, mx = rel_err(uv[idx], want)
+    _say(f"  velocities rel_L2={l2:.2e} rel_max={mx:.2e}")
+    assert mx <= TOL
+    # off-blob targets, and lists
+    pts = np.stack([workloads().uniform(seed, 90, 64), workloads().uniform(seed, 91, 64)], axis=1)
+    pts = xy.min(0) + pts * (xy.max(0) - xy.min(0))
+    assert rel_err(fmm.eval_batch(pts), T.eval(pts))[1] <= TOL
+    lists = fmm.dump_lists(pts[:8])
+    for j, (gid, kind_) in enumerate(lists):
+        g2, k2 = T.list(*pts[j])
+        assert np.array_equal(gid, g2) and np.array_equal(kind_, k2)
+
+
+def test_biot_tree_lookup_paths(fmm):
+    """Blob positions in order (cursor), out of order (hash) and off-blob points
+    (GPU evaluation) all give what the batch evaluation gives."""
+    xy, cir, cor = workloads().uniform_square(5000, seed=21)
+    fmm.set_params(10, 32, 0.5)
+    fmm.make_tree(xy, cir, cor)
+    ref = fmm.eval_sources()
+    for i in range(100):
+        assert fmm.biot_tree(*xy[i]) == tuple(ref[i])
+    rng = np.random.default_rng(0)
+    for i in rng.permutation(5000)[:100]:
+        assert fmm.biot_tree(*xy[i]) == tuple(ref[i])
+    pts = xy[:20] + 1e-7
+    want = fmm.eval_batch(pts)
+    for j in range(20):
+        assert fmm.biot_tree(*pts[j]) == tuple(want[j])
+
+
+def test_no_prefetch_matches(fmm):
+    xy, cir, cor = workloads().sheet(4000, seed=22)
+    fmm.set_params(12, 16, 0.5)
+    fmm.make_tree(xy, cir, cor)
+    ref = fmm.eval_sources()
+    fmm.set_prefetch(False)
+    try:
+        fmm.make_tree(xy, cir, cor)
+        for i in range(0, 4000, 397):
+            assert fmm.biot_tree(*xy[i]) == tuple(ref[i])
+    finally:
+        fmm.set_prefetch(True)
+
+
+def test_empty_and_errors(fmm):
+    fmm.set_params(8, 4, 0.5)
+    fmm.make_tree(np.zeros((0, 2)), np.zeros(0), np.zeros(0))
+    assert fmm.biot_tree(0.3, 0.4) == (0.0, 0.0)
+    assert fmm.eval_sources().shape == (0, 2)
+    # num_p coincident blobs can never be separated: an error, not a hang
+    xy = np.full((4, 2), 0.25)
+    with pytest.raises(fmm.FmmError):
+        fmm.make_tree(xy, np.ones(4), np.ones(4))
+    assert fmm.biot_tree(0.3, 0.4) == (0.0, 0.0)
+    # three coincident blobs with num_p = 4 are fine
+    fmm.make_tree(xy[:3], np.ones(3), np.ones(3))
+    with pytest.raises(fmm.FmmError):
+        fmm.set_params(0, 4, 0.5)
+
+
+def test_param_file(fmm, tmp_path):
+    """make_tree reads ../cal_cond/fmm_param.dat like quadtree.c:14-23."""
+    (tmp_path / "cal_cond").mkdir()
+    (tmp_path / "run").mkdir()
+    (tmp_path / "cal_cond" / "fmm_param.dat").write_text(
+        "15    nterms\n250   num_p\n0.5d0 theta\n0.01  r_min ! for Tree\n")
+    cwd = os.getcwd()
+    os.chdir(tmp_path / "run")
+    try:
+        fmm.use_param_file(None)
+        xy, cir, cor = workloads().uniform_square(1000, seed=23)
+        fmm.make_tree(xy, cir, cor)
+        nt, npp, th, _ = fmm.get_params()
+        assert (nt, npp, th) == (15, 250, 0.5)
+        assert fmm.tree_size()["depth"] >= 1
+    finally:
+        os.chdir(cwd)
+        fmm.set_params(10, 32, 0.5)
+
+
+def test_direct_judge(fmm, oracle):
+    xy, cir, cor = workloads().gaussian_mixture(20000, seed=24)
+    t = xy[::97]
+    for style in (0, 1):
+        got = fmm.direct(xy, cir, cor, t, style=style)
+        want = oracle.direct(xy, cir, cor, t, style=style)
+        assert rel_err(got, want)[1] <= 1e-13
+
+
+def test_full_size_sheet(fmm, oracle):
+    """Configuration 2 size (1M sheet, p=16, leaf 64): velocities are linear in
+    the circulations on a fixed tree and match the oracle on sampled targets;
+    the error against the FP64 direct sum is reported (the treecode's far field
+    is the singular kernel, fmm.c:65, so it is not small at this core size)."""
+    wl = workloads()
+    n = 1_000_000
+    xy, cir, cor = wl.sheet(n, seed=2)
+    fmm.set_params(16, 64, 0.5)
+    fmm.set_prefetch(False)
+    try:
+        g1 = wl.normal(31, 0, n) / n
+        g2 = wl.normal(32, 0, n) / n
+        idx = wl.sample_targets(n, 2000, seed=5)
+        fmm.make_tree(xy, g1, cor)
+        u1 = fmm.eval_batch(xy[idx])
+        fmm.make_tree(xy, g2, cor)
+        u2 = fmm.eval_batch(xy[idx])
+        fmm.make_tree(xy, g1 + 2.0 * g2, cor)
+        u3 = fmm.eval_batch(xy[idx])
+        assert rel_err(u3, u1 + 2.0 * u2)[0] <= 1e-12
+        fmm.make_tree(xy, cir, cor)
+        u = fmm.eval_batch(xy[idx])
+        T = oracle.Tree(xy, cir, cor, 16, 64, 0.5)
+        sub = idx[:300]
+        want = T.eval(xy[sub])
+        l2, mx = rel_err(u[:300], want)
+        _say(f"  1M sheet vs oracle: rel_L2={l2:.2e} rel_max={mx:.2e}")
+        assert mx <= TOL
+        d = fmm.direct(xy, cir, cor, xy[idx], style=1)
+        l2, mx = rel_err(u, d)
+        _say(f"  1M sheet treecode vs FP64 direct: rel_L2={l2:.2e} rel_max={mx:.2e}")
+    finally:
+        fmm.set_prefetch(True)
+
+
+def test_full_size_mixture_tree_invariants(fmm):
+    """Configuration 3 size (4M Gaussian mixture, leaf 64): every blob is in
+    exactly one leaf, leaves hold < num_p blobs, split nodes >= num_p, and the
+    four children of a node hold exactly its blobs."""
+    wl = workloads()
+    n = 4_000_000
+    xy, cir, cor = wl.gaussian_mixture(n, seed=3)
+    fmm.set_params(12, 64, 0.5)
+    fmm.set_prefetch(False)
+    try:
+        fmm.make_tree(xy, cir, cor)
+        t = fmm.dump_tree(n, 12)
+        _say(f"  4M mixture: nodes={len(t['gid'])} depth={fmm.tree_size()['depth']}")
+        leaf = t["is_leaf"] == 1
+        assert (t["count"][leaf] < 64).all()
+        assert (t["count"][~leaf][1:] >= 64).all()
+        assert np.array_equal(np.sort(t["leaf_members"]), np.arange(n))
+        pos = {int(g): i for i, g in enumerate(t["gid"])}
+        lvl = t["level"]
+        for i in np.nonzero(~leaf)[0][:2000]:
+            g = int(t["gid"][i])
+            off = ((1 << (2 * int(lvl[i]))) - 1) // 3
+            m = g - off
+            coff = ((1 << (2 * (int(lvl[i]) + 1))) - 1) // 3
+            kids = [pos[coff + 4 * m + q] for q in range(4)]
+            assert sum(int(t["count"][k]) for k in kids) == int(t["count"][i])
+            h = 0.25 * t["region"][i]
+            assert t["cx"][kids[1]] == t["cx"][i] + h and t["cy"][kids[2]] == t["cy"][i] - h
+    finally:
+        fmm.set_prefetch(True)
