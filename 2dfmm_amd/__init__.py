@@ -22,6 +22,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfmm2d.so")
 
 MODE_TREECODE = 0
+MODE_FMM64 = 1
+MODE_MIXED = 2
 
 
 class FmmError(RuntimeError):
@@ -47,6 +49,8 @@ lib.fmm_use_param_file.argtypes = [C.c_char_p]
 lib.fmm_get_params.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)]
 lib.fmm_set_mode.argtypes = [C.c_int]
+lib.fmm_set_fmm_params.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+lib.fmm_selftest_mfma.argtypes = [C.c_int, C.POINTER(C.c_double)]
 lib.fmm_set_prefetch.argtypes = [C.c_int]
 lib.fmm_last_error.restype = C.c_char_p
 lib.fmm_make_tree_device.argtypes = [C.c_int, _vp, _vp, _vp]
@@ -100,6 +104,17 @@ def get_params():
 
 def set_mode(mode: int) -> None:
     _check(lib.fmm_set_mode(int(mode)), "set_mode")
+
+
+def set_fmm_params(p: int = 16, leaf: int = 64, theta: float = 0.5, cut_sigmas: float = 4.0) -> None:
+    _check(lib.fmm_set_fmm_params(int(p), int(leaf), float(theta), float(cut_sigmas)),
+           "set_fmm_params")
+
+
+def selftest_mfma(fp32: bool) -> float:
+    e = C.c_double()
+    _check(lib.fmm_selftest_mfma(1 if fp32 else 0, C.byref(e)), "selftest_mfma")
+    return e.value
 
 
 def set_prefetch(on: bool) -> None:

@@ -17,6 +17,8 @@
 // (style 0) or of fmm.c:121-142 (style 1); LDS tiles of 256 sources.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fmm2d_internal.h"
 
 namespace fmm2d {
@@ -209,11 +211,13 @@ int launch_parity_counts(const TreeView &tv, int nt, const double *d_txy, int32_
 // ---------------------------------------------------------------------------
 constexpr int kDirTile = 256;
 
-__global__ __launch_bounds__(256) void direct_kernel(int ns, const double2 *__restrict__ sxy,
+// Block (bx, by): targets bx*256.., sources of chunk by; raw sums per chunk.
+__global__ __launch_bounds__(256) void direct_kernel(int ns, int csize,
+                                                     const double2 *__restrict__ sxy,
                                                      const double *__restrict__ cir,
                                                      const double *__restrict__ cor, int nt,
                                                      const double2 *__restrict__ txy,
-                                                     double2 *__restrict__ uv, int style) {
+                                                     double2 *__restrict__ part, int style) {
   __shared__ double lx[kDirTile], ly[kDirTile], lg[kDirTile], ls[kDirTile];
   const int t = blockIdx.x * 256 + threadIdx.x;
   double x = 0.0, y = 0.0;
@@ -223,11 +227,12 @@ __global__ __launch_bounds__(256) void direct_kernel(int ns, const double2 *__re
     y = p.y;
   }
   const double cut = style == 0 ? 1.0e-6 : 1.0e-10;
+  const int s0 = blockIdx.y * csize, s1 = min(ns, s0 + csize);
   double ax = 0.0, ay = 0.0;
-  for (int b = 0; b < ns; b += kDirTile) {
+  for (int b = s0; b < s1; b += kDirTile) {
     __syncthreads();
     const int j = b + threadIdx.x;
-    if (j < ns) {
+    if (j < s1) {
       const double2 q = sxy[j];
       lx[threadIdx.x] = q.x;
       ly[threadIdx.x] = q.y;
@@ -235,7 +240,7 @@ __global__ __launch_bounds__(256) void direct_kernel(int ns, const double2 *__re
       ls[threadIdx.x] = cor[j];
     }
     __syncthreads();
-    const int m = min(kDirTile, ns - b);
+    const int m = min(kDirTile, s1 - b);
     for (int k = 0; k < m; k++) {
       const double rx = x - lx[k];
       const double ry = y - ly[k];
@@ -254,23 +259,52 @@ __global__ __launch_bounds__(256) void direct_kernel(int ns, const double2 *__re
       }
     }
   }
-  if (t < nt) {
-    if (style == 0) {
-      const double pitwo = 0.50 / 3.141592653589793238;
-      uv[t] = make_double2(ax * pitwo, ay * pitwo);
-    } else {
-      uv[t] = make_double2(0.5 * ax / M_PI, 0.5 * ay / M_PI);
-    }
+  if (t < nt) part[(size_t)blockIdx.y * nt + t] = make_double2(ax, ay);
+}
+
+// Chunk sums in chunk order, then the scaling of the chosen rule.
+__global__ __launch_bounds__(256) void direct_reduce_kernel(const double2 *__restrict__ part,
+                                                            int nchunks, int nt,
+                                                            double2 *__restrict__ uv, int style) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nt) return;
+  double ax = 0.0, ay = 0.0;
+  for (int c = 0; c < nchunks; c++) {
+    const double2 v = part[(size_t)c * nt + t];
+    ax += v.x;
+    ay += v.y;
+  }
+  if (style == 0) {
+    const double pitwo = 0.50 / 3.141592653589793238;
+    uv[t] = make_double2(ax * pitwo, ay * pitwo);
+  } else {
+    uv[t] = make_double2(0.5 * ax / M_PI, 0.5 * ay / M_PI);
   }
 }
 
+int direct_chunks(int ns, int nt) {
+  const int tb = (nt + 255) / 256;
+  int c = (2048 + tb - 1) / tb;
+  c = std::min(c, std::max(1, ns / 1024));
+  return std::max(c, 1);
+}
+
 int launch_direct(int ns, const double *d_sxy, const double *d_cir, const double *d_cor, int nt,
-                  const double *d_txy, double *d_uv, int style, hipStream_t s) {
+                  const double *d_txy, double *d_uv, int style, double2 *d_ws, size_t ws_len,
+                  hipStream_t s) {
   if (nt <= 0) return 0;
-  direct_kernel<<<(nt + 255) / 256, 256, 0, s>>>(ns, reinterpret_cast<const double2 *>(d_sxy),
-                                                 d_cir, d_cor, nt,
-                                                 reinterpret_cast<const double2 *>(d_txy),
-                                                 reinterpret_cast<double2 *>(d_uv), style);
+  const int nc = direct_chunks(ns, nt);
+  if ((size_t)nc * nt > ws_len) {
+    set_error("direct sum workspace too small");
+    return -1;
+  }
+  const int csize = ns > 0 ? (ns + nc - 1) / nc : 0;
+  direct_kernel<<<dim3((nt + 255) / 256, nc), 256, 0, s>>>(
+      ns, csize, reinterpret_cast<const double2 *>(d_sxy), d_cir, d_cor, nt,
+      reinterpret_cast<const double2 *>(d_txy), d_ws, style);
+  FMM_HIP_TRY(hipGetLastError());
+  direct_reduce_kernel<<<(nt + 255) / 256, 256, 0, s>>>(d_ws, nc, nt,
+                                                        reinterpret_cast<double2 *>(d_uv), style);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }

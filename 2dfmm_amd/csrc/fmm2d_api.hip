@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -81,8 +82,18 @@ struct Context {
   bool valid = false;
   int n = 0, nnodes = 0, depth = 0, nleaves = 0;
   int key_depth = kMaxDepth;
-  int t_nterms = 0;  // nterms the tree was built with
+  int t_nterms = 0, t_num_p = 0, t_mode = 0;  // what the tree was built with
   double t_theta = 0.0;
+  // accurate fast mode parameters and data
+  int fmm_p = 16, fmm_leaf = 64;
+  double fmm_theta = 0.5, fmm_cut = 4.0;
+  double rcut = 0.0;
+  int ntleaves = 0;  // non-empty leaves (targets of the fused kernel)
+  DevBuf<float4> loc;
+  DevBuf<double> sinv2, smax;
+  DevBuf<int32_t> leaves, lat;
+  DevBuf<uint32_t> lflag, lpos;
+  DevBuf<double2> direct_ws;  // partial sums of the direct judge
   DevBuf<double> xy, cir, cor;
   DevBuf<uint64_t> key, key_alt;
   DevBuf<uint32_t> idx, idx_alt, hist, hist_scan, perm, flag, pos, total;
@@ -167,8 +178,10 @@ static int read_param_file() {
   return 0;
 }
 
+// The parameter file holds the treecode's parameters; the accurate modes take
+// theirs from fmm_set_fmm_params.
 static int resolve_params() {
-  if (g.fixed_params) return 0;
+  if (g.fixed_params || g.mode != FMM2D_MODE_TREECODE) return 0;
   return read_param_file();
 }
 
@@ -180,9 +193,17 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   g.hash.clear();
   g.cursor = 0;
   g.n = n;
-  g.t_nterms = g.nterms;
-  g.t_theta = g.theta;
-  g.nnodes = g.depth = g.nleaves = 0;
+  g.t_mode = g.mode;
+  if (g.mode == FMM2D_MODE_TREECODE) {
+    g.t_nterms = g.nterms;
+    g.t_num_p = g.num_p;
+    g.t_theta = g.theta;
+  } else {
+    g.t_nterms = g.fmm_p;
+    g.t_num_p = g.fmm_leaf;
+    g.t_theta = g.fmm_theta;
+  }
+  g.nnodes = g.depth = g.nleaves = g.ntleaves = 0;
   if (n == 0) {
     g.valid = true;
     return 0;
@@ -205,7 +226,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   // (the sort may swap key/key_alt and idx/idx_alt; each pair is always sized
   // together, so their capacities are equal and need no fix-up)
   // level-by-level build
-  size_t est = 4 + 16 * ((size_t)n / (size_t)g.num_p + 1);
+  size_t est = 4 + 16 * ((size_t)n / (size_t)g.t_num_p + 1);
   if (g.nodes.ensure_keep(est, 0, s) || g.gparent.ensure_keep(est / 4 + 1, 0, s) ||
       g.split.ensure_keep(est / 4 + 1, 0, s))
     return -1;
@@ -214,7 +235,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   int child_base = 4, nsplit_prev = 1, level = 1;
   while (nsplit_prev > 0) {
     if (level > D) {
-      set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D, g.num_p);
+      set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D, g.t_num_p);
       return -1;
     }
     const int nchild = 4 * nsplit_prev;
@@ -223,14 +244,14 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
         g.flag.ensure(nchild, s) || g.pos.ensure(nchild, s))
       return -1;
     if (launch_build_level(g.key.p, 2 * D, level, g.split.p + g.split_off[level - 1], nsplit_prev,
-                           child_base, g.num_p, g.nodes.p, g.gparent.p, g.flag.p, s))
+                           child_base, g.t_num_p, g.nodes.p, g.gparent.p, g.flag.p, s))
       return -1;
     if (scan_u32(g.flag.p, g.pos.p, nchild, g.total.p, s)) return -1;
     FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     FMM_HIP_TRY(hipStreamSynchronize(s));
     const int nsplit = (int)g.h_total[0];
     if (nsplit > 0 && level == D) {
-      set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D, g.num_p);
+      set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D, g.t_num_p);
       return -1;
     }
     const int off = g.split_off[level];
@@ -246,16 +267,35 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   }
   g.nnodes = child_base;
   g.nleaves = (child_base - 4 + 1) - g.split_off.back();
-  if (launch_leaf_order(g.nodes.p, g.nnodes, g.idx.p, g.perm.p, g.num_p, s)) return -1;
+  if (launch_leaf_order(g.nodes.p, g.nnodes, g.idx.p, g.perm.p, g.t_num_p, s)) return -1;
   if (launch_gather(g.perm.p, n, dxy, dcir, dcor, g.sx.p, g.sy.p, g.sg.p, g.ss.p, s)) return -1;
   mark(3);
-  const size_t ncoef = (size_t)g.nnodes * g.nterms;
+  const size_t ncoef = (size_t)g.nnodes * g.t_nterms;
   if (g.coef.ensure(ncoef, s)) return -1;
   FMM_HIP_TRY(hipMemsetAsync(g.coef.p, 0, ncoef * sizeof(double2), s));
-  if (launch_s2m(g.nodes.p, g.nnodes, g.sx.p, g.sy.p, g.sg.p, g.nterms, g.coef.p, s)) return -1;
+  if (launch_s2m(g.nodes.p, g.nnodes, g.sx.p, g.sy.p, g.sg.p, g.t_nterms, g.coef.p, s)) return -1;
   for (int l = g.depth - 1; l >= 1; l--) {
     const int a = g.split_off[l], b = g.split_off[l + 1];
-    if (launch_m2m_level(g.nodes.p, g.split.p + a, b - a, g.nterms, g.coef.p, s)) return -1;
+    if (launch_m2m_level(g.nodes.p, g.split.p + a, b - a, g.t_nterms, g.coef.p, s)) return -1;
+  }
+  if (g.t_mode != FMM2D_MODE_TREECODE) {
+    // target leaves, leaf-relative FP32 sources, near-field radius
+    if (g.leaves.ensure(n, s) || g.lat.ensure(n, s) || g.lflag.ensure(n, s) || g.lpos.ensure(n, s) ||
+        g.loc.ensure(n, s) || g.sinv2.ensure(n, s) || g.smax.ensure(1, s))
+      return -1;
+    if (launch_leaf_list(g.nodes.p, g.nnodes, n, g.lat.p, g.lflag.p, g.lpos.p, g.leaves.p,
+                         g.total.p, s))
+      return -1;
+    if (launch_max_reduce(dcor, n, g.smax.p, s)) return -1;
+    double smax = 0.0;
+    FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FMM_HIP_TRY(hipMemcpyAsync(&smax, g.smax.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    FMM_HIP_TRY(hipStreamSynchronize(s));
+    g.ntleaves = (int)g.h_total[0];
+    g.rcut = g.fmm_cut * smax;
+    if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
+                          g.loc.p, g.sinv2.p, s))
+      return -1;
   }
   mark(4);
   g.valid = true;
@@ -284,11 +324,35 @@ static int need_tree() {
   return 0;
 }
 
+static FmmArgs fmm_args(double *d_uv) {
+  FmmArgs a;
+  a.nodes = g.nodes.p;
+  a.gparent = g.gparent.p;
+  a.coef = g.coef.p;
+  a.sx = g.sx.p;
+  a.sy = g.sy.p;
+  a.sg = g.sg.p;
+  a.sinv2 = g.sinv2.p;
+  a.loc = g.loc.p;
+  a.leaves = g.leaves.p;
+  a.nleaves = g.ntleaves;
+  a.perm = g.perm.p;
+  a.uv = reinterpret_cast<double2 *>(d_uv);
+  a.p = g.t_nterms;
+  a.theta = g.t_theta;
+  a.rcut = g.rcut;
+  return a;
+}
+
 static int eval_sources_dev(double *d_uv) {
   if (need_tree()) return -1;
   if (g.n == 0) return 0;
   mark(5);
-  const int rc = launch_parity_eval(view(), g.n, nullptr, g.perm.p, d_uv, g.stream);
+  int rc;
+  if (g.t_mode == FMM2D_MODE_TREECODE)
+    rc = launch_parity_eval(view(), g.n, nullptr, g.perm.p, d_uv, g.stream);
+  else
+    rc = launch_fmm_eval(fmm_args(d_uv), g.t_mode == FMM2D_MODE_MIXED, g.stream);
   mark(6);
   return rc;
 }
@@ -525,11 +589,58 @@ int fmm_get_params(int *nterms, int *num_p, double *theta, double *r_min) {
 
 int fmm_set_mode(int mode) {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (mode != FMM2D_MODE_TREECODE) {
+  if (mode != FMM2D_MODE_TREECODE && mode != FMM2D_MODE_FMM64 && mode != FMM2D_MODE_MIXED) {
     set_error("fmm_set_mode: unknown mode %d", mode);
     return -1;
   }
   g.mode = mode;
+  return 0;
+}
+
+int fmm_set_fmm_params(int p, int leaf, double theta, double cut_sigmas) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (p < 2 || p > 16 || leaf < 1 || leaf > 64 || !(theta > 0.0 && theta < 1.0) ||
+      !(cut_sigmas >= 0.0)) {
+    set_error("fmm_set_fmm_params: p=%d (2..16), leaf=%d (1..64), theta=%g (0..1), "
+              "cut_sigmas=%g (>=0)", p, leaf, theta, cut_sigmas);
+    return -1;
+  }
+  g.fmm_p = p;
+  g.fmm_leaf = leaf;
+  g.fmm_theta = theta;
+  g.fmm_cut = cut_sigmas;
+  return 0;
+}
+
+int fmm_selftest_mfma(int fp32, double *max_abs_err) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  double ha[16 * 8], hb[8 * 16], hout[256], href[256];
+  for (int i = 0; i < 128; i++) {
+    ha[i] = 0.25 * ((i * 37) % 17) - 2.0 + 0.001 * i;
+    hb[i] = 0.5 * ((i * 11) % 13) - 3.0 - 0.002 * i;
+  }
+  DevBuf<double> d;
+  if (d.ensure(128 + 128 + 256 + 256, g.stream)) return -1;
+  int rc = 0;
+  if (hipMemcpyAsync(d.p, ha, sizeof ha, hipMemcpyHostToDevice, g.stream) != hipSuccess ||
+      hipMemcpyAsync(d.p + 128, hb, sizeof hb, hipMemcpyHostToDevice, g.stream) != hipSuccess)
+    rc = -1;
+  if (!rc) rc = launch_mfma_selftest(d.p, d.p + 128, d.p + 256, d.p + 512, fp32, g.stream);
+  if (!rc && (hipMemcpyAsync(hout, d.p + 256, sizeof hout, hipMemcpyDeviceToHost, g.stream) !=
+                  hipSuccess ||
+              hipMemcpyAsync(href, d.p + 512, sizeof href, hipMemcpyDeviceToHost, g.stream) !=
+                  hipSuccess ||
+              hipStreamSynchronize(g.stream) != hipSuccess))
+    rc = -1;
+  d.release();
+  if (rc) {
+    set_error("mfma self-test failed to run");
+    return -1;
+  }
+  double e = 0.0;
+  for (int i = 0; i < 256; i++) e = std::max(e, std::fabs(hout[i] - href[i]));
+  *max_abs_err = e;
   return 0;
 }
 
@@ -694,7 +805,11 @@ int fmm_direct_device(int ns, const double *d_xy, const double *d_cir, const dou
     set_error("style must be 0 or 1");
     return -1;
   }
-  return launch_direct(ns, d_xy, d_cir, d_cor, nt, d_txy, d_uv, style, g.stream);
+  if (nt <= 0) return 0;
+  const size_t need = (size_t)direct_chunks(ns, nt) * nt;
+  if (g.direct_ws.ensure(need, g.stream)) return -1;
+  return launch_direct(ns, d_xy, d_cir, d_cor, nt, d_txy, d_uv, style, g.direct_ws.p,
+                       g.direct_ws.cap, g.stream);
 }
 
 int fmm_direct(int ns, const double *xy, const double *cir, const double *cor, int nt,
@@ -724,7 +839,9 @@ int fmm_direct(int ns, const double *xy, const double *cir, const double *cor, i
     set_error("copy failed");
     rc = -1;
   }
-  if (!rc) rc = launch_direct(ns, a.p, b.p, c.p, nt, t.p, o.p, style, s);
+  if (!rc && g.direct_ws.ensure((size_t)direct_chunks(ns, nt) * nt, s)) rc = -1;
+  if (!rc)
+    rc = launch_direct(ns, a.p, b.p, c.p, nt, t.p, o.p, style, g.direct_ws.p, g.direct_ws.cap, s);
   if (!rc && (hipMemcpyAsync(uv, o.p, sizeof(double) * 2 * nt, hipMemcpyDeviceToHost, s) != hipSuccess ||
               hipStreamSynchronize(s) != hipSuccess)) {
     set_error("copy failed");
@@ -764,6 +881,28 @@ int fmm_count_interactions(long long *m2s, long long *pairs) {
   *m2s = 0;
   *pairs = 0;
   if (g.n == 0) return 0;
+  if (g.t_mode != FMM2D_MODE_TREECODE) {
+    DevBuf<int64_t> c;
+    if (c.ensure(2 * (size_t)std::max(g.ntleaves, 1), g.stream)) return -1;
+    std::vector<int64_t> h(2 * (size_t)g.ntleaves);
+    int rc = launch_fmm_count(fmm_args(nullptr), c.p, g.stream);
+    if (!rc && (hipMemcpyAsync(h.data(), c.p, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost,
+                               g.stream) != hipSuccess ||
+                hipStreamSynchronize(g.stream) != hipSuccess)) {
+      set_error("count download failed");
+      rc = -1;
+    }
+    c.release();
+    if (rc) return rc;
+    long long a = 0, b = 0;
+    for (int i = 0; i < g.ntleaves; i++) {
+      a += h[2 * i];
+      b += h[2 * i + 1];
+    }
+    *m2s = a;
+    *pairs = b;
+    return 0;
+  }
   DevBuf<int32_t> c;
   if (c.ensure(2 * (size_t)g.n, g.stream)) return -1;
   std::vector<int32_t> h(2 * (size_t)g.n);
@@ -803,6 +942,8 @@ void fmm_release(void) {
   g.total.release(); g.sx.release(); g.sy.release(); g.sg.release(); g.ss.release();
   g.bbox_part.release(); g.uv_dev.release(); g.tmp_in.release(); g.tmp_out.release();
   g.nodes.release(); g.gparent.release(); g.split.release(); g.coef.release(); g.root.release();
+  g.loc.release(); g.sinv2.release(); g.smax.release(); g.leaves.release(); g.lat.release();
+  g.lflag.release(); g.lpos.release(); g.direct_ws.release();
   if (g.h_uv) (void)hipHostFree(g.h_uv);
   g.h_uv = nullptr;
   g.h_uv_cap = 0;

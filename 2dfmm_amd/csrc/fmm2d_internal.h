@@ -102,7 +102,37 @@ int launch_parity_lists(const TreeView &tv, int nt, const double *d_txy, int cap
 int launch_parity_counts(const TreeView &tv, int nt, const double *d_txy, int32_t *d_counts,
                          hipStream_t s);
 // FP64 direct sum over all sources (style 0: biot_b rule, 1: direct_evaluate rule).
+// Sources are split into direct_chunks(ns, nt) chunks (grid y); d_ws holds
+// chunks * nt partial sums, reduced in chunk order.
+int direct_chunks(int ns, int nt);
 int launch_direct(int ns, const double *d_sxy, const double *d_cir, const double *d_cor, int nt,
-                  const double *d_txy, double *d_uv, int style, hipStream_t s);
+                  const double *d_txy, double *d_uv, int style, double2 *d_ws, size_t ws_len,
+                  hipStream_t s);
+
+// ---- accurate fast mode (fmm_kernels.hip) ------------------------------
+struct FmmArgs {
+  const NodeRec *nodes;
+  const int32_t *gparent;
+  const double2 *coef;  // unscaled a_k, p per node
+  const double *sx, *sy, *sg, *sinv2;
+  const float4 *loc;    // (x - c_leaf, y - c_leaf, cir, 1/sigma^2) in FP32
+  const int32_t *leaves;
+  int nleaves;
+  const uint32_t *perm;
+  double2 *uv;  // caller order
+  int p;
+  double theta, rcut;
+};
+int launch_mfma_selftest(const double *a, const double *b, double *out, double *ref, int fp32,
+                         hipStream_t s);
+int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleaves,
+                      const double *sx, const double *sy, const double *sg, const double *ss,
+                      float4 *loc, double *sinv2, hipStream_t s);
+// Non-empty leaves in blob (Morton) order; their number lands in *d_count.
+int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, uint32_t *d_flag,
+                     uint32_t *d_pos, int32_t *d_leaves, uint32_t *d_count, hipStream_t s);
+int launch_max_reduce(const double *d_v, int n, double *d_out, hipStream_t s);
+int launch_fmm_eval(const FmmArgs &args, bool mixed, hipStream_t s);
+int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s);
 
 }  // namespace fmm2d

@@ -3,16 +3,19 @@
 One step = one full pass of the hot path on device-resident blobs: bounding
 box, Morton keys, radix sort, level-by-level tree, S2M, M2M and the velocity
 at every blob (targets = sources), i.e. make_tree + biot_tree at all N blobs
-(e_velo_mpi.f90:32-48).  Default workload: N = 4,000,000 blobs from an
-8-component Gaussian mixture (core radius 5e-4, circulation N(0, 1/N)).
+(e_velo_mpi.f90:32-48).  Workload: N = 4,000,000 blobs from an 8-component
+Gaussian mixture (centres U[0,1]^2, std log-uniform in [1e-3, 5e-2], core
+radius 5e-4, circulation N(0, 1/N)), seeded.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--nterms P]
+The headline `value` is the mixed-precision mode (FP32 near field with
+compensated sums, FP32-MFMA far-field contraction, FP64 expansions and
+accumulation); the plain FP64 mode is timed on the same input, and each
+states its error against the FP64 direct sum on sampled targets.  The
+reference-rule treecode is timed beside them.
 
-Prints one JSON line.  Besides the throughput it carries the dominant
-kernel's roofline (algorithmic FP64 flops / its event-timed duration), the
-accuracy against an FP64 direct sum on sampled targets, and a CPU baseline
-(the oracle restatement of the reference algorithm on the host cores, on a
-bounded sample, extrapolated to all targets).
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--p P]
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -29,7 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "target velocities/s at N=4M, rel.err≤1e-6 vs FP64 direct: mixed+learned vs plain"
-FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+PEAK = {"f64": 78.6, "f32": 157.3}  # MI355X TFLOP/s; vector and matrix peaks coincide
 
 
 def log(*a):
@@ -37,9 +40,9 @@ def log(*a):
 
 
 def cpu_baseline(xy, cir, cor, nterms, num_p, theta, n_targets, seed, threads):
-    """Oracle (reference-semantics C restatement): the tree of all N blobs built
-    single-threaded as the reference does, then a sample of targets on
-    `threads` host threads; the rate is extrapolated to all N targets."""
+    """Oracle (reference-semantics C restatement of the treecode): the tree of
+    all N blobs built single-threaded as the reference does, then a sample of
+    targets on `threads` host threads; the rate is extrapolated to N targets."""
     from oracle import oracle as O
     wl = importlib.import_module("2dfmm_amd.workloads")
     n = xy.shape[0]
@@ -56,7 +59,8 @@ def cpu_baseline(xy, cir, cor, nterms, num_p, theta, n_targets, seed, threads):
         "unit": "target velocities/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle tree build of all {n} blobs on 1 thread ({t_build:.1f} s) + "
+        "sample": (f"reference-rule treecode (oracle restatement, p={nterms}, leaf {num_p}, "
+                   f"theta {theta}): tree of all {n} blobs on 1 thread ({t_build:.1f} s) + "
                    f"{len(idx)} sampled targets on {threads} threads ({t_eval:.2f} s), "
                    f"extrapolated to {n} targets"),
     }
@@ -69,9 +73,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--nterms", type=int, default=16)
-    ap.add_argument("--num-p", type=int, default=64)
+    ap.add_argument("--p", type=int, default=16, help="expansion terms of the accurate modes")
+    ap.add_argument("--leaf", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--cut", type=float, default=4.0, help="near-field radius in core radii")
+    ap.add_argument("--modes", default="mixed,fmm64,treecode")
     ap.add_argument("--judge-targets", type=int, default=10_000)
     ap.add_argument("--cpu-targets", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -96,55 +102,70 @@ def main():
     d_cor = torch.from_numpy(cor).to(dev)
     d_uv = torch.empty((n, 2), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
-
-    fmm.set_params(args.nterms, args.num_p, args.theta)
     fmm.set_timing(True)
+
+    # FP64 direct sum on sampled targets (the judge)
+    idx = wl.sample_targets(n, args.judge_targets, seed=7)
+    d_t = torch.from_numpy(np.ascontiguousarray(xy[idx])).to(dev)
+    d_ref = torch.empty((len(idx), 2), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fmm.direct_device(n, d_xy.data_ptr(), d_cir.data_ptr(), d_cor.data_ptr(), len(idx),
+                      d_t.data_ptr(), d_ref.data_ptr(), 1)
+    fmm.synchronize()
+    log(f"[bench] FP64 direct judge on {len(idx)} targets: {time.perf_counter() - t0:.2f} s")
+    ref = d_ref.cpu().numpy()
 
     def step():
         fmm.make_tree_device(n, d_xy.data_ptr(), d_cir.data_ptr(), d_cor.data_ptr())
         fmm.eval_sources_device(d_uv.data_ptr())
 
-    for w in range(args.warmup):
-        step()
+    results = {}
+    for mode in [m for m in args.modes.split(",") if m]:
+        if mode == "treecode":
+            fmm.set_mode(fmm.MODE_TREECODE)
+            fmm.set_params(args.p, args.leaf, args.theta)
+            prec, kernel = "f64", "parity_eval_kernel<0>"
+        else:
+            fmm.set_mode(fmm.MODE_MIXED if mode == "mixed" else fmm.MODE_FMM64)
+            fmm.set_fmm_params(args.p, args.leaf, args.theta, args.cut)
+            prec = "f32" if mode == "mixed" else "f64"
+            kernel = "fmm_leaf_kernel<%s>" % ("true" if mode == "mixed" else "false")
+        for w in range(args.warmup):
+            step()
+            log(f"[bench] {mode} warmup {w + 1}/{args.warmup}: {fmm.get_timing()}")
+        phases = []
+        torch.cuda.synchronize()
         fmm.synchronize()
-        log(f"[bench] warmup {w + 1}/{args.warmup}: {fmm.get_timing()}")
-    phases = []
-    torch.cuda.synchronize()
-    fmm.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step()
-        phases.append(fmm.get_timing())  # waits for the step (events on the library stream)
-    fmm.synchronize()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
-    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step; phases(ms) {ph}")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            phases.append(fmm.get_timing())  # waits for the step (events on the library stream)
+        fmm.synchronize()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+        got = d_uv.cpu().numpy()[idx]
+        l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        mx = float(np.abs(got - ref).max() / np.abs(ref).max())
+        far, pairs = fmm.count_interactions()
+        p = args.p
+        if mode == "treecode":
+            flops = far * (8 * (p - 1) + 10) + pairs * 20
+        else:
+            flops = far * 8 * p * p + pairs * 20 + n * 8 * p
+        ach = flops / (ph["evaluate"] * 1e-3) / 1e12
+        results[mode] = {"ms_per_step": ms, "value": n / (ms * 1e-3), "rel_L2": l2, "rel_max": mx,
+                         "phases_ms": ph, "tree": fmm.tree_size(), "far": far, "pairs": pairs,
+                         "flops": flops, "achieved_tflops": ach, "precision": prec,
+                         "kernel": kernel}
+        log(f"[bench] {mode}: {ms:.2f} ms/step ({n / (ms * 1e-3):.4g} vel/s), rel_L2={l2:.3e} "
+            f"rel_max={mx:.3e}, far={far} pairs={pairs}, eval {ph['evaluate']:.2f} ms "
+            f"= {ach:.2f} TFLOP/s; phases {ph}")
 
-    # algorithmic work of the evaluation kernel (outside the timed region)
-    m2s, pairs = fmm.count_interactions()
-    p = args.nterms
-    flops = m2s * (8 * (p - 1) + 10) + pairs * 20
-    eval_ms = ph["evaluate"]
-    achieved = flops / (eval_ms * 1e-3) / 1e12
-    info = fmm.tree_size()
-
-    # accuracy against the FP64 direct sum on sampled targets
-    idx = wl.sample_targets(n, args.judge_targets, seed=7)
-    uv = d_uv.cpu().numpy()
-    d_t = torch.from_numpy(np.ascontiguousarray(xy[idx])).to(dev)
-    d_ref = torch.empty((len(idx), 2), dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    fmm.direct_device(n, d_xy.data_ptr(), d_cir.data_ptr(), d_cor.data_ptr(), len(idx),
-                      d_t.data_ptr(), d_ref.data_ptr(), 1)
-    fmm.synchronize()
-    ref = d_ref.cpu().numpy()
-    got = uv[idx]
-    rel_l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-    rel_max = float(np.abs(got - ref).max() / np.abs(ref).max())
-    log(f"[bench] accuracy vs FP64 direct on {len(idx)} targets: rel_L2={rel_l2:.3e} "
-        f"rel_max={rel_max:.3e}")
+    head_mode = "mixed" if "mixed" in results else next(iter(results))
+    head = results[head_mode]
+    plain = results.get("fmm64")
 
     cpu = None
     if not args.no_cpu:
@@ -153,35 +174,36 @@ def main():
         except AttributeError:
             threads = min(16, os.cpu_count() or 1)
         log(f"[bench] cpu baseline: oracle on {threads} threads")
-        cpu = cpu_baseline(xy, cir, cor, args.nterms, args.num_p, args.theta, args.cpu_targets,
-                           11, threads)
+        cpu = cpu_baseline(xy, cir, cor, args.p, args.leaf, args.theta, args.cpu_targets, 11,
+                           threads)
         log(f"[bench] cpu baseline {cpu['value']:.4g} velocities/s ({cpu['sample']})")
 
-    value = n / (ms_per_step * 1e-3)
+    peak = PEAK[head["precision"]]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "target velocities/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head["precision"] == "f32" else "f64",
         "data": "synthetic (seeded Gaussian mixture)",
         "config": {"workload": "gaussian_mixture_8", "n_blobs": n, "targets": n,
-                   "core_radius": 5e-4, "mode": "treecode", "nterms": args.nterms,
-                   "num_p": args.num_p, "theta": args.theta, "parallelism": "single GPU"},
-        "accuracy": {"vs": "FP64 direct sum (direct_evaluate rule)", "targets": len(idx),
-                     "rel_L2": rel_l2, "rel_max": rel_max, "meets_1e-6": rel_l2 <= 1e-6},
-        "phases_ms": ph,
-        "tree": info,
-        "roofline": {"kernel": "parity_eval_kernel<0>", "bound": "mfma", "achieved": achieved,
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "flops_per_launch": flops, "m2s_per_launch": m2s,
-                     "p2p_pairs_per_launch": pairs, "avg_launch_ms": eval_ms},
+                   "core_radius": 5e-4, "mode": head_mode, "p": args.p, "leaf": args.leaf,
+                   "theta": args.theta, "cut_core_radii": args.cut, "parallelism": "single GPU"},
+        "accuracy": {"vs": "FP64 direct sum (fmm.c:121-142 rule)", "targets": len(idx),
+                     "rel_L2": head["rel_L2"], "rel_max": head["rel_max"],
+                     "meets_1e-6": head["rel_L2"] <= 1e-6},
+        "speedup_vs_plain_fp64": (plain["ms_per_step"] / head["ms_per_step"]) if plain else None,
+        "modes": results,
+        "roofline": {"kernel": head["kernel"], "bound": "mfma", "achieved": head["achieved_tflops"],
+                     "peak": peak, "unit": "TFLOP/s", "frac": head["achieved_tflops"] / peak,
+                     "traffic": None, "flops_per_launch": head["flops"],
+                     "avg_launch_ms": head["phases_ms"]["evaluate"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

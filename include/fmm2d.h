@@ -58,10 +58,29 @@ int fmm_set_params(int nterms, int num_p, double theta, double r_min);
 int fmm_use_param_file(const char *path);
 int fmm_get_params(int *nterms, int *num_p, double *theta, double *r_min);
 
-/* Evaluation mode.  FMM2D_MODE_TREECODE is the reference's algorithm
- * (per-target opening test, M2S far field, direct near field). */
+/* Evaluation mode (no reference counterpart beyond mode 0).
+ * FMM2D_MODE_TREECODE is the reference's algorithm (per-target opening test,
+ * M2S far field, direct near field), parameters from fmm_param.dat.
+ * FMM2D_MODE_FMM64 and FMM2D_MODE_MIXED bound the error against the
+ * regularised FP64 direct sum: cells are far only when theta-separated from
+ * the target leaf AND farther than cut_sigmas * (largest core radius), far
+ * cells enter one local expansion per target leaf through a dense complex
+ * contraction on MFMA, near leaves are summed directly.  MIXED runs the near
+ * sums in FP32 (leaf-relative coordinates, Kahan sums) and the contraction on
+ * FP32 MFMA; expansions and the final accumulation stay FP64.  These modes
+ * apply to the all-blob evaluation (make_tree's prefetch, fmm_eval_sources*);
+ * other points are evaluated by the treecode rule on the same tree. */
 #define FMM2D_MODE_TREECODE 0
+#define FMM2D_MODE_FMM64 1
+#define FMM2D_MODE_MIXED 2
 int fmm_set_mode(int mode);
+/* p in [2, 16] expansion terms, leaf in [1, 64] leaf capacity, theta in (0, 1),
+ * cut_sigmas >= 0.  Defaults 16, 64, 0.5, 4. */
+int fmm_set_fmm_params(int p, int leaf, double theta, double cut_sigmas);
+/* Checks the MFMA lane layout the contraction relies on: a 16x8 by 8x16
+ * product through the FP64 (fp32 = 0) or FP32 (fp32 = 1) MFMA path against a
+ * plain loop on the device; *max_abs_err gets the largest difference. */
+int fmm_selftest_mfma(int fp32, double *max_abs_err);
 
 /* 0: make_tree does not evaluate at the blobs (biot_tree then evaluates on
  * demand); 1 (default): it does, asynchronously. */
