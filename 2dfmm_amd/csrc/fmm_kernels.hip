@@ -70,7 +70,10 @@ __device__ inline void kahan_add(float &s, float &c, float v) {
 
 // ---------------------------------------------------------------------------
 // MFMA helpers: D(16x16) += A(16x4) B(4x16); lane l supplies A[l%16][l/16] and
-// B[l/16][l%16] and holds D[4*(l/16)+r][l%16], r = 0..3.
+// B[l/16][l%16] and holds D[row][l%16] in its register r = 0..3, where
+//   f64 (v_mfma_f64_16x16x4_f64): row = l/16 + 4 r   (rows interleaved)
+//   f32 (v_mfma_f32_16x16x4_f32): row = 4 (l/16) + r (rows blocked)
+// (measured with one-hot operands on gfx950; fmm_selftest_mfma checks it).
 // ---------------------------------------------------------------------------
 __device__ inline d4_t mfma16x16x4(double a, double b, d4_t c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -78,6 +81,8 @@ __device__ inline d4_t mfma16x16x4(double a, double b, d4_t c) {
 __device__ inline f4_t mfma16x16x4(float a, float b, f4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+__device__ inline int mfma_row(double, int k, int r) { return k + 4 * r; }
+__device__ inline int mfma_row(float, int k, int r) { return 4 * k + r; }
 
 // Self-test of the lane layout: D = A B for A 16x8, B 8x16 (two MFMA steps)
 // against a plain loop.  a, b row-major; out gets D (16x16) from MFMA and ref.
@@ -88,7 +93,7 @@ __global__ __launch_bounds__(64) void mfma_selftest_kernel(const double *a, cons
   V acc = {0, 0, 0, 0};
   for (int s = 0; s < 2; s++)
     acc = mfma16x16x4((R)a[i * 8 + 4 * s + k], (R)b[(4 * s + k) * 16 + i], acc);
-  for (int r = 0; r < 4; r++) out[(4 * k + r) * 16 + i] = (double)acc[r];
+  for (int r = 0; r < 4; r++) out[mfma_row(R(0), k, r) * 16 + i] = (double)acc[r];
   if (l < 16)
     for (int j = 0; j < 16; j++) {
       double s = 0.0;
@@ -232,6 +237,7 @@ struct FarAcc<false> {
   __device__ inline void fold() {}
   __device__ inline double r0(int r) const { return g0[r]; }
   __device__ inline double r1(int r) const { return g1[r]; }
+  __device__ inline int row(int k, int r) const { return mfma_row(0.0, k, r); }
 };
 
 template <>
@@ -257,6 +263,7 @@ struct FarAcc<true> {
   }
   __device__ inline double r0(int r) const { return g0[r]; }
   __device__ inline double r1(int r) const { return g1[r]; }
+  __device__ inline int row(int k, int r) const { return mfma_row(0.0f, k, r); }
 };
 
 template <bool MIXED>
@@ -403,9 +410,9 @@ __global__ __launch_bounds__(256) void fmm_leaf_kernel(FmmArgs a) {
   acc.fold();
 
   // local expansion: Lt_l = (1/s_A) sum_k C(l+k,k) G[l][k]; lane holds rows
-  // 4*mk + r, column mi of G (real part in r0, imaginary in r1)
+  // acc.row(mk, r), column mi of G (real part in r0, imaginary in r1)
   for (int r = 0; r < 4; r++) {
-    const int row = 4 * mk + r;
+    const int row = acc.row(mk, r);
     // C(row + mi, mi)
     double bc = 1.0;
     for (int t = 1; t <= mi; t++) bc = bc * (double)(row + t) / (double)t;
