@@ -104,7 +104,9 @@ __global__ __launch_bounds__(256) void parity_eval_kernel(TreeView tv, int nt,
   const double theta = tv.theta;
   double phr = 0.0, phi = 0.0, ud = 0.0, vd = 0.0;
   Slots sl;
-  int d = 0, cur = 0, node = 0, nl = 0;
+  int d = 0, cur = 0, nl = 0;
+  // root: 0 for a single tree; in a forest, the system of the (sorted) blob
+  int node = (tv.root_shift < 0 || txy) ? 0 : (int)(tv.key[t] >> tv.root_shift);
   bool expand = true;
   constexpr bool RECORD = MODE == 1, COUNT = MODE == 2;
   int32_t *my_list = RECORD ? list + (size_t)t * cap * 2 : nullptr;

@@ -94,6 +94,17 @@ struct Context {
   DevBuf<int32_t> leaves, lat;
   DevBuf<uint32_t> lflag, lpos;
   DevBuf<double2> direct_ws;  // partial sums of the direct judge
+  // forests (batched systems)
+  int nsys = 1, root_shift = -1;
+  DevBuf<int32_t> doff;
+  std::vector<int32_t> h_roots;
+  // persistent evaluation and its graph
+  DevBuf<uint32_t> ntl_dev, counter;
+  int persist_blocks = 0;  // 0: one wave per leaf over a sized grid
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  unsigned char graph_key[sizeof(FmmArgs) + 16] = {};
   DevBuf<double> xy, cir, cor;
   DevBuf<uint64_t> key, key_alt;
   DevBuf<uint32_t> idx, idx_alt, hist, hist_scan, perm, flag, pos, total;
@@ -185,14 +196,19 @@ static int resolve_params() {
   return read_param_file();
 }
 
-// The tree build of make_tree on device inputs (caller order).
-static int build(int n, const double *dxy, const double *dcir, const double *dcor) {
+// The tree build of make_tree on device inputs (caller order).  With nsys > 1
+// the inputs are nsys independent systems, system s owning blobs
+// [h_off[s], h_off[s+1]); each gets its own tree (a forest) in the same pass.
+static int build(int n, const double *dxy, const double *dcir, const double *dcor, int nsys = 1,
+                 const int32_t *h_off = nullptr) {
   hipStream_t s = g.stream;
   g.valid = false;
   g.uv_pending = g.uv_ready = false;
   g.hash.clear();
   g.cursor = 0;
   g.n = n;
+  g.nsys = nsys;
+  g.root_shift = -1;
   g.t_mode = g.mode;
   if (g.mode == FMM2D_MODE_TREECODE) {
     g.t_nterms = g.nterms;
@@ -208,31 +224,53 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     g.valid = true;
     return 0;
   }
-  const int D = g.key_depth;
+  int sysbits = 0;
+  while ((1 << sysbits) < nsys) sysbits++;
+  const int D = nsys == 1 ? g.key_depth : std::min(g.key_depth, (64 - sysbits) / 2);
+  const int nroot_slots = (nsys + 3) & ~3;
   if (g.key.ensure(n, s) || g.key_alt.ensure(n, s) || g.idx.ensure(n, s) || g.idx_alt.ensure(n, s) ||
       g.perm.ensure(n, s) || g.sx.ensure(n, s) || g.sy.ensure(n, s) || g.sg.ensure(n, s) ||
-      g.ss.ensure(n, s) || g.root.ensure(1, s) || g.bbox_part.ensure(4 * 512, s) ||
+      g.ss.ensure(n, s) || g.root.ensure(nsys, s) || g.bbox_part.ensure(4 * 512, s) ||
       g.total.ensure(4, s) || g.hist.ensure(radix_sort_hist_words(n), s) ||
       g.hist_scan.ensure(radix_sort_hist_words(n), s))
     return -1;
+  size_t est = nroot_slots + 16 * ((size_t)n / (size_t)g.t_num_p + 1);
+  if (g.nodes.ensure_keep(est, 0, s) || g.gparent.ensure_keep(est / 4 + 1, 0, s) ||
+      g.split.ensure_keep(est / 4 + nsys + 1, 0, s))
+    return -1;
   mark(0);
-  if (launch_bbox(dxy, n, g.root.p, g.bbox_part.p, s)) return -1;
-  if (launch_keygen(dxy, n, g.root.p, D, g.key.p, g.idx.p, s)) return -1;
+  int nroots = 1;
+  if (nsys == 1) {
+    if (launch_bbox(dxy, n, g.root.p, g.bbox_part.p, s)) return -1;
+    if (launch_keygen(dxy, n, g.root.p, D, g.key.p, g.idx.p, s)) return -1;
+    if (launch_init_root(g.nodes.p, g.gparent.p, g.root.p, n, g.split.p, s)) return -1;
+  } else {
+    g.root_shift = 2 * D;
+    if (g.doff.ensure(nsys + 1, s)) return -1;
+    FMM_HIP_TRY(hipMemcpyAsync(g.doff.p, h_off, sizeof(int32_t) * (nsys + 1),
+                               hipMemcpyHostToDevice, s));
+    if (launch_forest_roots(dxy, n, g.doff.p, nsys, D, g.root.p, g.key.p, g.idx.p, g.nodes.p,
+                            g.gparent.p, nroot_slots, s))
+      return -1;
+    // level 0 split list: the non-empty systems
+    g.h_roots.clear();
+    for (int k = 0; k < nsys; k++)
+      if (h_off[k + 1] > h_off[k]) g.h_roots.push_back(k);
+    nroots = (int)g.h_roots.size();
+    if (nroots > 0)
+      FMM_HIP_TRY(hipMemcpyAsync(g.split.p, g.h_roots.data(), sizeof(int32_t) * nroots,
+                                 hipMemcpyHostToDevice, s));
+  }
   mark(1);
-  if (radix_sort_pairs(&g.key.p, &g.idx.p, &g.key_alt.p, &g.idx_alt.p, n, 2 * D, g.hist.p,
-                       g.hist_scan.p, s))
+  if (radix_sort_pairs(&g.key.p, &g.idx.p, &g.key_alt.p, &g.idx_alt.p, n, 2 * D + sysbits,
+                       g.hist.p, g.hist_scan.p, s))
     return -1;
   mark(2);
   // (the sort may swap key/key_alt and idx/idx_alt; each pair is always sized
   // together, so their capacities are equal and need no fix-up)
   // level-by-level build
-  size_t est = 4 + 16 * ((size_t)n / (size_t)g.t_num_p + 1);
-  if (g.nodes.ensure_keep(est, 0, s) || g.gparent.ensure_keep(est / 4 + 1, 0, s) ||
-      g.split.ensure_keep(est / 4 + 1, 0, s))
-    return -1;
-  if (launch_init_root(g.nodes.p, g.gparent.p, g.root.p, n, g.split.p, s)) return -1;
-  g.split_off.assign({0, 1});
-  int child_base = 4, nsplit_prev = 1, level = 1;
+  g.split_off.assign({0, nroots});
+  int child_base = nroot_slots, nsplit_prev = nroots, level = 1;
   while (nsplit_prev > 0) {
     if (level > D) {
       set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D, g.t_num_p);
@@ -266,7 +304,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     nsplit_prev = nsplit;
   }
   g.nnodes = child_base;
-  g.nleaves = (child_base - 4 + 1) - g.split_off.back();
+  g.nleaves = (child_base - nroot_slots + nroots) - g.split_off.back();
   if (launch_leaf_order(g.nodes.p, g.nnodes, g.idx.p, g.perm.p, g.t_num_p, s)) return -1;
   if (launch_gather(g.perm.p, n, dxy, dcir, dcor, g.sx.p, g.sy.p, g.sg.p, g.ss.p, s)) return -1;
   mark(3);
@@ -283,12 +321,13 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     if (g.leaves.ensure(n, s) || g.lat.ensure(n, s) || g.lflag.ensure(n, s) || g.lpos.ensure(n, s) ||
         g.loc.ensure(n, s) || g.sinv2.ensure(n, s) || g.smax.ensure(1, s))
       return -1;
+    if (g.ntl_dev.ensure(1, s) || g.counter.ensure(1, s)) return -1;
     if (launch_leaf_list(g.nodes.p, g.nnodes, n, g.lat.p, g.lflag.p, g.lpos.p, g.leaves.p,
-                         g.total.p, s))
+                         g.ntl_dev.p, s))
       return -1;
     if (launch_max_reduce(dcor, n, g.smax.p, s)) return -1;
     double smax = 0.0;
-    FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.ntl_dev.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     FMM_HIP_TRY(hipMemcpyAsync(&smax, g.smax.p, sizeof(double), hipMemcpyDeviceToHost, s));
     FMM_HIP_TRY(hipStreamSynchronize(s));
     g.ntleaves = (int)g.h_total[0];
@@ -313,6 +352,8 @@ static TreeView view() {
   tv.ss = g.ss.p;
   tv.nterms = g.t_nterms;
   tv.theta = g.t_theta;
+  tv.key = g.key.p;
+  tv.root_shift = g.root_shift;
   return tv;
 }
 
@@ -341,7 +382,44 @@ static FmmArgs fmm_args(double *d_uv) {
   a.p = g.t_nterms;
   a.theta = g.t_theta;
   a.rcut = g.rcut;
+  a.key = g.key.p;
+  a.root_shift = g.root_shift;
   return a;
+}
+
+// Persistent evaluation through a captured graph [reset counter, persistent
+// kernel].  The launch reads its work size from device memory, so the graph
+// stays valid across rebuilt trees; it is re-captured only when a pointer or
+// parameter in the kernel arguments changes.
+static int eval_fmm_graph(const FmmArgs &a, bool mixed) {
+  unsigned char key[sizeof g.graph_key];
+  memset(key, 0, sizeof key);
+  FmmArgs ka = a;
+  ka.nleaves = 0;  // the persistent kernel reads the count from the device
+  memcpy(key, &ka, sizeof(FmmArgs));
+  key[sizeof(FmmArgs)] = mixed ? 1 : 0;
+  memcpy(key + sizeof(FmmArgs) + 4, &g.persist_blocks, sizeof(int));
+  if (!g.graph_exec || memcmp(key, g.graph_key, sizeof key) != 0) {
+    if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.graph_exec = nullptr;
+    g.graph = nullptr;
+    FMM_HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launch_fmm_eval_persistent(a, mixed, g.ntl_dev.p, g.counter.p, g.persist_blocks,
+                                              g.stream);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(g.stream, &gr);
+    if (rc || e != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      if (!rc) set_error("graph capture failed: %s", hipGetErrorString(e));
+      return -1;
+    }
+    g.graph = gr;
+    FMM_HIP_TRY(hipGraphInstantiate(&g.graph_exec, g.graph, nullptr, nullptr, 0));
+    memcpy(g.graph_key, key, sizeof key);
+  }
+  FMM_HIP_TRY(hipGraphLaunch(g.graph_exec, g.stream));
+  return 0;
 }
 
 static int eval_sources_dev(double *d_uv) {
@@ -349,16 +427,26 @@ static int eval_sources_dev(double *d_uv) {
   if (g.n == 0) return 0;
   mark(5);
   int rc;
+  const bool mixed = g.t_mode == FMM2D_MODE_MIXED;
   if (g.t_mode == FMM2D_MODE_TREECODE)
     rc = launch_parity_eval(view(), g.n, nullptr, g.perm.p, d_uv, g.stream);
+  else if (g.persist_blocks > 0 && g.use_graph)
+    rc = eval_fmm_graph(fmm_args(d_uv), mixed);
+  else if (g.persist_blocks > 0)
+    rc = launch_fmm_eval_persistent(fmm_args(d_uv), mixed, g.ntl_dev.p, g.counter.p,
+                                    g.persist_blocks, g.stream);
   else
-    rc = launch_fmm_eval(fmm_args(d_uv), g.t_mode == FMM2D_MODE_MIXED, g.stream);
+    rc = launch_fmm_eval(fmm_args(d_uv), mixed, g.stream);
   mark(6);
   return rc;
 }
 
 static int eval_batch_dev(int nt, const double *d_txy, double *d_uv) {
   if (need_tree()) return -1;
+  if (g.nsys > 1) {
+    set_error("a forest of systems evaluates at its blobs only (fmm_eval_sources)");
+    return -1;
+  }
   if (nt <= 0) return 0;
   if (g.n == 0) {
     FMM_HIP_TRY(hipMemsetAsync(d_uv, 0, sizeof(double) * 2 * (size_t)nt, g.stream));
@@ -424,6 +512,10 @@ struct HostTree {
 
 static int fetch_tree(HostTree &h) {
   hipStream_t s = g.stream;
+  if (g.nsys > 1) {
+    set_error("tree dumps are for a single system");
+    return -1;
+  }
   h.nodes.resize(g.nnodes);
   h.gparent.resize(g.nnodes / 4);
   FMM_HIP_TRY(hipMemcpyAsync(h.nodes.data(), g.nodes.p, sizeof(NodeRec) * g.nnodes,
@@ -661,6 +753,63 @@ int fmm_make_tree_device(int n, const double *d_xy, const double *d_cir, const d
   }
   g.h_src.clear();
   return build(n, d_xy, d_cir, d_cor);
+}
+
+static int check_offsets(int nsys, const int32_t *off) {
+  if (nsys < 1 || !off || off[0] != 0) {
+    set_error("forest: need nsys >= 1 and offsets starting at 0");
+    return -1;
+  }
+  for (int k = 0; k < nsys; k++)
+    if (off[k + 1] < off[k]) {
+      set_error("forest: offsets must be non-decreasing");
+      return -1;
+    }
+  return 0;
+}
+
+int fmm_make_forest_device(int nsys, const int32_t *offsets, const double *d_xy,
+                           const double *d_cir, const double *d_cor) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init() || resolve_params() || check_offsets(nsys, offsets)) return -1;
+  g.h_src.clear();
+  return build(offsets[nsys], d_xy, d_cir, d_cor, nsys, offsets);
+}
+
+int fmm_make_forest(int nsys, const int32_t *offsets, const double *xy, const double *cir,
+                    const double *cor) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init() || resolve_params() || check_offsets(nsys, offsets)) return -1;
+  const int n = offsets[nsys];
+  hipStream_t s = g.stream;
+  g.valid = false;
+  if (n > 0) {
+    if (g.xy.ensure(2 * (size_t)n, s) || g.cir.ensure(n, s) || g.cor.ensure(n, s)) return -1;
+    FMM_HIP_TRY(hipMemcpyAsync(g.xy.p, xy, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+    FMM_HIP_TRY(hipMemcpyAsync(g.cir.p, cir, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    FMM_HIP_TRY(hipMemcpyAsync(g.cor.p, cor, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  }
+  g.h_src.clear();
+  return build(n, g.xy.p, g.cir.p, g.cor.p, nsys, offsets);
+}
+
+int fmm_set_persistent(int blocks, int use_graph) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (blocks < 0) {
+    set_error("blocks must be >= 0");
+    return -1;
+  }
+  g.persist_blocks = blocks;
+  g.use_graph = use_graph != 0 && blocks > 0;
+  return 0;
+}
+
+int fmm_device_cus(void) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return -1;
+  return cus;
 }
 
 int fmm_eval_sources_device(double *d_uv) {
@@ -944,6 +1093,11 @@ void fmm_release(void) {
   g.nodes.release(); g.gparent.release(); g.split.release(); g.coef.release(); g.root.release();
   g.loc.release(); g.sinv2.release(); g.smax.release(); g.leaves.release(); g.lat.release();
   g.lflag.release(); g.lpos.release(); g.direct_ws.release();
+  g.doff.release(); g.ntl_dev.release(); g.counter.release();
+  if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g.graph_exec = nullptr;
+  g.graph = nullptr;
   if (g.h_uv) (void)hipHostFree(g.h_uv);
   g.h_uv = nullptr;
   g.h_uv_cap = 0;

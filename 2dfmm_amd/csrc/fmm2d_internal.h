@@ -64,6 +64,12 @@ int launch_keygen(const double *d_xy, int n, const RootBox *d_root, int depth, u
 int radix_sort_pairs(uint64_t **key, uint32_t **idx, uint64_t **key_alt, uint32_t **idx_alt,
                      int n, int bits, uint32_t *d_hist, uint32_t *d_hist_scan, hipStream_t s);
 size_t radix_sort_hist_words(int n);
+// Forest of nsys systems (blobs [off[s], off[s+1]) on the device): per-system
+// root squares, keys (s << 2*depth) | path, root nodes 0..nsys-1 and padding
+// up to nroot_slots (a multiple of 4).
+int launch_forest_roots(const double *d_xy, int n, const int32_t *d_off, int nsys, int depth,
+                        RootBox *d_roots, uint64_t *d_key, uint32_t *d_idx, NodeRec *d_nodes,
+                        int32_t *d_gparent, int nroot_slots, hipStream_t s);
 int launch_init_root(NodeRec *d_nodes, int32_t *d_gparent, const RootBox *d_root, int n,
                      int32_t *d_split, hipStream_t s);
 int launch_build_level(const uint64_t *d_key, int depth_bits, int level, const int32_t *d_split_prev,
@@ -90,6 +96,9 @@ struct TreeView {
   const double *sx, *sy, *sg, *ss;
   int nterms;
   double theta;
+  // forests: the root of sorted blob t is key[t] >> root_shift (< 0: single tree)
+  const uint64_t *key;
+  int root_shift;
 };
 // Treecode velocities at targets (txy interleaved; or the sorted blobs
 // themselves when txy is null, then results are scattered through perm).
@@ -122,6 +131,10 @@ struct FmmArgs {
   double2 *uv;  // caller order
   int p;
   double theta, rcut;
+  // forests (many independent systems): root = key[first] >> root_shift;
+  // root_shift < 0 for a single tree (root 0)
+  const uint64_t *key;
+  int root_shift;
 };
 int launch_mfma_selftest(const double *a, const double *b, double *out, double *ref, int fp32,
                          hipStream_t s);
@@ -133,6 +146,10 @@ int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, u
                      uint32_t *d_pos, int32_t *d_leaves, uint32_t *d_count, hipStream_t s);
 int launch_max_reduce(const double *d_v, int n, double *d_out, hipStream_t s);
 int launch_fmm_eval(const FmmArgs &args, bool mixed, hipStream_t s);
+// Persistent grid of nblocks x 256 threads over a work counter; the number of
+// target leaves is read from *d_nleaves (graph-capturable).
+int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
+                               uint32_t *d_counter, int nblocks, hipStream_t s);
 int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s);
 
 }  // namespace fmm2d

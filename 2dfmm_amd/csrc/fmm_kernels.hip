@@ -266,13 +266,16 @@ struct FarAcc<true> {
   __device__ inline int row(int k, int r) const { return mfma_row(0.0f, k, r); }
 };
 
+// Root of the tree a node belongs to: 0 for a single tree; in a forest the
+// system id sits above the quadrant digits of the sorted keys.
+__device__ inline int root_of(const FmmArgs &a, int first) {
+  return a.root_shift < 0 ? 0 : (int)(a.key[first] >> a.root_shift);
+}
+
+// All work of one target leaf (index li of the leaf list) for one wave.
 template <bool MIXED>
-__global__ __launch_bounds__(256) void fmm_leaf_kernel(FmmArgs a) {
-  __shared__ double2 s_loc[4][kFmmMaxP];
-  __shared__ double2 s_red[4][64];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int li = blockIdx.x * 4 + w;
-  if (li >= a.nleaves) return;
+__device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
+                                 double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64]) {
   const int A = a.leaves[li];
   const NodeRec NA = a.nodes[A];
   const int cA = node_count(NA.cnt);
@@ -357,7 +360,7 @@ __global__ __launch_bounds__(256) void fmm_leaf_kernel(FmmArgs a) {
 
   // wave-uniform traversal of the whole tree (masks of opened children per level)
   Slots2 sl;
-  int d = 0, cur = 0, node = 0;
+  int d = 0, cur = 0, node = root_of(a, NA.first);
   bool expand = true;
   for (;;) {
     if (expand) {
@@ -455,6 +458,55 @@ __global__ __launch_bounds__(256) void fmm_leaf_kernel(FmmArgs a) {
     const double v = 0.5 * phr / M_PI + 0.5 * sya / M_PI;
     a.uv[a.perm[tj]] = make_double2(u, v);
   }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One wave per target leaf, leaf = 4 * block + wave.
+template <bool MIXED>
+__global__ __launch_bounds__(256) void fmm_leaf_kernel(FmmArgs a) {
+  __shared__ double2 s_loc[4][kFmmMaxP];
+  __shared__ double2 s_red[4][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = blockIdx.x * 4 + w;
+  if (li >= a.nleaves) return;
+  fmm_process_leaf<MIXED>(a, li, w, lane, s_loc, s_red);
+}
+
+// Persistent form: a fixed grid whose waves take target leaves from a work
+// counter until the device-side leaf count is exhausted.  Every wave exits
+// once the counter passes the count, so the grid always drains.  Nothing in
+// the launch depends on host-side counts, so it can be captured in a graph
+// and replayed on a rebuilt tree.
+template <bool MIXED>
+__global__ __launch_bounds__(256) void fmm_leaf_persistent_kernel(FmmArgs a,
+                                                                  const uint32_t *__restrict__ nleaves,
+                                                                  uint32_t *__restrict__ counter) {
+  __shared__ double2 s_loc[4][kFmmMaxP];
+  __shared__ double2 s_red[4][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t total = *nleaves;
+  for (;;) {
+    uint32_t li = 0;
+    if (lane == 0) li = atomicAdd(counter, 1u);
+    li = __shfl(li, 0, 64);
+    if (li >= total) break;
+    fmm_process_leaf<MIXED>(a, (int)li, w, lane, s_loc, s_red);
+  }
+}
+
+__global__ void zero_counter_kernel(uint32_t *c) { *c = 0u; }
+
+int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
+                               uint32_t *d_counter, int nblocks, hipStream_t s) {
+  zero_counter_kernel<<<1, 1, 0, s>>>(d_counter);
+  FMM_HIP_TRY(hipGetLastError());
+  if (mixed)
+    fmm_leaf_persistent_kernel<true><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
+  else
+    fmm_leaf_persistent_kernel<false><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
 }
 
 // Work of the fused kernel, one thread per target leaf with the same walk:
@@ -468,7 +520,7 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
   const double RA = NA.region * 0.70710678118654752;
   int64_t far = 0, pairs = 0;
   Slots2 sl;
-  int d = 0, cur = 0, node = 0;
+  int d = 0, cur = 0, node = root_of(a, NA.first);
   bool expand = true;
   for (;;) {
     if (expand) {

@@ -226,6 +226,129 @@ int launch_keygen(const double *d_xy, int n, const RootBox *d_root, int depth, u
 }
 
 // ---------------------------------------------------------------------------
+// forests: many independent systems, concatenated; system s owns blobs
+// [off[s], off[s+1]).  One block per system for its root square; the key of a
+// blob is (s << 2D) | quadrant path, so one sort orders systems, then paths.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bbox_segments_kernel(const double2 *__restrict__ xy,
+                                                            const int32_t *__restrict__ off,
+                                                            RootBox *__restrict__ roots) {
+  const int s = blockIdx.x;
+  const int b = off[s], e = off[s + 1];
+  double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+  for (int i = b + threadIdx.x; i < e; i += 256) {
+    const double2 p = xy[i];
+    mnx = fmin(mnx, p.x);
+    mxx = fmax(mxx, p.x);
+    mny = fmin(mny, p.y);
+    mxy = fmax(mxy, p.y);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mnx = fmin(mnx, __shfl_xor(mnx, o, 64));
+    mxx = fmax(mxx, __shfl_xor(mxx, o, 64));
+    mny = fmin(mny, __shfl_xor(mny, o, 64));
+    mxy = fmax(mxy, __shfl_xor(mxy, o, 64));
+  }
+  __shared__ double sm[4][4];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sm[w][0] = mnx; sm[w][1] = mxx; sm[w][2] = mny; sm[w][3] = mxy;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) {
+      mnx = fmin(mnx, sm[k][0]);
+      mxx = fmax(mxx, sm[k][1]);
+      mny = fmin(mny, sm[k][2]);
+      mxy = fmax(mxy, sm[k][3]);
+    }
+    mnx = fmin(mnx, sm[0][0]); mxx = fmax(mxx, sm[0][1]);
+    mny = fmin(mny, sm[0][2]); mxy = fmax(mxy, sm[0][3]);
+    RootBox r;
+    if (e > b) {
+      const double rx = fabs(mxx - mnx) + 0.0001;
+      const double ry = fabs(mxy - mny) + 0.0001;
+      r.region = rx > ry ? rx : ry;
+      r.cx = 0.5 * (mxx + mnx);
+      r.cy = 0.5 * (mxy + mny);
+    } else {
+      r.region = 0.0; r.cx = 0.0; r.cy = 0.0;
+    }
+    r.pad = 0.0;
+    roots[s] = r;
+  }
+}
+
+__global__ __launch_bounds__(256) void keygen_forest_kernel(const double2 *__restrict__ xy, int n,
+                                                            const int32_t *__restrict__ off,
+                                                            int nsys,
+                                                            const RootBox *__restrict__ roots,
+                                                            int depth, uint64_t *__restrict__ key,
+                                                            uint32_t *__restrict__ idx) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  // system of blob i: last s with off[s] <= i
+  int lo = 0, hi = nsys;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid;
+  }
+  const RootBox R = roots[lo];
+  const double2 p = xy[i];
+  double cx = R.cx, cy = R.cy, reg = R.region;
+  uint64_t k = 0;
+  for (int l = 0; l < depth; l++) {
+    const bool right = p.x >= cx;
+    const bool down = p.y <= cy;
+    k = (k << 2) | (uint64_t)((right ? 1 : 0) + (down ? 2 : 0));
+    const double h = 0.25 * reg;
+    cx = right ? cx + h : cx - h;
+    cy = down ? cy - h : cy + h;
+    reg = 0.5 * reg;
+  }
+  key[i] = ((uint64_t)lo << (2 * depth)) | k;
+  idx[i] = (uint32_t)i;
+}
+
+// Roots 0..nsys-1 (padding nodes up to nroot_slots, a multiple of 4).
+__global__ __launch_bounds__(256) void init_forest_kernel(NodeRec *__restrict__ nodes,
+                                                          int32_t *__restrict__ gparent,
+                                                          const RootBox *__restrict__ roots,
+                                                          const int32_t *__restrict__ off,
+                                                          int nsys, int nroot_slots) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= nroot_slots) return;
+  NodeRec r;
+  if (s < nsys) {
+    const RootBox b = roots[s];
+    const int cnt = off[s + 1] - off[s];
+    r.cx = b.cx; r.cy = b.cy; r.region = b.region;
+    r.first = off[s];
+    r.cnt = cnt > 0 ? (cnt | kInternalBit) : 0;
+  } else {
+    r.cx = 0.0; r.cy = 0.0; r.region = 0.0; r.first = 0; r.cnt = 0;
+  }
+  nodes[s] = r;
+  if ((s & 3) == 0) gparent[s >> 2] = -1;
+}
+
+int launch_forest_roots(const double *d_xy, int n, const int32_t *d_off, int nsys, int depth,
+                        RootBox *d_roots, uint64_t *d_key, uint32_t *d_idx, NodeRec *d_nodes,
+                        int32_t *d_gparent, int nroot_slots, hipStream_t s) {
+  bbox_segments_kernel<<<nsys, 256, 0, s>>>(reinterpret_cast<const double2 *>(d_xy), d_off,
+                                            d_roots);
+  FMM_HIP_TRY(hipGetLastError());
+  keygen_forest_kernel<<<(n + 255) / 256, 256, 0, s>>>(reinterpret_cast<const double2 *>(d_xy), n,
+                                                        d_off, nsys, d_roots, depth, d_key, d_idx);
+  FMM_HIP_TRY(hipGetLastError());
+  init_forest_kernel<<<(nroot_slots + 255) / 256, 256, 0, s>>>(d_nodes, d_gparent, d_roots, d_off,
+                                                               nsys, nroot_slots);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // LSD radix sort, 8-bit digits.  A tile is 4096 pairs, 1024 per wave: item j
 // of lane l of wave w is pair w*1024 + j*64 + l, so the tile order that must
 // be preserved is (wave, item, lane).  Lanes holding the same digit are

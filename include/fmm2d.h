@@ -92,6 +92,26 @@ const char *fmm_last_error(void);
  * velocity prefetch.  For callers whose blobs already live in HBM. */
 int fmm_make_tree_device(int n, const double *d_xy, const double *d_cir, const double *d_cor);
 
+/* Batched systems (no reference counterpart; for callers that advance many
+ * independent blob systems at once).  The nsys systems are concatenated:
+ * system s owns blobs [offsets[s], offsets[s+1]) (offsets on the host, nsys+1
+ * entries, offsets[0] = 0).  One pass builds one tree per system (a forest:
+ * per-system root squares, keys prefixed by the system id, one sort, one
+ * level loop); fmm_eval_sources* then evaluates every blob against its own
+ * system only.  Results per system equal those of make_tree on that system
+ * alone with the same parameters. */
+int fmm_make_forest(int nsys, const int32_t *offsets, const double *xy, const double *cir,
+                    const double *cor);
+int fmm_make_forest_device(int nsys, const int32_t *offsets, const double *d_xy,
+                           const double *d_cir, const double *d_cor);
+
+/* Accurate modes: evaluate with a persistent grid of `blocks` workgroups that
+ * take target leaves from a work counter (0 = one workgroup per four leaves),
+ * optionally replayed from a captured HIP graph (use_graph = 1). */
+int fmm_set_persistent(int blocks, int use_graph);
+/* Compute units of the current device (for sizing persistent grids). */
+int fmm_device_cus(void);
+
 /* Velocities at all blobs of the last tree, in the caller's blob order,
  * interleaved u0,v0,u1,v1,...  Host or device output buffer. */
 int fmm_eval_sources(double *uv);

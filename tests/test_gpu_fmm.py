@@ -51,7 +51,8 @@ def test_fmm64_vs_direct(fmm, oracle, kind, n, seed):
     uv = _run(fmm, fmm.MODE_FMM64, xy, cir, cor)
     l2, mx = rel_err(uv[idx], d)
     _say(f"  FP64 FMM p=16: rel_L2={l2:.2e} rel_max={mx:.2e}")
-    assert l2 <= 1e-7
+    # the 4-core-radius cut leaves exp(-16) = 1.1e-7 of each far pair's core term
+    assert l2 <= 5e-7
     uvm = _run(fmm, fmm.MODE_MIXED, xy, cir, cor)
     l2m, mxm = rel_err(uvm[idx], d)
     _say(f"  mixed FMM p=16: rel_L2={l2m:.2e} rel_max={mxm:.2e}")
@@ -69,7 +70,7 @@ def test_fmm_converges_in_p(fmm, oracle):
         errs.append(rel_err(uv[idx], d)[0])
     _say(f"  rel_L2 by p 4/8/12/16: {['%.2e' % e for e in errs]}")
     assert errs[0] > errs[1] > errs[2] and errs[3] <= errs[2]
-    assert errs[3] < 1e-7
+    assert errs[3] < 5e-7
 
 
 def test_fmm_cut_radius_matters(fmm, oracle):
@@ -93,7 +94,7 @@ def test_fmm_large_vs_gpu_direct(fmm):
     xy, cir, cor = wl.gaussian_mixture(n, seed=56)
     idx = wl.sample_targets(n, 3000, seed=56)
     d = fmm.direct(xy, cir, cor, xy[idx], style=1)
-    for mode, tol in ((fmm.MODE_FMM64, 1e-7), (fmm.MODE_MIXED, 1e-6)):
+    for mode, tol in ((fmm.MODE_FMM64, 5e-7), (fmm.MODE_MIXED, 1e-6)):
         uv = _run(fmm, mode, xy, cir, cor)
         l2, mx = rel_err(uv[idx], d)
         _say(f"  400k mixture mode {mode}: rel_L2={l2:.2e} rel_max={mx:.2e}")
