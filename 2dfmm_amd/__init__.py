@@ -63,6 +63,10 @@ lib.fmm_dump_tree.argtypes = [_up, _ip, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _dp]
 lib.fmm_dump_lists.argtypes = [C.c_int, _dp, C.c_int, _up, _ip, _ip]
 lib.fmm_direct.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_int]
 lib.fmm_direct_device.argtypes = [C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int]
+lib.fmm_make_forest.argtypes = [C.c_int, _ip, _dp, _dp, _dp]
+lib.fmm_make_forest_device.argtypes = [C.c_int, _ip, _vp, _vp, _vp]
+lib.fmm_set_persistent.argtypes = [C.c_int, C.c_int]
+lib.fmm_device_cus.argtypes = []
 lib.fmm_set_timing.argtypes = [C.c_int]
 lib.fmm_get_timing.argtypes = [_dp, C.c_int]
 lib.fmm_count_interactions.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
@@ -147,6 +151,45 @@ def make_tree_device(n: int, xy_ptr: int, cir_ptr: int, cor_ptr: int) -> None:
     _state["n"] = -1
     _check(lib.fmm_make_tree_device(int(n), xy_ptr, cir_ptr, cor_ptr), "make_tree_device")
     _state["n"] = int(n)
+
+
+def concat_systems(systems):
+    """[(xy, cir, cor), ...] -> (offsets int32 (S+1,), xy, cir, cor) concatenated."""
+    sizes = [np.asarray(s[1]).shape[0] for s in systems]
+    off = np.zeros(len(systems) + 1, np.int32)
+    off[1:] = np.cumsum(sizes)
+    xy = np.ascontiguousarray(np.concatenate([np.asarray(s[0], np.float64).reshape(-1, 2)
+                                              for s in systems]))
+    cir = np.ascontiguousarray(np.concatenate([np.asarray(s[1], np.float64) for s in systems]))
+    cor = np.ascontiguousarray(np.concatenate([np.asarray(s[2], np.float64) for s in systems]))
+    return off, xy, cir, cor
+
+
+def make_forest(systems) -> np.ndarray:
+    """One tree per system, built in one pass; returns the offsets used."""
+    off, xy, cir, cor = concat_systems(systems)
+    n = int(off[-1])
+    _state["n"] = -1
+    _check(lib.fmm_make_forest(len(systems), off, xy.ravel() if n else np.zeros(2),
+                               cir if n else np.zeros(1), cor if n else np.zeros(1)), "make_forest")
+    _state["n"] = n
+    return off
+
+
+def make_forest_device(offsets, xy_ptr: int, cir_ptr: int, cor_ptr: int) -> None:
+    off = np.ascontiguousarray(offsets, dtype=np.int32)
+    _state["n"] = -1
+    _check(lib.fmm_make_forest_device(len(off) - 1, off, xy_ptr, cir_ptr, cor_ptr),
+           "make_forest_device")
+    _state["n"] = int(off[-1])
+
+
+def set_persistent(blocks: int, graph: bool = False) -> None:
+    _check(lib.fmm_set_persistent(int(blocks), 1 if graph else 0), "set_persistent")
+
+
+def device_cus() -> int:
+    return int(lib.fmm_device_cus())
 
 
 def eval_sources() -> np.ndarray:

@@ -90,7 +90,8 @@ struct Context {
   double rcut = 0.0;
   int ntleaves = 0;  // non-empty leaves (targets of the fused kernel)
   DevBuf<float4> loc;
-  DevBuf<double> sinv2, smax;
+  DevBuf<double> smax;
+  DevBuf<double4> loc64;
   DevBuf<int32_t> leaves, lat;
   DevBuf<uint32_t> lflag, lpos;
   DevBuf<double2> direct_ws;  // partial sums of the direct judge
@@ -319,7 +320,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   if (g.t_mode != FMM2D_MODE_TREECODE) {
     // target leaves, leaf-relative FP32 sources, near-field radius
     if (g.leaves.ensure(n, s) || g.lat.ensure(n, s) || g.lflag.ensure(n, s) || g.lpos.ensure(n, s) ||
-        g.loc.ensure(n, s) || g.sinv2.ensure(n, s) || g.smax.ensure(1, s))
+        g.loc.ensure(n, s) || g.loc64.ensure(n, s) || g.smax.ensure(1, s))
       return -1;
     if (g.ntl_dev.ensure(1, s) || g.counter.ensure(1, s)) return -1;
     if (launch_leaf_list(g.nodes.p, g.nnodes, n, g.lat.p, g.lflag.p, g.lpos.p, g.leaves.p,
@@ -333,7 +334,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     g.ntleaves = (int)g.h_total[0];
     g.rcut = g.fmm_cut * smax;
     if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
-                          g.loc.p, g.sinv2.p, s))
+                          g.loc.p, g.loc64.p, s))
       return -1;
   }
   mark(4);
@@ -373,7 +374,7 @@ static FmmArgs fmm_args(double *d_uv) {
   a.sx = g.sx.p;
   a.sy = g.sy.p;
   a.sg = g.sg.p;
-  a.sinv2 = g.sinv2.p;
+  a.loc64 = g.loc64.p;
   a.loc = g.loc.p;
   a.leaves = g.leaves.p;
   a.nleaves = g.ntleaves;
@@ -1091,7 +1092,7 @@ void fmm_release(void) {
   g.total.release(); g.sx.release(); g.sy.release(); g.sg.release(); g.ss.release();
   g.bbox_part.release(); g.uv_dev.release(); g.tmp_in.release(); g.tmp_out.release();
   g.nodes.release(); g.gparent.release(); g.split.release(); g.coef.release(); g.root.release();
-  g.loc.release(); g.sinv2.release(); g.smax.release(); g.leaves.release(); g.lat.release();
+  g.loc.release(); g.loc64.release(); g.smax.release(); g.leaves.release(); g.lat.release();
   g.lflag.release(); g.lpos.release(); g.direct_ws.release();
   g.doff.release(); g.ntl_dev.release(); g.counter.release();
   if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);

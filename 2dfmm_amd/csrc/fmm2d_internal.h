@@ -123,8 +123,9 @@ struct FmmArgs {
   const NodeRec *nodes;
   const int32_t *gparent;
   const double2 *coef;  // unscaled a_k, p per node
-  const double *sx, *sy, *sg, *sinv2;
+  const double *sx, *sy, *sg;
   const float4 *loc;    // (x - c_leaf, y - c_leaf, cir, 1/sigma^2) in FP32
+  const double4 *loc64; // (x, y, cir, 1/sigma^2) in FP64
   const int32_t *leaves;
   int nleaves;
   const uint32_t *perm;
@@ -140,7 +141,7 @@ int launch_mfma_selftest(const double *a, const double *b, double *out, double *
                          hipStream_t s);
 int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleaves,
                       const double *sx, const double *sy, const double *sg, const double *ss,
-                      float4 *loc, double *sinv2, hipStream_t s);
+                      float4 *loc, double4 *loc64, hipStream_t s);
 // Non-empty leaves in blob (Morton) order; their number lands in *d_count.
 int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, uint32_t *d_flag,
                      uint32_t *d_pos, int32_t *d_leaves, uint32_t *d_count, hipStream_t s);

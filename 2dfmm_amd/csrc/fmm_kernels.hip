@@ -113,7 +113,8 @@ int launch_mfma_selftest(const double *a, const double *b, double *out, double *
 }
 
 // ---------------------------------------------------------------------------
-// per-source FP32 data of the mixed form: (x - c_leaf, y - c_leaf, cir, 1/sigma^2)
+// per-source data of the accurate modes: FP32 (x - c_leaf, y - c_leaf, cir,
+// 1/sigma^2) for the mixed form, FP64 (x, y, cir, 1/sigma^2) for the plain one
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void leaf_local_kernel(const NodeRec *__restrict__ nodes,
                                                          const int32_t *__restrict__ leaves,
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(256) void leaf_local_kernel(const NodeRec *__restri
                                                          const double *__restrict__ sg,
                                                          const double *__restrict__ ss,
                                                          float4 *__restrict__ loc,
-                                                         double *__restrict__ sinv2) {
+                                                         double4 *__restrict__ loc64) {
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (w >= nleaves) return;
   const NodeRec r = nodes[leaves[w]];
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(256) void leaf_local_kernel(const NodeRec *__restri
   for (int e = lane; e < c; e += 64) {
     const int j = r.first + e;
     const double s2 = ss[j] * ss[j];
-    sinv2[j] = 1.0 / s2;
+    loc64[j] = make_double4(sx[j], sy[j], sg[j], 1.0 / s2);
     loc[j] = make_float4((float)(sx[j] - r.cx), (float)(sy[j] - r.cy), (float)sg[j],
                          (float)(1.0 / s2));
   }
@@ -138,10 +139,10 @@ __global__ __launch_bounds__(256) void leaf_local_kernel(const NodeRec *__restri
 
 int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleaves,
                       const double *sx, const double *sy, const double *sg, const double *ss,
-                      float4 *loc, double *sinv2, hipStream_t s) {
+                      float4 *loc, double4 *loc64, hipStream_t s) {
   if (nleaves <= 0) return 0;
   leaf_local_kernel<<<(nleaves + 3) / 4, 256, 0, s>>>(d_nodes, d_leaves, nleaves, sx, sy, sg, ss,
-                                                      loc, sinv2);
+                                                      loc, loc64);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -332,29 +333,53 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
       const float dx = (float)(C.cx - NA.cx), dy = (float)(C.cy - NA.cy);
       const float px = lx - dx, py = ly - dy;
       float sxs = 0.f, sxc = 0.f, sys = 0.f, syc = 0.f;
-      for (int j = rep; j < cnt; j += nrep) {
-        const float4 s = a.loc[f + j];
+      // pair (target, source s): u += -f ry, v += f rx with
+      // f = cir (1 - exp(-r^2/sigma^2)) / r^2, pairs closer than 1e-10 skipped
+      auto pair = [&](const float4 s) {
         const float rx = px - s.x, ry = py - s.y;
         const float r2 = rx * rx + ry * ry;
-        const float fac = s.z * one_minus_exp_neg(r2 * s.w) * __frcp_rn(r2);
-        if (r2 > 1e-20f) {
-          kahan_add(sxs, sxc, -fac * ry);
-          kahan_add(sys, syc, fac * rx);
-        }
+        float fac = s.z * one_minus_exp_neg(r2 * s.w) * __builtin_amdgcn_rcpf(r2);
+        fac = r2 > 1e-20f ? fac : 0.0f;
+        kahan_add(sxs, sxc, -fac * ry);
+        kahan_add(sys, syc, fac * rx);
+      };
+      const float4 *src = a.loc + f;
+      int j = rep;
+      for (; j + 3 * nrep < cnt; j += 4 * nrep) {
+        const float4 s0 = src[j], s1 = src[j + nrep], s2 = src[j + 2 * nrep],
+                     s3 = src[j + 3 * nrep];
+        pair(s0);
+        pair(s1);
+        pair(s2);
+        pair(s3);
       }
+      for (; j < cnt; j += nrep) pair(src[j]);
       ax += (double)sxs - (double)sxc;
       ay += (double)sys - (double)syc;
     } else {
-      for (int j = rep; j < cnt; j += nrep) {
-        const int k = f + j;
-        const double rx = tx - a.sx[k], ry = ty - a.sy[k];
+      auto pair = [&](const double4 s) {
+        const double rx = tx - s.x, ry = ty - s.y;
         const double r2 = rx * rx + ry * ry;
-        if (r2 > 1e-20) {
-          const double fac = a.sg[k] * (1.0 - exp(-r2 * a.sinv2[k])) / r2;
-          ax -= fac * ry;
-          ay += fac * rx;
-        }
+        // 1 / r^2: hardware reciprocal refined by two Newton steps
+        double ri = __builtin_amdgcn_rcp(r2);
+        ri = ri * (2.0 - r2 * ri);
+        ri = ri * (2.0 - r2 * ri);
+        double fac = s.z * (1.0 - exp(-r2 * s.w)) * ri;
+        fac = r2 > 1e-20 ? fac : 0.0;
+        ax -= fac * ry;
+        ay += fac * rx;
+      };
+      const double4 *src = a.loc64 + f;
+      int j = rep;
+      for (; j + 3 * nrep < cnt; j += 4 * nrep) {
+        const double4 s0 = src[j], s1 = src[j + nrep], s2 = src[j + 2 * nrep],
+                      s3 = src[j + 3 * nrep];
+        pair(s0);
+        pair(s1);
+        pair(s2);
+        pair(s3);
       }
+      for (; j < cnt; j += nrep) pair(src[j]);
     }
   };
 
