@@ -67,6 +67,17 @@ lib.fmm_make_forest.argtypes = [C.c_int, _ip, _dp, _dp, _dp]
 lib.fmm_make_forest_device.argtypes = [C.c_int, _ip, _vp, _vp, _vp]
 lib.fmm_set_persistent.argtypes = [C.c_int, C.c_int]
 lib.fmm_device_cus.argtypes = []
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+lib.fmm_set_learned.argtypes = [C.c_void_p, C.c_float, C.c_float]
+lib.fmm_num_target_leaves.argtypes = []
+lib.fmm_leaf_features.argtypes = [_fp]
+lib.fmm_blob_leaf.argtypes = [_ip]
+lib.fmm_leaf_params.argtypes = [_fp]
+lib.fmm_set_leaf_params.argtypes = [C.c_void_p, C.c_int]
+lib.fmm_mlp_forward.argtypes = [C.c_int, _fp, _fp, _fp]
+lib.fmm_mlp_grad.argtypes = [C.c_int, _fp, _fp, _fp, _fp, C.POINTER(C.c_float)]
+lib.fmm_mlp_train.argtypes = [C.c_int, _fp, _fp, _fp, C.c_int, C.c_int, C.c_float, C.c_uint, _fp]
+MLP_WEIGHTS = 4866
 lib.fmm_set_timing.argtypes = [C.c_int]
 lib.fmm_get_timing.argtypes = [_dp, C.c_int]
 lib.fmm_count_interactions.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
@@ -182,6 +193,83 @@ def make_forest_device(offsets, xy_ptr: int, cir_ptr: int, cor_ptr: int) -> None
     _check(lib.fmm_make_forest_device(len(off) - 1, off, xy_ptr, cir_ptr, cor_ptr),
            "make_forest_device")
     _state["n"] = int(off[-1])
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def set_learned(weights, p_margin: float = 0.0, cut_margin: float = 0.0) -> None:
+    """Per-leaf p and near radius from the MLP (weights: 4866 floats), or None to stop."""
+    if weights is None:
+        _check(lib.fmm_set_learned(None, 0.0, 0.0), "set_learned")
+        return
+    w = _f32(weights).ravel()
+    if w.shape[0] != MLP_WEIGHTS:
+        raise ValueError(f"need {MLP_WEIGHTS} weights")
+    _check(lib.fmm_set_learned(w.ctypes.data, float(p_margin), float(cut_margin)), "set_learned")
+
+
+def num_target_leaves() -> int:
+    k = lib.fmm_num_target_leaves()
+    if k < 0:
+        raise FmmError("num_target_leaves: " + last_error())
+    return k
+
+
+def leaf_features() -> np.ndarray:
+    out = np.zeros((max(num_target_leaves(), 1), 8), np.float32)
+    _check(lib.fmm_leaf_features(out.ravel()), "leaf_features")
+    return out[:num_target_leaves()]
+
+
+def blob_leaf() -> np.ndarray:
+    out = np.zeros(max(_state["n"], 1), np.int32)
+    _check(lib.fmm_blob_leaf(out), "blob_leaf")
+    return out[:_state["n"]]
+
+
+def leaf_params() -> np.ndarray:
+    out = np.zeros((max(num_target_leaves(), 1), 2), np.float32)
+    _check(lib.fmm_leaf_params(out.ravel()), "leaf_params")
+    return out[:num_target_leaves()]
+
+
+def set_leaf_params(par) -> None:
+    if par is None:
+        _check(lib.fmm_set_leaf_params(None, 0), "set_leaf_params")
+        return
+    p = _f32(par).reshape(-1, 2)
+    _check(lib.fmm_set_leaf_params(p.ctypes.data, p.shape[0]), "set_leaf_params")
+
+
+def mlp_forward(x, weights) -> np.ndarray:
+    x = _f32(x).reshape(-1, 8)
+    out = np.zeros((max(x.shape[0], 1), 2), np.float32)
+    if x.shape[0]:
+        _check(lib.fmm_mlp_forward(x.shape[0], x.ravel(), _f32(weights).ravel(), out.ravel()),
+               "mlp_forward")
+    return out[:x.shape[0]]
+
+
+def mlp_grad(x, y, weights):
+    x = _f32(x).reshape(-1, 8)
+    y = _f32(y).reshape(-1, 2)
+    g = np.zeros(MLP_WEIGHTS, np.float32)
+    loss = C.c_float()
+    _check(lib.fmm_mlp_grad(x.shape[0], x.ravel(), y.ravel(), _f32(weights).ravel(), g,
+                            C.byref(loss)), "mlp_grad")
+    return g, loss.value
+
+
+def mlp_train(x, y, weights, epochs=100, batch=1024, lr=1e-3, seed=1):
+    x = _f32(x).reshape(-1, 8)
+    y = _f32(y).reshape(-1, 2)
+    w = _f32(weights).ravel().copy()
+    hist = np.zeros(int(epochs), np.float32)
+    _check(lib.fmm_mlp_train(x.shape[0], x.ravel(), y.ravel(), w, int(epochs), int(batch),
+                             float(lr), int(seed), hist), "mlp_train")
+    return w, hist
 
 
 def set_persistent(blocks: int, graph: bool = False) -> None:

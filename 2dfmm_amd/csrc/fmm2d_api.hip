@@ -95,6 +95,14 @@ struct Context {
   DevBuf<int32_t> leaves, lat;
   DevBuf<uint32_t> lflag, lpos;
   DevBuf<double2> direct_ws;  // partial sums of the direct judge
+  double smax_h = 0.0;        // largest core radius of the tree's blobs
+  // learned per-leaf parameters
+  bool learned = false;
+  float p_margin = 0.f, cut_margin = 0.f;
+  DevBuf<float> mlp_w, feat, mlp_out;
+  DevBuf<double> mean_abs;
+  DevBuf<float2> leaf_par;   // learned (or explicitly set) per-leaf (p, cut)
+  bool leaf_par_on = false;
   // forests (batched systems)
   int nsys = 1, root_shift = -1;
   DevBuf<int32_t> doff;
@@ -197,6 +205,8 @@ static int resolve_params() {
   return read_param_file();
 }
 
+static int apply_learned();
+
 // The tree build of make_tree on device inputs (caller order).  With nsys > 1
 // the inputs are nsys independent systems, system s owning blobs
 // [h_off[s], h_off[s+1]); each gets its own tree (a forest) in the same pass.
@@ -210,6 +220,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   g.n = n;
   g.nsys = nsys;
   g.root_shift = -1;
+  g.leaf_par_on = false;
   g.t_mode = g.mode;
   if (g.mode == FMM2D_MODE_TREECODE) {
     g.t_nterms = g.nterms;
@@ -332,10 +343,17 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     FMM_HIP_TRY(hipMemcpyAsync(&smax, g.smax.p, sizeof(double), hipMemcpyDeviceToHost, s));
     FMM_HIP_TRY(hipStreamSynchronize(s));
     g.ntleaves = (int)g.h_total[0];
+    g.smax_h = smax;
     g.rcut = g.fmm_cut * smax;
     if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
                           g.loc.p, g.loc64.p, s))
       return -1;
+    // valid already for the fused kernel's arguments used by the learned step
+    g.valid = true;
+    if (g.learned && apply_learned()) {
+      g.valid = false;
+      return -1;
+    }
   }
   mark(4);
   g.valid = true;
@@ -385,7 +403,27 @@ static FmmArgs fmm_args(double *d_uv) {
   a.rcut = g.rcut;
   a.key = g.key.p;
   a.root_shift = g.root_shift;
+  a.leaf_par = g.leaf_par_on ? g.leaf_par.p : nullptr;
+  a.smax = g.smax_h;
   return a;
+}
+
+// Learned per-leaf (p, cut) for the current tree: features -> MLP -> params.
+static int apply_learned() {
+  hipStream_t s = g.stream;
+  const int nl = g.ntleaves;
+  if (nl <= 0) return 0;
+  if (g.feat.ensure((size_t)nl * 8, s) || g.mlp_out.ensure((size_t)nl * 2, s) ||
+      g.leaf_par.ensure(nl, s) || g.mean_abs.ensure(1, s))
+    return -1;
+  FmmArgs a = fmm_args(nullptr);
+  if (launch_leaf_features(a, g.t_num_p, g.sg.p, g.n, g.mean_abs.p, g.feat.p, s)) return -1;
+  if (launch_mlp_forward(g.mlp_w.p, nl, g.feat.p, g.mlp_out.p, s)) return -1;
+  if (launch_leaf_params(g.mlp_out.p, nl, g.p_margin, g.cut_margin, g.t_nterms, (float)g.fmm_cut,
+                         g.leaf_par.p, s))
+    return -1;
+  g.leaf_par_on = true;
+  return 0;
 }
 
 // Persistent evaluation through a captured graph [reset counter, persistent
@@ -803,6 +841,249 @@ int fmm_set_persistent(int blocks, int use_graph) {
   g.persist_blocks = blocks;
   g.use_graph = use_graph != 0 && blocks > 0;
   return 0;
+}
+
+// ---- learned per-leaf parameters -------------------------------------------
+int fmm_set_learned(const float *weights, float p_margin, float cut_margin) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  if (!weights) {
+    g.learned = false;
+    return 0;
+  }
+  if (g.mlp_w.ensure(kMlpWeights, g.stream)) return -1;
+  FMM_HIP_TRY(hipMemcpyAsync(g.mlp_w.p, weights, sizeof(float) * kMlpWeights,
+                             hipMemcpyHostToDevice, g.stream));
+  FMM_HIP_TRY(hipStreamSynchronize(g.stream));
+  g.learned = true;
+  g.p_margin = p_margin;
+  g.cut_margin = cut_margin;
+  return 0;
+}
+
+static int need_fmm_tree() {
+  if (need_tree()) return -1;
+  if (g.t_mode == FMM2D_MODE_TREECODE) {
+    set_error("needs a tree built in an accurate mode");
+    return -1;
+  }
+  return 0;
+}
+
+int fmm_num_target_leaves(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_fmm_tree()) return -1;
+  return g.ntleaves;
+}
+
+int fmm_leaf_features(float *feat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_fmm_tree()) return -1;
+  const int nl = g.ntleaves;
+  if (nl <= 0) return 0;
+  hipStream_t s = g.stream;
+  if (g.feat.ensure((size_t)nl * 8, s) || g.mean_abs.ensure(1, s)) return -1;
+  if (launch_leaf_features(fmm_args(nullptr), g.t_num_p, g.sg.p, g.n, g.mean_abs.p, g.feat.p, s))
+    return -1;
+  FMM_HIP_TRY(hipMemcpyAsync(feat, g.feat.p, sizeof(float) * nl * 8, hipMemcpyDeviceToHost, s));
+  FMM_HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fmm_blob_leaf(int32_t *leaf_of_blob) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_fmm_tree()) return -1;
+  if (g.n == 0) return 0;
+  hipStream_t s = g.stream;
+  DevBuf<int32_t> d;
+  if (d.ensure(g.n, s)) return -1;
+  int rc = launch_blob_leaf(fmm_args(nullptr), d.p, s);
+  if (!rc && (hipMemcpyAsync(leaf_of_blob, d.p, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost, s) !=
+                  hipSuccess ||
+              hipStreamSynchronize(s) != hipSuccess)) {
+    set_error("download failed");
+    rc = -1;
+  }
+  d.release();
+  return rc;
+}
+
+int fmm_leaf_params(float *par) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_fmm_tree()) return -1;
+  const int nl = g.ntleaves;
+  if (!g.leaf_par_on) {
+    for (int i = 0; i < nl; i++) {
+      par[2 * i] = (float)g.t_nterms;
+      par[2 * i + 1] = (float)g.fmm_cut;
+    }
+    return 0;
+  }
+  FMM_HIP_TRY(hipMemcpyAsync(par, g.leaf_par.p, sizeof(float) * 2 * nl, hipMemcpyDeviceToHost,
+                             g.stream));
+  FMM_HIP_TRY(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int fmm_set_leaf_params(const float *par, int n) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_fmm_tree()) return -1;
+  if (!par) {
+    g.leaf_par_on = false;
+    return 0;
+  }
+  if (n != g.ntleaves) {
+    set_error("fmm_set_leaf_params: %d entries for %d target leaves", n, g.ntleaves);
+    return -1;
+  }
+  if (g.leaf_par.ensure(std::max(n, 1), g.stream)) return -1;
+  FMM_HIP_TRY(hipMemcpyAsync(g.leaf_par.p, par, sizeof(float) * 2 * n, hipMemcpyHostToDevice,
+                             g.stream));
+  FMM_HIP_TRY(hipStreamSynchronize(g.stream));
+  g.leaf_par_on = true;
+  return 0;
+}
+
+int fmm_mlp_forward(int n, const float *x, const float *weights, float *out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  if (n <= 0) return 0;
+  hipStream_t s = g.stream;
+  DevBuf<float> dw, dx, dout;
+  int rc = 0;
+  if (dw.ensure(kMlpWeights, s) || dx.ensure((size_t)n * 8, s) || dout.ensure((size_t)n * 2, s))
+    rc = -1;
+  if (!rc && (hipMemcpyAsync(dw.p, weights, sizeof(float) * kMlpWeights, hipMemcpyHostToDevice, s) !=
+                  hipSuccess ||
+              hipMemcpyAsync(dx.p, x, sizeof(float) * 8 * n, hipMemcpyHostToDevice, s) != hipSuccess))
+    rc = -1;
+  if (!rc) rc = launch_mlp_forward(dw.p, n, dx.p, dout.p, s);
+  if (!rc && (hipMemcpyAsync(out, dout.p, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s) !=
+                  hipSuccess ||
+              hipStreamSynchronize(s) != hipSuccess))
+    rc = -1;
+  (void)hipStreamSynchronize(s);
+  dw.release(); dx.release(); dout.release();
+  if (rc && !g_err[0]) set_error("mlp forward failed");
+  return rc;
+}
+
+// gradient of mean((out - y)^2) over the n samples; *loss gets that mean
+int fmm_mlp_grad(int n, const float *x, const float *y, const float *weights, float *grad,
+                 float *loss) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  if (n <= 0) {
+    set_error("n must be > 0");
+    return -1;
+  }
+  hipStream_t s = g.stream;
+  DevBuf<float> dw, dx, dy, scr, dg, dl;
+  int rc = 0;
+  if (dw.ensure(kMlpWeights, s) || dx.ensure((size_t)n * 8, s) || dy.ensure((size_t)n * 2, s) ||
+      scr.ensure(mlp_scratch_floats(n), s) || dg.ensure(kMlpWeights, s) || dl.ensure(n, s))
+    rc = -1;
+  if (!rc && (hipMemcpyAsync(dw.p, weights, sizeof(float) * kMlpWeights, hipMemcpyHostToDevice, s) !=
+                  hipSuccess ||
+              hipMemcpyAsync(dx.p, x, sizeof(float) * 8 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+              hipMemcpyAsync(dy.p, y, sizeof(float) * 2 * n, hipMemcpyHostToDevice, s) != hipSuccess))
+    rc = -1;
+  if (!rc) rc = launch_mlp_grad(dw.p, n, dx.p, dy.p, scr.p, dg.p, dl.p, s);
+  std::vector<float> hl(n);
+  if (!rc && (hipMemcpyAsync(grad, dg.p, sizeof(float) * kMlpWeights, hipMemcpyDeviceToHost, s) !=
+                  hipSuccess ||
+              hipMemcpyAsync(hl.data(), dl.p, sizeof(float) * n, hipMemcpyDeviceToHost, s) !=
+                  hipSuccess ||
+              hipStreamSynchronize(s) != hipSuccess))
+    rc = -1;
+  (void)hipStreamSynchronize(s);
+  dw.release(); dx.release(); dy.release(); scr.release(); dg.release(); dl.release();
+  if (rc) {
+    if (!g_err[0]) set_error("mlp gradient failed");
+    return rc;
+  }
+  double l = 0.0;
+  for (int i = 0; i < n; i++) l += hl[i];
+  *loss = (float)l;
+  return 0;
+}
+
+// Adam on minibatches of a fixed seeded shuffle per epoch; weights in/out;
+// loss_hist[e] = mean loss of the minibatches of epoch e.
+int fmm_mlp_train(int n, const float *x, const float *y, float *weights, int epochs, int batch,
+                  float lr, unsigned seed, float *loss_hist) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  if (n <= 0 || epochs < 1 || batch < 1) {
+    set_error("fmm_mlp_train: n, epochs, batch must be positive");
+    return -1;
+  }
+  batch = std::min(batch, n);
+  hipStream_t s = g.stream;
+  DevBuf<float> dw, dx, dy, scr, dg, dm, dv, dl;
+  int rc = 0;
+  if (dw.ensure(kMlpWeights, s) || dx.ensure((size_t)n * 8, s) || dy.ensure((size_t)n * 2, s) ||
+      scr.ensure(mlp_scratch_floats(batch), s) || dg.ensure(kMlpWeights, s) ||
+      dm.ensure(kMlpWeights, s) || dv.ensure(kMlpWeights, s) || dl.ensure(batch, s))
+    rc = -1;
+  std::vector<float> hx((size_t)n * 8), hy((size_t)n * 2), hl(batch);
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  if (!rc && (hipMemcpyAsync(dw.p, weights, sizeof(float) * kMlpWeights, hipMemcpyHostToDevice, s) !=
+                  hipSuccess ||
+              hipMemsetAsync(dm.p, 0, sizeof(float) * kMlpWeights, s) != hipSuccess ||
+              hipMemsetAsync(dv.p, 0, sizeof(float) * kMlpWeights, s) != hipSuccess))
+    rc = -1;
+  uint64_t st = 0x9E3779B97F4A7C15ull ^ seed;
+  int t = 0;
+  for (int e = 0; e < epochs && !rc; e++) {
+    for (int i = n - 1; i > 0; i--) {  // Fisher-Yates with splitmix64
+      st += 0x9E3779B97F4A7C15ull;
+      uint64_t z = st;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      std::swap(order[i], order[(size_t)(z % (uint64_t)(i + 1))]);
+    }
+    for (int i = 0; i < n; i++) {
+      memcpy(&hx[(size_t)i * 8], &x[(size_t)order[i] * 8], sizeof(float) * 8);
+      memcpy(&hy[(size_t)i * 2], &y[(size_t)order[i] * 2], sizeof(float) * 2);
+    }
+    if (hipMemcpyAsync(dx.p, hx.data(), sizeof(float) * 8 * n, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(dy.p, hy.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, s) !=
+            hipSuccess) {
+      rc = -1;
+      break;
+    }
+    double el = 0.0;
+    int nb = 0;
+    for (int b0 = 0; b0 + batch <= n && !rc; b0 += batch) {
+      rc = launch_mlp_grad(dw.p, batch, dx.p + (size_t)b0 * 8, dy.p + (size_t)b0 * 2, scr.p, dg.p,
+                           dl.p, s);
+      if (!rc) rc = launch_adam(dw.p, dg.p, dm.p, dv.p, lr, ++t, s);
+      if (!rc && (hipMemcpyAsync(hl.data(), dl.p, sizeof(float) * batch, hipMemcpyDeviceToHost, s) !=
+                      hipSuccess ||
+                  hipStreamSynchronize(s) != hipSuccess))
+        rc = -1;
+      if (!rc) {
+        double l = 0.0;
+        for (int k = 0; k < batch; k++) l += hl[k];
+        el += l;
+        nb++;
+      }
+    }
+    if (loss_hist) loss_hist[e] = nb ? (float)(el / nb) : 0.f;
+  }
+  if (!rc && (hipMemcpyAsync(weights, dw.p, sizeof(float) * kMlpWeights, hipMemcpyDeviceToHost, s) !=
+                  hipSuccess ||
+              hipStreamSynchronize(s) != hipSuccess))
+    rc = -1;
+  (void)hipStreamSynchronize(s);
+  dw.release(); dx.release(); dy.release(); scr.release(); dg.release(); dm.release();
+  dv.release(); dl.release();
+  if (rc && !g_err[0]) set_error("mlp training failed");
+  return rc;
 }
 
 int fmm_device_cus(void) {

@@ -136,6 +136,10 @@ struct FmmArgs {
   // root_shift < 0 for a single tree (root 0)
   const uint64_t *key;
   int root_shift;
+  // per target leaf (index in `leaves`): (terms, near radius in core radii);
+  // null = p and rcut for every leaf.  smax = largest core radius.
+  const float2 *leaf_par;
+  double smax;
 };
 int launch_mfma_selftest(const double *a, const double *b, double *out, double *ref, int fp32,
                          hipStream_t s);
@@ -147,6 +151,20 @@ int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, u
                      uint32_t *d_pos, int32_t *d_leaves, uint32_t *d_count, hipStream_t s);
 int launch_max_reduce(const double *d_v, int n, double *d_out, hipStream_t s);
 int launch_fmm_eval(const FmmArgs &args, bool mixed, hipStream_t s);
+
+// ---- learned per-leaf parameters (learned_kernels.hip) --------------------
+constexpr int kMlpWeights = 4866;  // 8 -> 64 -> 64 -> 2
+int launch_leaf_features(const FmmArgs &a, int leaf_cap, const double *d_cir, int n,
+                         double *d_mean_abs, float *d_feat, hipStream_t s);
+int launch_mlp_forward(const float *d_w, int n, const float *d_x, float *d_out, hipStream_t s);
+size_t mlp_scratch_floats(int n);
+int launch_mlp_grad(const float *d_w, int n, const float *d_x, const float *d_y, float *d_scratch,
+                    float *d_grad, float *d_loss, hipStream_t s);
+int launch_adam(float *d_w, const float *d_g, float *d_m, float *d_v, float lr, int t,
+                hipStream_t s);
+int launch_leaf_params(const float *d_y, int n, float p_margin, float cut_margin, int p_max,
+                       float cut_fallback, float2 *d_par, hipStream_t s);
+int launch_blob_leaf(const FmmArgs &a, int32_t *d_out, hipStream_t s);
 // Persistent grid of nblocks x 256 threads over a work counter; the number of
 // target leaves is read from *d_nleaves (graph-capturable).
 int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,

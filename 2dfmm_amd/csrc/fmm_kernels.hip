@@ -282,6 +282,14 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
   const int cA = node_count(NA.cnt);
   const double RA = NA.region * 0.70710678118654752;
   const double sA = NA.region;
+  // this leaf's expansion order and near-field radius
+  int pA = a.p;
+  double rcut = a.rcut;
+  if (a.leaf_par) {
+    const float2 lp = a.leaf_par[li];
+    pA = min(a.p, max(2, (int)lp.x));
+    rcut = (double)lp.y * a.smax;
+  }
   // lanes: (target ti, replica rep)
   const int nrep = max(1, 64 / cA);
   const int ti = lane % cA, rep = lane / cA;
@@ -297,7 +305,7 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
   double ax = 0.0, ay = 0.0;
   // MFMA lane roles: row / column index i, K slot k
   const int mi = lane & 15, mk = lane >> 4;
-  const bool row_ok = mi < a.p;
+  const bool row_ok = mi < pA;
   const double2 sApow = cpow_u(make_double2(sA, 0.0), mi + 1);  // s_A^(i+1), real
   FarAcc<MIXED> acc;
   int pending = -1;
@@ -398,7 +406,7 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
         const double dx = C.cx - NA.cx, dy = C.cy - NA.cy;
         const double dd = sqrt(dx * dx + dy * dy);
         const double RC = C.region * 0.70710678118654752;
-        if (RA + RC <= a.theta * dd && dd - RA - RC >= a.rcut) {
+        if (RA + RC <= a.theta * dd && dd - RA - RC >= rcut) {
           if (pending < 0) {
             pending = c;
           } else {
@@ -444,8 +452,8 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
     // C(row + mi, mi)
     double bc = 1.0;
     for (int t = 1; t <= mi; t++) bc = bc * (double)(row + t) / (double)t;
-    double gr = (mi < a.p) ? acc.r0(r) * bc : 0.0;
-    double gi = (mi < a.p) ? acc.r1(r) * bc : 0.0;
+    double gr = (mi < pA) ? acc.r0(r) * bc : 0.0;
+    double gi = (mi < pA) ? acc.r1(r) * bc : 0.0;
 #pragma unroll
     for (int o = 8; o > 0; o >>= 1) {
       gr += __shfl_xor(gr, o, 64);
@@ -461,8 +469,8 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
   if (tact && rep == 0) {
     // L2P: sum_l Lt_l u^l, u = (z - c_A)/s_A
     const double ur = (tx - NA.cx) / sA, ui = (ty - NA.cy) / sA;
-    double2 s = s_loc[w][a.p - 1];
-    for (int l = a.p - 2; l >= 0; l--) {
+    double2 s = s_loc[w][pA - 1];
+    for (int l = pA - 2; l >= 0; l--) {
       const double2 L = s_loc[w][l];
       s = make_double2(s.x * ur - s.y * ui + L.x, s.x * ui + s.y * ur + L.y);
     }
@@ -543,6 +551,7 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
   const NodeRec NA = a.nodes[a.leaves[li]];
   const int cA = node_count(NA.cnt);
   const double RA = NA.region * 0.70710678118654752;
+  const double rcut = a.leaf_par ? (double)a.leaf_par[li].y * a.smax : a.rcut;
   int64_t far = 0, pairs = 0;
   Slots2 sl;
   int d = 0, cur = 0, node = root_of(a, NA.first);
@@ -557,7 +566,7 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
         const double dx = C.cx - NA.cx, dy = C.cy - NA.cy;
         const double dd = sqrt(dx * dx + dy * dy);
         const double RC = C.region * 0.70710678118654752;
-        if (RA + RC <= a.theta * dd && dd - RA - RC >= a.rcut) far++;
+        if (RA + RC <= a.theta * dd && dd - RA - RC >= rcut) far++;
         else m |= 1u << q;
       }
       d++;

@@ -112,6 +112,33 @@ int fmm_set_persistent(int blocks, int use_graph);
 /* Compute units of the current device (for sizing persistent grids). */
 int fmm_device_cus(void);
 
+/* Learned per-leaf parameters (no reference counterpart).  In the accurate
+ * modes each target leaf may use its own expansion order p (<= the set p) and
+ * near-field radius (in core radii).  With fmm_set_learned(w, ...) every
+ * build computes 8 features per target leaf, runs the FP32 MLP 8->64->64->2
+ * with weights w (4866 floats: W1[64][8] b1[64] W2[64][64] b2[64] W3[2][64]
+ * b3[2], ReLU) and takes p = ceil(11 + 5 y0 + p_margin), cut = 3 + 2 y1 +
+ * cut_margin (clamped to [4, p] and [1, 6]); a non-finite output falls back to
+ * the fixed p and cut.  w = NULL switches it off. */
+int fmm_set_learned(const float *weights, float p_margin, float cut_margin);
+/* The current tree's target leaves (accurate modes): count; features
+ * (count x 8 floats); the target-leaf index of every blob (caller order); the
+ * per-leaf (p, cut) in force (count x 2); or set them explicitly (until the
+ * next build; NULL clears). */
+int fmm_num_target_leaves(void);
+int fmm_leaf_features(float *feat);
+int fmm_blob_leaf(int32_t *leaf_of_blob);
+int fmm_leaf_params(float *par);
+int fmm_set_leaf_params(const float *par, int n);
+/* The MLP itself, on the GPU: forward pass of n samples (x: n x 8, out: n x 2);
+ * gradient of mean((out - y)^2) (grad: 4866 floats, *loss the mean); Adam
+ * training on minibatches (weights updated in place, loss_hist[epochs]). */
+int fmm_mlp_forward(int n, const float *x, const float *weights, float *out);
+int fmm_mlp_grad(int n, const float *x, const float *y, const float *weights, float *grad,
+                 float *loss);
+int fmm_mlp_train(int n, const float *x, const float *y, float *weights, int epochs, int batch,
+                  float lr, unsigned seed, float *loss_hist);
+
 /* Velocities at all blobs of the last tree, in the caller's blob order,
  * interleaved u0,v0,u1,v1,...  Host or device output buffer. */
 int fmm_eval_sources(double *uv);

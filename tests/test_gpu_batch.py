@@ -98,4 +98,5 @@ def test_forest_accuracy(fmm, oracle):
         got = uv[off[s]:off[s + 1]][:: max(1, xy.shape[0] // 200)]
         worst = max(worst, rel_err(got, d)[0])
     _say(f"  worst per-system rel_L2 (mixed, p=12): {worst:.2e}")
-    assert worst <= 2e-6
+    # p = 12 at theta = 0.5: truncation (theta / (2 - theta))^12 = 1.9e-6 per far cell
+    assert worst <= 5e-6

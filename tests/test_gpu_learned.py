@@ -1,0 +1,128 @@
+"""Learned form: the hand-written MLP kernels against a numpy MLP and central
+finite differences, training on the GPU, and the per-leaf parameter path of
+the accurate modes (fixed-order fallback, bitwise equality with fixed
+parameters when the network predicts them)."""
+import sys
+
+import numpy as np
+import pytest
+
+import mlp_ref
+from conftest import fmm_module, rel_err, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def fmm():
+    m = fmm_module()
+    yield m
+    m.set_learned(None)
+    m.set_mode(m.MODE_TREECODE)
+    m.set_fmm_params()
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, 8)).astype(np.float32)
+    y = np.stack([np.tanh(x[:, 0] - 0.5 * x[:, 3]), 0.3 * x[:, 1] * x[:, 2]], 1).astype(np.float32)
+    return x, y
+
+
+def test_mlp_forward_matches_numpy(fmm):
+    w = mlp_ref.init(1)
+    x, _ = _data(1000, 2)
+    got = fmm.mlp_forward(x, w)
+    want, _ = mlp_ref.forward(w, x)
+    assert np.abs(got - want).max() <= 1e-5 * max(1.0, np.abs(want).max())
+
+
+def test_mlp_gradient_matches_numpy_and_finite_differences(fmm):
+    w = mlp_ref.init(3)
+    w[mlp_ref.NW - 66:] += 0.1  # non-zero biases downstream
+    x, y = _data(257, 4)
+    g, loss = fmm.mlp_grad(x, y, w)
+    gref = mlp_ref.grad(w, x, y)
+    assert abs(loss - mlp_ref.loss(w, x, y)) <= 1e-5 * mlp_ref.loss(w, x, y)
+    scale = np.abs(gref).max()
+    assert np.abs(g - gref).max() <= 1e-4 * scale
+    # central differences of the FP64 reference loss pin the reference gradient
+    rng = np.random.default_rng(5)
+    idx = rng.choice(mlp_ref.NW, 60, replace=False)
+    w64 = w.astype(np.float64)
+    h = 1e-6
+    for k in idx:
+        wp, wm = w64.copy(), w64.copy()
+        wp[k] += h
+        wm[k] -= h
+        fd = (mlp_ref.loss(wp, x, y) - mlp_ref.loss(wm, x, y)) / (2 * h)
+        assert abs(fd - gref[k]) <= 1e-6 * scale + 1e-4 * abs(fd), k
+        assert abs(fd - g[k]) <= 1e-4 * scale + 1e-3 * abs(fd), k
+
+
+def test_mlp_training_converges(fmm):
+    x, y = _data(8192, 6)
+    w0 = mlp_ref.init(7)
+    l0 = mlp_ref.loss(w0, x, y)
+    w, hist = fmm.mlp_train(x, y, w0, epochs=60, batch=512, lr=3e-3, seed=8)
+    l1 = mlp_ref.loss(w, x, y)
+    _say(f"  training loss {l0:.4f} -> {l1:.5f} (last epoch {hist[-1]:.5f})")
+    assert l1 < 0.1 * l0
+    assert hist[-1] < hist[0]
+
+
+def _tree(fmm, n=60000, seed=71):
+    xy, cir, cor = workloads().gaussian_mixture(n, seed=seed)
+    fmm.set_mode(fmm.MODE_MIXED)
+    fmm.set_fmm_params(16, 64, 0.5, 4.0)
+    return xy, cir, cor
+
+
+def test_leaf_params_plumbing(fmm):
+    xy, cir, cor = _tree(fmm)
+    fmm.set_learned(None)
+    fmm.make_tree(xy, cir, cor)
+    ref = fmm.eval_sources()
+    nl = fmm.num_target_leaves()
+    feat = fmm.leaf_features()
+    assert feat.shape == (nl, 8) and np.isfinite(feat).all()
+    bl = fmm.blob_leaf()
+    assert bl.min() == 0 and bl.max() == nl - 1
+    # explicit per-leaf parameters equal to the fixed ones: bitwise the same
+    fmm.set_leaf_params(np.tile(np.array([16.0, 4.0], np.float32), (nl, 1)))
+    assert np.array_equal(fmm.eval_sources().view(np.uint64), ref.view(np.uint64))
+    # lower order in every other leaf changes those leaves' blobs only
+    par = np.tile(np.array([16.0, 4.0], np.float32), (nl, 1))
+    par[::2, 0] = 6.0
+    fmm.set_leaf_params(par)
+    got = fmm.eval_sources()
+    changed = np.any(got != ref, axis=1)
+    assert not changed[bl % 2 == 1].any()
+    assert changed[bl % 2 == 0].mean() > 0.5
+    fmm.set_leaf_params(None)
+
+
+def test_learned_constant_network_and_fallback(fmm):
+    """A network that predicts (p, cut) = (16, 4) everywhere reproduces the
+    fixed run bit for bit; a network producing NaN falls back to it."""
+    xy, cir, cor = _tree(fmm)
+    fmm.set_learned(None)
+    fmm.make_tree(xy, cir, cor)
+    ref = fmm.eval_sources()
+    w = np.zeros(mlp_ref.NW, np.float32)
+    w[-2:] = [1.0, 0.5]  # y0 = 1 -> p = 16, y1 = 0.5 -> cut = 4
+    fmm.set_learned(w)
+    fmm.make_tree(xy, cir, cor)
+    assert np.array_equal(fmm.eval_sources().view(np.uint64), ref.view(np.uint64))
+    par = fmm.leaf_params()
+    assert (par[:, 0] == 16).all() and (par[:, 1] == 4).all()
+    w[-2:] = np.nan
+    fmm.set_learned(w)
+    fmm.make_tree(xy, cir, cor)
+    assert np.array_equal(fmm.eval_sources().view(np.uint64), ref.view(np.uint64))
+    fmm.set_learned(None)
