@@ -78,6 +78,11 @@ lib.fmm_mlp_forward.argtypes = [C.c_int, _fp, _fp, _fp]
 lib.fmm_mlp_grad.argtypes = [C.c_int, _fp, _fp, _fp, _fp, C.POINTER(C.c_float)]
 lib.fmm_mlp_train.argtypes = [C.c_int, _fp, _fp, _fp, C.c_int, C.c_int, C.c_float, C.c_uint, _fp]
 MLP_WEIGHTS = 4866
+lib.fmm_make_tree_mirror_device.argtypes = [C.c_int, _vp, _vp, _vp, C.c_double]
+lib.fmm_drift_ab3_device.argtypes = [C.c_int, _vp, _vp, C.c_double, C.c_int, C.c_double]
+lib.fmm_history_shift_device.argtypes = [C.c_int, _vp, C.c_int]
+lib.fmm_store_velocity_device.argtypes = [C.c_int, _vp, _vp, _vp, C.c_double, C.c_double, C.c_double]
+lib.fmm_core_spread_device.argtypes = [C.c_int, _vp, C.c_double, C.c_double]
 lib.fmm_set_timing.argtypes = [C.c_int]
 lib.fmm_get_timing.argtypes = [_dp, C.c_int]
 lib.fmm_count_interactions.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
@@ -193,6 +198,33 @@ def make_forest_device(offsets, xy_ptr: int, cir_ptr: int, cor_ptr: int) -> None
     _check(lib.fmm_make_forest_device(len(off) - 1, off, xy_ptr, cir_ptr, cor_ptr),
            "make_forest_device")
     _state["n"] = int(off[-1])
+
+
+def make_tree_mirror_device(n: int, xy_ptr: int, cir_ptr: int, cor_ptr: int, d_mirror: float):
+    """Tree over n blobs plus their mirror images about y = d_mirror."""
+    _state["n"] = -1
+    _check(lib.fmm_make_tree_mirror_device(int(n), xy_ptr, cir_ptr, cor_ptr, float(d_mirror)),
+           "make_tree_mirror_device")
+    _state["n"] = int(n)
+
+
+def drift_ab3_device(n, xy_ptr, hist_ptr, dt, mirror=False, d_mirror=0.0) -> None:
+    _check(lib.fmm_drift_ab3_device(int(n), xy_ptr, hist_ptr, float(dt), 1 if mirror else 0,
+                                    float(d_mirror)), "drift_ab3_device")
+
+
+def history_shift_device(n, hist_ptr, first_step=False) -> None:
+    _check(lib.fmm_history_shift_device(int(n), hist_ptr, 1 if first_step else 0),
+           "history_shift_device")
+
+
+def store_velocity_device(n, xy_ptr, uv_ptr, hist_ptr, uinf=0.0, vinf=0.0, omz=0.0) -> None:
+    _check(lib.fmm_store_velocity_device(int(n), xy_ptr, uv_ptr, hist_ptr, float(uinf), float(vinf),
+                                         float(omz)), "store_velocity_device")
+
+
+def core_spread_device(n, cor_ptr, dt, re) -> None:
+    _check(lib.fmm_core_spread_device(int(n), cor_ptr, float(dt), float(re)), "core_spread_device")
 
 
 def _f32(a) -> np.ndarray:

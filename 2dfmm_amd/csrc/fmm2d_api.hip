@@ -103,6 +103,9 @@ struct Context {
   DevBuf<double> mean_abs;
   DevBuf<float2> leaf_par;   // learned (or explicitly set) per-leaf (p, cut)
   bool leaf_par_on = false;
+  // mirror images: the tree holds n_real blobs and their images after them
+  int n_real = -1;  // < 0: no images
+  DevBuf<double> mxy, mcir, mcor, muv;
   // forests (batched systems)
   int nsys = 1, root_shift = -1;
   DevBuf<int32_t> doff;
@@ -136,6 +139,7 @@ struct Context {
   // 0 start, 1 keys, 2 sorted, 3 tree, 4 expansions | 5, 6 around the last evaluation
   bool timing = false;
   hipEvent_t tev[7] = {};
+  bool tev_rec[7] = {};  // recorded since timing was switched on
 };
 
 static Context g;
@@ -158,7 +162,7 @@ static int ctx_init() {
 }
 
 static inline void mark(int i) {
-  if (g.timing) (void)hipEventRecord(g.tev[i], g.stream);
+  if (g.timing && hipEventRecord(g.tev[i], g.stream) == hipSuccess) g.tev_rec[i] = true;
 }
 
 // quadtree.c:12-27 (the same fscanf formats, so "0.5d0" reads as 0.5 and the
@@ -221,6 +225,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   g.nsys = nsys;
   g.root_shift = -1;
   g.leaf_par_on = false;
+  g.n_real = -1;
   g.t_mode = g.mode;
   if (g.mode == FMM2D_MODE_TREECODE) {
     g.t_nterms = g.nterms;
@@ -464,6 +469,18 @@ static int eval_fmm_graph(const FmmArgs &a, bool mixed) {
 static int eval_sources_dev(double *d_uv) {
   if (need_tree()) return -1;
   if (g.n == 0) return 0;
+  if (g.n_real >= 0) {
+    // mirror images ride along as sources: evaluate all, return the real blobs
+    if (g.muv.ensure(2 * (size_t)g.n, g.stream)) return -1;
+    const int nr = g.n_real;
+    g.n_real = -1;
+    const int rc = eval_sources_dev(g.muv.p);
+    g.n_real = nr;
+    if (rc) return rc;
+    FMM_HIP_TRY(hipMemcpyAsync(d_uv, g.muv.p, sizeof(double) * 2 * nr, hipMemcpyDeviceToDevice,
+                               g.stream));
+    return 0;
+  }
   mark(5);
   int rc;
   const bool mixed = g.t_mode == FMM2D_MODE_MIXED;
@@ -792,6 +809,61 @@ int fmm_make_tree_device(int n, const double *d_xy, const double *d_cir, const d
   }
   g.h_src.clear();
   return build(n, d_xy, d_cir, d_cor);
+}
+
+int fmm_make_tree_mirror_device(int n, const double *d_xy, const double *d_cir, const double *d_cor,
+                                double d_mirror) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init() || resolve_params()) return -1;
+  if (n < 0) {
+    set_error("n < 0");
+    return -1;
+  }
+  hipStream_t s = g.stream;
+  const size_t m = 2 * (size_t)n;
+  if (g.mxy.ensure(2 * m + 2, s) || g.mcir.ensure(m + 1, s) || g.mcor.ensure(m + 1, s)) return -1;
+  if (n > 0) {
+    FMM_HIP_TRY(hipMemcpyAsync(g.mxy.p, d_xy, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, s));
+    FMM_HIP_TRY(hipMemcpyAsync(g.mcir.p, d_cir, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    FMM_HIP_TRY(hipMemcpyAsync(g.mcor.p, d_cor, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    if (launch_mirror_images(n, d_xy, d_cir, d_mirror, g.mxy.p + 2 * (size_t)n, g.mcir.p + n,
+                             g.mcor.p + n, s))
+      return -1;
+  }
+  g.h_src.clear();
+  if (build((int)m, g.mxy.p, g.mcir.p, g.mcor.p)) return -1;
+  g.n_real = n;
+  return 0;
+}
+
+int fmm_drift_ab3_device(int n, double *d_xy, const double *d_hist, double dt, int mirror,
+                         double d_mirror) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  return launch_drift_ab3(n, d_xy, d_hist, dt, mirror, d_mirror, g.stream);
+}
+
+int fmm_history_shift_device(int n, double *d_hist, int first_step) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  return launch_history_shift(n, d_hist, first_step, g.stream);
+}
+
+int fmm_store_velocity_device(int n, const double *d_xy, const double *d_uv, double *d_hist,
+                              double uinf, double vinf, double omz) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  return launch_store_velocity(n, d_xy, d_uv, d_hist, uinf, vinf, omz, g.stream);
+}
+
+int fmm_core_spread_device(int n, double *d_cor, double dt, double re) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ctx_init()) return -1;
+  if (!(re > 0.0)) {
+    set_error("Re must be > 0");
+    return -1;
+  }
+  return launch_core_spread(n, d_cor, dt, re, g.stream);
 }
 
 static int check_offsets(int nsys, const int32_t *off) {
@@ -1287,6 +1359,7 @@ int fmm_set_timing(int on) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (ctx_init()) return -1;
   g.timing = on != 0;
+  for (bool &r : g.tev_rec) r = false;
   return 0;
 }
 
@@ -1299,8 +1372,14 @@ int fmm_get_timing(double *ms, int n) {
   FMM_HIP_TRY(hipStreamSynchronize(g.stream));
   const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {5, 6}};
   for (int i = 0; i < n && i < 5; i++) {
-    float t = 0.f;
-    if (hipEventElapsedTime(&t, g.tev[pairs[i][0]], g.tev[pairs[i][1]]) != hipSuccess) t = -1.f;
+    float t = -1.f;
+    // only events recorded since timing was switched on are queried, so no
+    // error is left behind for a later launch check to report
+    if (g.tev_rec[pairs[i][0]] && g.tev_rec[pairs[i][1]] &&
+        hipEventElapsedTime(&t, g.tev[pairs[i][0]], g.tev[pairs[i][1]]) != hipSuccess) {
+      (void)hipGetLastError();
+      t = -1.f;
+    }
     ms[i] = t;
   }
   return 0;

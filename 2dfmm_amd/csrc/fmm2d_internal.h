@@ -152,6 +152,16 @@ int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, u
 int launch_max_reduce(const double *d_v, int n, double *d_out, hipStream_t s);
 int launch_fmm_eval(const FmmArgs &args, bool mixed, hipStream_t s);
 
+// ---- time stepping around the evaluation (step_kernels.hip) ---------------
+int launch_drift_ab3(int n, double *d_xy, const double *d_hist, double dt, int mirror, double dm,
+                     hipStream_t s);
+int launch_history_shift(int n, double *d_hist, int first, hipStream_t s);
+int launch_store_velocity(int n, const double *d_xy, const double *d_uv, double *d_hist,
+                          double uinf, double vinf, double omz, hipStream_t s);
+int launch_core_spread(int n, double *d_cor, double dt, double re, hipStream_t s);
+int launch_mirror_images(int n, const double *d_xy, const double *d_cir, double dm, double *d_oxy,
+                         double *d_ocir, double *d_ocor, hipStream_t s);
+
 // ---- learned per-leaf parameters (learned_kernels.hip) --------------------
 constexpr int kMlpWeights = 4866;  // 8 -> 64 -> 64 -> 2
 int launch_leaf_features(const FmmArgs &a, int leaf_cap, const double *d_cir, int n,

@@ -52,11 +52,18 @@ __device__ inline double2 cpow_u(double2 b, int e) {
   return r;
 }
 
-// 1 - exp(-x) for x >= 0 in FP32 without the cancellation of small x
+// 1 - exp(-x) for x >= 0 in FP32 without the cancellation of small x: the
+// Taylor series to x^8 below 0.5 (truncation < 1e-8 relative), 1 - exp(-x)
+// above.  Explicit fused multiply-adds (the build disables contraction).
 __device__ inline float one_minus_exp_neg(float x) {
-  const float p = x * (1.0f + x * (-0.5f + x * (1.0f / 6.0f + x * (-1.0f / 24.0f +
-                 x * (1.0f / 120.0f + x * (-1.0f / 720.0f + x * (1.0f / 5040.0f -
-                 x * (1.0f / 40320.0f))))))));
+  float q = fmaf(x, -1.0f / 40320.0f, 1.0f / 5040.0f);
+  q = fmaf(x, q, -1.0f / 720.0f);
+  q = fmaf(x, q, 1.0f / 120.0f);
+  q = fmaf(x, q, -1.0f / 24.0f);
+  q = fmaf(x, q, 1.0f / 6.0f);
+  q = fmaf(x, q, -0.5f);
+  q = fmaf(x, q, 1.0f);
+  const float p = x * q;
   const float e = 1.0f - __expf(-x);
   return x < 0.5f ? p : e;
 }
@@ -343,39 +350,49 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
       float sxs = 0.f, sxc = 0.f, sys = 0.f, syc = 0.f;
       // pair (target, source s): u += -f ry, v += f rx with
       // f = cir (1 - exp(-r^2/sigma^2)) / r^2, pairs closer than 1e-10 skipped
-      auto pair = [&](const float4 s) {
+      auto pair = [&](const float4 s, float &gx, float &gy) {
         const float rx = px - s.x, ry = py - s.y;
-        const float r2 = rx * rx + ry * ry;
+        const float r2 = fmaf(rx, rx, ry * ry);
         float fac = s.z * one_minus_exp_neg(r2 * s.w) * __builtin_amdgcn_rcpf(r2);
         fac = r2 > 1e-20f ? fac : 0.0f;
-        kahan_add(sxs, sxc, -fac * ry);
-        kahan_add(sys, syc, fac * rx);
+        gx = fmaf(-fac, ry, gx);
+        gy = fmaf(fac, rx, gy);
       };
       const float4 *src = a.loc + f;
       int j = rep;
+      // groups of four sources: plain FP32 sum inside a group, compensated
+      // (Kahan) accumulation of the group sums
       for (; j + 3 * nrep < cnt; j += 4 * nrep) {
         const float4 s0 = src[j], s1 = src[j + nrep], s2 = src[j + 2 * nrep],
                      s3 = src[j + 3 * nrep];
-        pair(s0);
-        pair(s1);
-        pair(s2);
-        pair(s3);
+        float gx = 0.f, gy = 0.f;
+        pair(s0, gx, gy);
+        pair(s1, gx, gy);
+        pair(s2, gx, gy);
+        pair(s3, gx, gy);
+        kahan_add(sxs, sxc, gx);
+        kahan_add(sys, syc, gy);
       }
-      for (; j < cnt; j += nrep) pair(src[j]);
+      {
+        float gx = 0.f, gy = 0.f;
+        for (; j < cnt; j += nrep) pair(src[j], gx, gy);
+        kahan_add(sxs, sxc, gx);
+        kahan_add(sys, syc, gy);
+      }
       ax += (double)sxs - (double)sxc;
       ay += (double)sys - (double)syc;
     } else {
       auto pair = [&](const double4 s) {
         const double rx = tx - s.x, ry = ty - s.y;
-        const double r2 = rx * rx + ry * ry;
+        const double r2 = fma(rx, rx, ry * ry);
         // 1 / r^2: hardware reciprocal refined by two Newton steps
         double ri = __builtin_amdgcn_rcp(r2);
-        ri = ri * (2.0 - r2 * ri);
-        ri = ri * (2.0 - r2 * ri);
+        ri = ri * fma(-r2, ri, 2.0);
+        ri = ri * fma(-r2, ri, 2.0);
         double fac = s.z * (1.0 - exp(-r2 * s.w)) * ri;
         fac = r2 > 1e-20 ? fac : 0.0;
-        ax -= fac * ry;
-        ay += fac * rx;
+        ax = fma(-fac, ry, ax);
+        ay = fma(fac, rx, ay);
       };
       const double4 *src = a.loc64 + f;
       int j = rep;
@@ -392,12 +409,14 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
   };
 
   // wave-uniform traversal of the whole tree (masks of opened children per level)
+  // (node indices are uniform across the wave: readfirstlane lets the node
+  // records come through scalar loads)
   Slots2 sl;
-  int d = 0, cur = 0, node = root_of(a, NA.first);
+  int d = 0, cur = 0, node = __builtin_amdgcn_readfirstlane(root_of(a, NA.first));
   bool expand = true;
   for (;;) {
     if (expand) {
-      const int base = a.nodes[node].first;
+      const int base = __builtin_amdgcn_readfirstlane(a.nodes[node].first);
       uint32_t m = 0;
       for (int q = 0; q < 4; q++) {
         const int c = base + q;
@@ -427,13 +446,13 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
       m = sl.get(d);
       if (m) break;
       if (d == 1) { done = true; break; }
-      cur = a.gparent[cur >> 2];
+      cur = __builtin_amdgcn_readfirstlane(a.gparent[cur >> 2]);
       d--;
     }
     if (done) break;
     const int q = 31 - __clz(m);
     sl.set(d, m & ~(1u << q));
-    node = (cur & ~3) + q;
+    node = __builtin_amdgcn_readfirstlane((cur & ~3) + q);
     const int32_t cn = a.nodes[node].cnt;
     if (node_internal(cn)) {
       expand = true;

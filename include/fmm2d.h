@@ -139,6 +139,30 @@ int fmm_mlp_grad(int n, const float *x, const float *y, const float *weights, fl
 int fmm_mlp_train(int n, const float *x, const float *y, float *weights, int epochs, int batch,
                   float lr, unsigned seed, float *loss_hist);
 
+/* Mirror images (biot_b's i_mirror, ver-c/src/includes/d_biot.f90:39-53; the
+ * reference treecode has none, README.md:8): the tree is built over the n
+ * blobs and their images (x, 2 d_mirror - y) with circulation -cir and the
+ * point-vortex kernel (core 0); fmm_eval_sources* then return the n real
+ * blobs' velocities.  Device pointers. */
+int fmm_make_tree_mirror_device(int n, const double *d_xy, const double *d_cir,
+                                const double *d_cor, double d_mirror);
+
+/* Time stepping on device-resident blobs (the solver's updates around the
+ * evaluation; no counterpart in the tree code).  hist is laid out like the
+ * solver's vor_vb(2,3,n): hist[(i*3 + k)*2 + c], k = 0 newest.
+ *   drift: x += dt (23/12 v1 - 4/3 v2 + 5/12 v3) (e_drift.f90:29-44), with
+ *          reflection below y = d_mirror when mirror != 0 (e_drift.f90:46);
+ *   history shift: v3 <- v2, v2 <- v1; first_step copies v1 to v2 first
+ *          (e_velo_update.f90:17-38);
+ *   store velocity: v1 <- (uinf - omz y, vinf + omz x) + uv (e_velo_mpi.f90:44-47);
+ *   core spreading: sigma <- sqrt(sigma^2 + 2.2418^2 dt / re) (e_core.f90:15-19). */
+int fmm_drift_ab3_device(int n, double *d_xy, const double *d_hist, double dt, int mirror,
+                         double d_mirror);
+int fmm_history_shift_device(int n, double *d_hist, int first_step);
+int fmm_store_velocity_device(int n, const double *d_xy, const double *d_uv, double *d_hist,
+                              double uinf, double vinf, double omz);
+int fmm_core_spread_device(int n, double *d_cor, double dt, double re);
+
 /* Velocities at all blobs of the last tree, in the caller's blob order,
  * interleaved u0,v0,u1,v1,...  Host or device output buffer. */
 int fmm_eval_sources(double *uv);
