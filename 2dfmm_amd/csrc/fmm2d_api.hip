@@ -568,8 +568,8 @@ struct HostTree {
 
 static int fetch_tree(HostTree &h) {
   hipStream_t s = g.stream;
-  if (g.nsys > 1) {
-    set_error("tree dumps are for a single system");
+  if (g.nsys > 1 || g.n_real >= 0) {
+    set_error("tree dumps are for a single system without mirror images");
     return -1;
   }
   h.nodes.resize(g.nnodes);
@@ -970,7 +970,9 @@ int fmm_blob_leaf(int32_t *leaf_of_blob) {
   DevBuf<int32_t> d;
   if (d.ensure(g.n, s)) return -1;
   int rc = launch_blob_leaf(fmm_args(nullptr), d.p, s);
-  if (!rc && (hipMemcpyAsync(leaf_of_blob, d.p, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost, s) !=
+  // with mirror images only the real blobs (the first n_real) are reported
+  const size_t nout = g.n_real >= 0 ? (size_t)g.n_real : (size_t)g.n;
+  if (!rc && (hipMemcpyAsync(leaf_of_blob, d.p, sizeof(int32_t) * nout, hipMemcpyDeviceToHost, s) !=
                   hipSuccess ||
               hipStreamSynchronize(s) != hipSuccess)) {
     set_error("download failed");
@@ -1176,9 +1178,11 @@ int fmm_eval_sources(double *uv) {
   if (need_tree()) return -1;
   if (g.n == 0) return 0;
   hipStream_t s = g.stream;
+  // with mirror images the tree holds 2 n_real blobs but only n_real are returned
+  const size_t nout = g.n_real >= 0 ? (size_t)g.n_real : (size_t)g.n;
   if (g.uv_dev.ensure(2 * (size_t)g.n, s)) return -1;
   if (eval_sources_dev(g.uv_dev.p)) return -1;
-  FMM_HIP_TRY(hipMemcpyAsync(uv, g.uv_dev.p, sizeof(double) * 2 * g.n, hipMemcpyDeviceToHost, s));
+  FMM_HIP_TRY(hipMemcpyAsync(uv, g.uv_dev.p, sizeof(double) * 2 * nout, hipMemcpyDeviceToHost, s));
   FMM_HIP_TRY(hipStreamSynchronize(s));
   return 0;
 }

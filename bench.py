@@ -157,6 +157,11 @@ def main():
     ap.add_argument("--cpu-targets", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="mixture4m", choices=["mixture4m", "batch"])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 256 MiB device to device (a kernel of known traffic for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--traffic", default=None,
+                    help="JSON file with measured HBM bytes per launch of the dominant kernel")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -171,6 +176,16 @@ def main():
     if args.workload == "batch":
         bench_batch(args, fmm, wl, torch)
         return
+
+    if args.copy_calibration:
+        a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 256 MiB read + 256 MiB written")
+        del a, b
 
     n = args.n
     log(f"[bench] generating gaussian_mixture N={n} seed={args.seed}")
@@ -258,6 +273,10 @@ def main():
         log(f"[bench] cpu baseline {cpu['value']:.4g} velocities/s ({cpu['sample']})")
 
     peak = PEAK[head["precision"]]
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            traffic = json.load(f).get(head["kernel"])
     out = {
         "metric": METRIC,
         "value": head["value"],
@@ -281,7 +300,7 @@ def main():
         "modes": results,
         "roofline": {"kernel": head["kernel"], "bound": "mfma", "achieved": head["achieved_tflops"],
                      "peak": peak, "unit": "TFLOP/s", "frac": head["achieved_tflops"] / peak,
-                     "traffic": None, "flops_per_launch": head["flops"],
+                     "traffic": traffic, "flops_per_launch": head["flops"],
                      "avg_launch_ms": head["phases_ms"]["evaluate"]},
         "cpu_baseline": cpu,
     }
