@@ -55,17 +55,21 @@ __device__ inline double2 cpow_u(double2 b, int e) {
 // 1 - exp(-x) for x >= 0 in FP32 without the cancellation of small x: the
 // Taylor series to x^8 below 0.5 (truncation < 1e-8 relative), 1 - exp(-x)
 // above.  Explicit fused multiply-adds (the build disables contraction).
+// The series sits in a branch: a wave skips it whenever none of its lanes
+// has a pair closer than 0.7 core radii (most pairs of a near field).
 __device__ inline float one_minus_exp_neg(float x) {
-  float q = fmaf(x, -1.0f / 40320.0f, 1.0f / 5040.0f);
-  q = fmaf(x, q, -1.0f / 720.0f);
-  q = fmaf(x, q, 1.0f / 120.0f);
-  q = fmaf(x, q, -1.0f / 24.0f);
-  q = fmaf(x, q, 1.0f / 6.0f);
-  q = fmaf(x, q, -0.5f);
-  q = fmaf(x, q, 1.0f);
-  const float p = x * q;
-  const float e = 1.0f - __expf(-x);
-  return x < 0.5f ? p : e;
+  float r = 1.0f - __expf(-x);
+  if (x < 0.5f) {
+    float q = fmaf(x, -1.0f / 40320.0f, 1.0f / 5040.0f);
+    q = fmaf(x, q, -1.0f / 720.0f);
+    q = fmaf(x, q, 1.0f / 120.0f);
+    q = fmaf(x, q, -1.0f / 24.0f);
+    q = fmaf(x, q, 1.0f / 6.0f);
+    q = fmaf(x, q, -0.5f);
+    q = fmaf(x, q, 1.0f);
+    r = x * q;
+  }
+  return r;
 }
 
 __device__ inline void kahan_add(float &s, float &c, float v) {
