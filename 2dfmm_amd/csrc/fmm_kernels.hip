@@ -287,7 +287,8 @@ __device__ inline int root_of(const FmmArgs &a, int first) {
 // All work of one target leaf (index li of the leaf list) for one wave.
 template <bool MIXED>
 __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
-                                 double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64]) {
+                                 double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
+                                 double4 (*s_stage)[64]) {
   const int A = a.leaves[li];
   const NodeRec NA = a.nodes[A];
   const int cA = node_count(NA.cnt);
@@ -344,10 +345,15 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
     acc.tick();
   };
 
+  // Direct sums against one near leaf.  With one replica (every lane needs
+  // the same source at each step) the source index is uniform and the
+  // sources come through the scalar cache; otherwise the leaf's sources are
+  // staged once in this wave's LDS slice and read from there, so each source
+  // crosses the vector memory path once per wave instead of once per lane.
   auto near_leaf = [&](int c) {
     const NodeRec C = a.nodes[c];
-    const int cnt = node_count(C.cnt), f = C.first;
-    if (!tact) return;
+    const int cnt = node_count(C.cnt);
+    const int f = __builtin_amdgcn_readfirstlane(C.first);
     if (MIXED) {
       const float dx = (float)(C.cx - NA.cx), dy = (float)(C.cy - NA.cy);
       const float px = lx - dx, py = ly - dy;
@@ -362,26 +368,50 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
         gx = fmaf(-fac, ry, gx);
         gy = fmaf(fac, rx, gy);
       };
-      const float4 *src = a.loc + f;
-      int j = rep;
       // groups of four sources: plain FP32 sum inside a group, compensated
       // (Kahan) accumulation of the group sums
-      for (; j + 3 * nrep < cnt; j += 4 * nrep) {
-        const float4 s0 = src[j], s1 = src[j + nrep], s2 = src[j + 2 * nrep],
-                     s3 = src[j + 3 * nrep];
+      if (nrep == 1) {
+        const float4 *src = a.loc + f;
+        int j = 0;
+        for (; j + 3 < cnt; j += 4) {
+          const float4 s0 = src[j], s1 = src[j + 1], s2 = src[j + 2], s3 = src[j + 3];
+          float gx = 0.f, gy = 0.f;
+          pair(s0, gx, gy);
+          pair(s1, gx, gy);
+          pair(s2, gx, gy);
+          pair(s3, gx, gy);
+          kahan_add(sxs, sxc, gx);
+          kahan_add(sys, syc, gy);
+        }
         float gx = 0.f, gy = 0.f;
-        pair(s0, gx, gy);
-        pair(s1, gx, gy);
-        pair(s2, gx, gy);
-        pair(s3, gx, gy);
+        for (; j < cnt; j++) pair(src[j], gx, gy);
         kahan_add(sxs, sxc, gx);
         kahan_add(sys, syc, gy);
-      }
-      {
-        float gx = 0.f, gy = 0.f;
-        for (; j < cnt; j += nrep) pair(src[j], gx, gy);
-        kahan_add(sxs, sxc, gx);
-        kahan_add(sys, syc, gy);
+      } else {
+        float4 *st = reinterpret_cast<float4 *>(s_stage[w]);
+        if (lane < cnt) st[lane] = a.loc[f + lane];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (tact) {
+          int j = rep;
+          for (; j + 3 * nrep < cnt; j += 4 * nrep) {
+            const float4 s0 = st[j], s1 = st[j + nrep], s2 = st[j + 2 * nrep],
+                         s3 = st[j + 3 * nrep];
+            float gx = 0.f, gy = 0.f;
+            pair(s0, gx, gy);
+            pair(s1, gx, gy);
+            pair(s2, gx, gy);
+            pair(s3, gx, gy);
+            kahan_add(sxs, sxc, gx);
+            kahan_add(sys, syc, gy);
+          }
+          float gx = 0.f, gy = 0.f;
+          for (; j < cnt; j += nrep) pair(st[j], gx, gy);
+          kahan_add(sxs, sxc, gx);
+          kahan_add(sys, syc, gy);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
       ax += (double)sxs - (double)sxc;
       ay += (double)sys - (double)syc;
@@ -398,17 +428,37 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
         ax = fma(-fac, ry, ax);
         ay = fma(fac, rx, ay);
       };
-      const double4 *src = a.loc64 + f;
-      int j = rep;
-      for (; j + 3 * nrep < cnt; j += 4 * nrep) {
-        const double4 s0 = src[j], s1 = src[j + nrep], s2 = src[j + 2 * nrep],
-                      s3 = src[j + 3 * nrep];
-        pair(s0);
-        pair(s1);
-        pair(s2);
-        pair(s3);
+      if (nrep == 1) {
+        const double4 *src = a.loc64 + f;
+        int j = 0;
+        for (; j + 3 < cnt; j += 4) {
+          const double4 s0 = src[j], s1 = src[j + 1], s2 = src[j + 2], s3 = src[j + 3];
+          pair(s0);
+          pair(s1);
+          pair(s2);
+          pair(s3);
+        }
+        for (; j < cnt; j++) pair(src[j]);
+      } else {
+        double4 *st = s_stage[w];
+        if (lane < cnt) st[lane] = a.loc64[f + lane];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (tact) {
+          int j = rep;
+          for (; j + 3 * nrep < cnt; j += 4 * nrep) {
+            const double4 s0 = st[j], s1 = st[j + nrep], s2 = st[j + 2 * nrep],
+                          s3 = st[j + 3 * nrep];
+            pair(s0);
+            pair(s1);
+            pair(s2);
+            pair(s3);
+          }
+          for (; j < cnt; j += nrep) pair(st[j]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
-      for (; j < cnt; j += nrep) pair(src[j]);
     }
   };
 
@@ -523,10 +573,11 @@ template <bool MIXED>
 __global__ __launch_bounds__(256) void fmm_leaf_kernel(FmmArgs a) {
   __shared__ double2 s_loc[4][kFmmMaxP];
   __shared__ double2 s_red[4][64];
+  __shared__ double4 s_stage[4][64];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = blockIdx.x * 4 + w;
   if (li >= a.nleaves) return;
-  fmm_process_leaf<MIXED>(a, li, w, lane, s_loc, s_red);
+  fmm_process_leaf<MIXED>(a, li, w, lane, s_loc, s_red, s_stage);
 }
 
 // Persistent form: a fixed grid whose waves take target leaves from a work
@@ -540,6 +591,7 @@ __global__ __launch_bounds__(256) void fmm_leaf_persistent_kernel(FmmArgs a,
                                                                   uint32_t *__restrict__ counter) {
   __shared__ double2 s_loc[4][kFmmMaxP];
   __shared__ double2 s_red[4][64];
+  __shared__ double4 s_stage[4][64];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = *nleaves;
   for (;;) {
@@ -547,7 +599,7 @@ __global__ __launch_bounds__(256) void fmm_leaf_persistent_kernel(FmmArgs a,
     if (lane == 0) li = atomicAdd(counter, 1u);
     li = __shfl(li, 0, 64);
     if (li >= total) break;
-    fmm_process_leaf<MIXED>(a, (int)li, w, lane, s_loc, s_red);
+    fmm_process_leaf<MIXED>(a, (int)li, w, lane, s_loc, s_red, s_stage);
   }
 }
 
