@@ -66,6 +66,8 @@ lib.fmm_direct_device.argtypes = [C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, C.c
 lib.fmm_make_forest.argtypes = [C.c_int, _ip, _dp, _dp, _dp]
 lib.fmm_make_forest_device.argtypes = [C.c_int, _ip, _vp, _vp, _vp]
 lib.fmm_set_persistent.argtypes = [C.c_int, C.c_int]
+lib.fmm_set_split.argtypes = [C.c_int]
+lib.fmm_get_split.argtypes = []
 lib.fmm_device_cus.argtypes = []
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 lib.fmm_set_learned.argtypes = [C.c_void_p, C.c_float, C.c_float]
@@ -306,6 +308,15 @@ def mlp_train(x, y, weights, epochs=100, batch=1024, lr=1e-3, seed=1):
 
 def set_persistent(blocks: int, graph: bool = False) -> None:
     _check(lib.fmm_set_persistent(int(blocks), 1 if graph else 0), "set_persistent")
+
+
+def set_split(budget: int) -> None:
+    """Near work units per chunk of a split target leaf (0: no splitting)."""
+    _check(lib.fmm_set_split(int(budget)), "set_split")
+
+
+def get_split() -> int:
+    return int(lib.fmm_get_split())
 
 
 def device_cus() -> int:

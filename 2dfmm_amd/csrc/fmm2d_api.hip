@@ -113,6 +113,13 @@ struct Context {
   // persistent evaluation and its graph
   DevBuf<uint32_t> ntl_dev, counter;
   int persist_blocks = 0;  // 0: one wave per leaf over a sized grid
+  // splitting of target leaves with much near work (0: off)
+  int split_budget = 8192;
+  DevBuf<SplitCk> ck;
+  DevBuf<int2> leaf_ck;
+  DevBuf<double2> part;
+  DevBuf<uint32_t> split_ctr;
+  int ncu = 0;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -391,6 +398,7 @@ static int need_tree() {
 
 static FmmArgs fmm_args(double *d_uv) {
   FmmArgs a;
+  memset(&a, 0, sizeof a);  // padding too: the graph key compares the bytes
   a.nodes = g.nodes.p;
   a.gparent = g.gparent.p;
   a.coef = g.coef.p;
@@ -410,8 +418,34 @@ static FmmArgs fmm_args(double *d_uv) {
   a.root_shift = g.root_shift;
   a.leaf_par = g.leaf_par_on ? g.leaf_par.p : nullptr;
   a.smax = g.smax_h;
+  a.split_budget = g.split_budget;
+  a.ck_cap = (int)g.ck.cap;
+  a.ck = g.ck.p;
+  a.ck_count = g.split_ctr.p;
+  a.leaf_ck = g.leaf_ck.p;
+  a.part = g.part.p;
   return a;
 }
+
+// Buffers of the split passes for the current tree: room for two chunks per
+// target leaf on average (a fused wave that finds no room runs its chunks
+// itself), and the grid of the continuation passes.
+static int ensure_split() {
+  if (g.split_budget <= 0) return 0;
+  hipStream_t s = g.stream;
+  const size_t cap = 2 * (size_t)g.ntleaves + 4096;
+  if (g.ck.ensure(cap, s) || g.leaf_ck.ensure((size_t)g.ntleaves + 1, s) ||
+      g.part.ensure(64 * g.ck.cap, s) || g.split_ctr.ensure(2, s))
+    return -1;
+  if (g.ncu <= 0) {
+    int dev = 0;
+    FMM_HIP_TRY(hipGetDevice(&dev));
+    FMM_HIP_TRY(hipDeviceGetAttribute(&g.ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  return 0;
+}
+
+static int split_blocks() { return 4 * (g.ncu > 0 ? g.ncu : 256); }
 
 // Learned per-leaf (p, cut) for the current tree: features -> MLP -> params.
 static int apply_learned() {
@@ -443,6 +477,8 @@ static int eval_fmm_graph(const FmmArgs &a, bool mixed) {
   memcpy(key, &ka, sizeof(FmmArgs));
   key[sizeof(FmmArgs)] = mixed ? 1 : 0;
   memcpy(key + sizeof(FmmArgs) + 4, &g.persist_blocks, sizeof(int));
+  const int sb = split_blocks();
+  memcpy(key + sizeof(FmmArgs) + 8, &sb, sizeof(int));
   if (!g.graph_exec || memcmp(key, g.graph_key, sizeof key) != 0) {
     if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);
     if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -450,7 +486,7 @@ static int eval_fmm_graph(const FmmArgs &a, bool mixed) {
     g.graph = nullptr;
     FMM_HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
     const int rc = launch_fmm_eval_persistent(a, mixed, g.ntl_dev.p, g.counter.p, g.persist_blocks,
-                                              g.stream);
+                                              g.split_ctr.p, sb, g.stream);
     hipGraph_t gr = nullptr;
     const hipError_t e = hipStreamEndCapture(g.stream, &gr);
     if (rc || e != hipSuccess) {
@@ -484,15 +520,17 @@ static int eval_sources_dev(double *d_uv) {
   mark(5);
   int rc;
   const bool mixed = g.t_mode == FMM2D_MODE_MIXED;
+  if (g.t_mode != FMM2D_MODE_TREECODE && ensure_split()) return -1;
   if (g.t_mode == FMM2D_MODE_TREECODE)
     rc = launch_parity_eval(view(), g.n, nullptr, g.perm.p, d_uv, g.stream);
   else if (g.persist_blocks > 0 && g.use_graph)
     rc = eval_fmm_graph(fmm_args(d_uv), mixed);
   else if (g.persist_blocks > 0)
     rc = launch_fmm_eval_persistent(fmm_args(d_uv), mixed, g.ntl_dev.p, g.counter.p,
-                                    g.persist_blocks, g.stream);
+                                    g.persist_blocks, g.split_ctr.p, split_blocks(), g.stream);
   else
-    rc = launch_fmm_eval(fmm_args(d_uv), mixed, g.stream);
+    rc = launch_fmm_eval(fmm_args(d_uv), mixed, g.ntl_dev.p, g.split_ctr.p, split_blocks(),
+                         g.stream);
   mark(6);
   return rc;
 }
@@ -913,6 +951,21 @@ int fmm_set_persistent(int blocks, int use_graph) {
   g.persist_blocks = blocks;
   g.use_graph = use_graph != 0 && blocks > 0;
   return 0;
+}
+
+int fmm_set_split(int budget) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (budget < 0) {
+    set_error("split budget must be >= 0");
+    return -1;
+  }
+  g.split_budget = budget;
+  return 0;
+}
+
+int fmm_get_split(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g.split_budget;
 }
 
 // ---- learned per-leaf parameters -------------------------------------------
@@ -1459,6 +1512,7 @@ void fmm_release(void) {
   g.loc.release(); g.loc64.release(); g.smax.release(); g.leaves.release(); g.lat.release();
   g.lflag.release(); g.lpos.release(); g.direct_ws.release();
   g.doff.release(); g.ntl_dev.release(); g.counter.release();
+  g.ck.release(); g.leaf_ck.release(); g.part.release(); g.split_ctr.release();
   if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);
   if (g.graph) (void)hipGraphDestroy(g.graph);
   g.graph_exec = nullptr;

@@ -119,12 +119,21 @@ int launch_direct(int ns, const double *d_sxy, const double *d_cir, const double
                   hipStream_t s);
 
 // ---- accurate fast mode (fmm_kernels.hip) ------------------------------
+// Saved walk of a target leaf with much near work: the continuation pass
+// resumes it at leaf `node` and sums near leaves while the work count is
+// below `end`.
+struct SplitCk {
+  uint64_t lo, hi;   // open-child masks, 4 bits per level
+  int64_t cum, end;  // near work units before this point; end of the chunk
+  int32_t li, d, cur, node;  // target leaf (-1: skip), level, group, leaf
+};
+
 struct FmmArgs {
   const NodeRec *nodes;
   const int32_t *gparent;
   const double2 *coef;  // unscaled a_k, p per node
   const double *sx, *sy, *sg;
-  const float4 *loc;    // (x - c_leaf, y - c_leaf, cir, 1/sigma^2) in FP32
+  const float4 *loc;    // (x - c_leaf, y - c_leaf, cir w, w) in FP32, w = log2(e)/sigma^2
   const double4 *loc64; // (x, y, cir, 1/sigma^2) in FP64
   const int32_t *leaves;
   int nleaves;
@@ -140,6 +149,15 @@ struct FmmArgs {
   // null = p and rcut for every leaf.  smax = largest core radius.
   const float2 *leaf_par;
   double smax;
+  // splitting of target leaves with much near work (split_budget 0: off):
+  // checkpoint records (ck_cap of them; *ck_count taken), per target leaf
+  // (first record, records), 64 partial sums per record
+  int split_budget;
+  int ck_cap;
+  SplitCk *ck;
+  uint32_t *ck_count;
+  int2 *leaf_ck;
+  double2 *part;
 };
 int launch_mfma_selftest(const double *a, const double *b, double *out, double *ref, int fp32,
                          hipStream_t s);
@@ -150,7 +168,10 @@ int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleav
 int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, uint32_t *d_flag,
                      uint32_t *d_pos, int32_t *d_leaves, uint32_t *d_count, hipStream_t s);
 int launch_max_reduce(const double *d_v, int n, double *d_out, hipStream_t s);
-int launch_fmm_eval(const FmmArgs &args, bool mixed, hipStream_t s);
+// One wave per target leaf; with splitting on, d_split_ctr holds two counters
+// and the continuation passes run on split_blocks x 256 threads.
+int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
+                    uint32_t *d_split_ctr, int split_blocks, hipStream_t s);
 
 // ---- time stepping around the evaluation (step_kernels.hip) ---------------
 int launch_drift_ab3(int n, double *d_xy, const double *d_hist, double dt, int mirror, double dm,
@@ -178,7 +199,8 @@ int launch_blob_leaf(const FmmArgs &a, int32_t *d_out, hipStream_t s);
 // Persistent grid of nblocks x 256 threads over a work counter; the number of
 // target leaves is read from *d_nleaves (graph-capturable).
 int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
-                               uint32_t *d_counter, int nblocks, hipStream_t s);
+                               uint32_t *d_counter, int nblocks, uint32_t *d_split_ctr,
+                               int split_blocks, hipStream_t s);
 int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s);
 
 }  // namespace fmm2d

@@ -23,10 +23,10 @@
 // (target, replica) pairs for the direct part.
 //
 // Precision: FP64 everywhere for the plain form.  The mixed form runs the
-// direct part in FP32 on leaf-relative coordinates with Kahan-compensated
-// sums (promoted to FP64 per source leaf) and the contraction on FP32 MFMA
-// (promoted to FP64 every 8 cells); expansions (S2M/M2M) and the L2P
-// accumulation stay in FP64.
+// direct part in FP32 on coordinates relative to the target leaf's centre
+// (sums of four pairs in FP32, accumulated in FP64) and the contraction on
+// FP32 MFMA (promoted to FP64 every 8 cells); expansions (S2M/M2M) and the
+// L2P accumulation stay in FP64.
 #include <hip/hip_runtime.h>
 
 #include "fmm2d_internal.h"
@@ -52,31 +52,30 @@ __device__ inline double2 cpow_u(double2 b, int e) {
   return r;
 }
 
-// 1 - exp(-x) for x >= 0 in FP32 without the cancellation of small x: the
-// Taylor series to x^8 below 0.5 (truncation < 1e-8 relative), 1 - exp(-x)
-// above.  Explicit fused multiply-adds (the build disables contraction).
-// The series sits in a branch: a wave skips it whenever none of its lanes
-// has a pair closer than 0.7 core radii (most pairs of a near field).
-__device__ inline float one_minus_exp_neg(float x) {
-  float r = 1.0f - __expf(-x);
-  if (x < 0.5f) {
-    float q = fmaf(x, -1.0f / 40320.0f, 1.0f / 5040.0f);
-    q = fmaf(x, q, -1.0f / 720.0f);
-    q = fmaf(x, q, 1.0f / 120.0f);
-    q = fmaf(x, q, -1.0f / 24.0f);
-    q = fmaf(x, q, 1.0f / 6.0f);
-    q = fmaf(x, q, -0.5f);
-    q = fmaf(x, q, 1.0f);
-    r = x * q;
-  }
-  return r;
-}
+// Pair kernel of the mixed form.  Each source stores w = log2(e) / sigma^2
+// (capped at 7.2e19) and z = cir w, so with y = r^2 w
+//   f = cir (1 - exp(-r^2 / sigma^2)) / r^2 = z h(y),  h(y) = (1 - 2^-y) / y.
+// h is (1 - 2^-y) / y for y >= 0.7213 (2^-y <= 0.61: no cancellation; one
+// v_exp_f32 and one v_rcp_f32) and its Taylor series below, which a wave
+// runs only when one of its lanes needs it.  Pairs closer than 1e-10 (the
+// direct sum's cut-off, r^2 <= 1e-20) have y < 0.7213 because of the cap
+// and are zeroed inside the series branch; the cap makes a zero core radius
+// (mirror images) the point vortex for every pair beyond that cut-off up to
+// a 2^-y term, y >= r^2 7.2e19.
+constexpr float kSeriesY = 0.7213475f;  // y ln 2 = 0.5
+constexpr double kMaxW = 7.2e19;
 
-__device__ inline void kahan_add(float &s, float &c, float v) {
-  const float y = v - c;
-  const float t = s + y;
-  c = (t - s) - y;
-  s = t;
+// h(y) = sum_k ln2 (-ln2)^k y^k / (k+1)!, k <= 7; truncation below 1.1e-8
+// relative on [0, 0.7213].  Explicit fused multiply-adds (the build
+// disables contraction).
+__device__ inline float h_series(float y) {
+  float q = fmaf(y, -1.3215487e-06f, 1.5252734e-05f);
+  q = fmaf(y, q, -1.5403530e-04f);
+  q = fmaf(y, q, 1.3333558e-03f);
+  q = fmaf(y, q, -9.6181291e-03f);
+  q = fmaf(y, q, 5.5504109e-02f);
+  q = fmaf(y, q, -2.4022651e-01f);
+  return fmaf(y, q, 6.9314718e-01f);
 }
 
 // ---------------------------------------------------------------------------
@@ -124,8 +123,9 @@ int launch_mfma_selftest(const double *a, const double *b, double *out, double *
 }
 
 // ---------------------------------------------------------------------------
-// per-source data of the accurate modes: FP32 (x - c_leaf, y - c_leaf, cir,
-// 1/sigma^2) for the mixed form, FP64 (x, y, cir, 1/sigma^2) for the plain one
+// per-source data of the accurate modes: FP32 (x - c_leaf, y - c_leaf, z, w)
+// with w = log2(e)/sigma^2 (capped) and z = cir w for the mixed form, FP64
+// (x, y, cir, 1/sigma^2) for the plain one
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void leaf_local_kernel(const NodeRec *__restrict__ nodes,
                                                          const int32_t *__restrict__ leaves,
@@ -143,8 +143,9 @@ __global__ __launch_bounds__(256) void leaf_local_kernel(const NodeRec *__restri
     const int j = r.first + e;
     const double s2 = ss[j] * ss[j];
     loc64[j] = make_double4(sx[j], sy[j], sg[j], 1.0 / s2);
-    loc[j] = make_float4((float)(sx[j] - r.cx), (float)(sy[j] - r.cy), (float)sg[j],
-                         (float)(1.0 / s2));
+    const double wj = fmin(1.4426950408889634 / s2, kMaxW);
+    loc[j] = make_float4((float)(sx[j] - r.cx), (float)(sy[j] - r.cy), (float)(sg[j] * wj),
+                         (float)wj);
   }
 }
 
@@ -284,36 +285,328 @@ __device__ inline int root_of(const FmmArgs &a, int first) {
   return a.root_shift < 0 ? 0 : (int)(a.key[first] >> a.root_shift);
 }
 
-// All work of one target leaf (index li of the leaf list) for one wave.
+// ---------------------------------------------------------------------------
+// Work of one target leaf.  A wave walks the tree for its leaf; far cells go
+// into the MFMA contraction, near leaves are summed directly.  Leaves with
+// much near work (dense clusters: a few leaves can carry hundreds of
+// thousands of near sources) are split: the fused pass sums the first
+// `split_budget` work units itself and saves a checkpoint of the walk every
+// `split_budget` units after that; continuation chunks resume the walk from
+// the checkpoints on other waves, and a last pass adds their partial sums in
+// chunk order (deterministic).
+// ---------------------------------------------------------------------------
+constexpr int kMaxCk = 64;  // checkpoints per leaf kept in LDS
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Arrays that no kernel of the evaluation writes (tree, sources, leaf list,
+// checkpoint records in the continuation pass) are read through the
+// constant address space: loads at wave-uniform indices then go through the
+// scalar unit even where the kernel stores elsewhere.
+template <typename T>
+__device__ inline T ldro(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized records only");
+  typedef const uint32_t __attribute__((address_space(4))) *cwords;
+  const cwords q = (cwords)p;
+  T v;
+  uint32_t *d = reinterpret_cast<uint32_t *>(&v);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); i++) d[i] = q[i];
+  return v;
+}
+__device__ inline uint64_t rfl64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Per-target-leaf values; lanes are (target ti, replica rep) pairs.
+struct LeafCtx {
+  NodeRec NA;
+  int cA, pA, nrep, ti, rep, tj;
+  bool tact;
+  double RA, sA, rcut, tx, ty;
+  float lx, ly;
+};
+
+template <bool MIXED>
+__device__ inline LeafCtx leaf_ctx(const FmmArgs &a, int li, int lane) {
+  LeafCtx L;
+  L.NA = ldro(a.nodes + rfl(ldro(a.leaves + li)));
+  L.cA = node_count(L.NA.cnt);
+  L.RA = L.NA.region * 0.70710678118654752;
+  L.sA = L.NA.region;
+  // this leaf's expansion order and near-field radius
+  L.pA = a.p;
+  L.rcut = a.rcut;
+  if (a.leaf_par) {
+    const float2 lp = a.leaf_par[li];
+    L.pA = min(a.p, max(2, (int)lp.x));
+    L.rcut = (double)lp.y * a.smax;
+  }
+  L.nrep = max(1, 64 / L.cA);
+  L.ti = lane % L.cA;
+  L.rep = lane / L.cA;
+  L.tact = L.rep < L.nrep;
+  L.tj = L.NA.first + L.ti;
+  L.tx = a.sx[L.tj];
+  L.ty = a.sy[L.tj];
+  L.lx = 0.f;
+  L.ly = 0.f;
+  if (MIXED) {
+    const float4 q = a.loc[L.tj];
+    L.lx = q.x;
+    L.ly = q.y;
+  }
+  return L;
+}
+
+// Work units of one near leaf of cnt sources: wave steps plus a fixed cost.
+__device__ inline int64_t near_units(int cnt, int nrep) {
+  return (int64_t)((cnt + nrep - 1) / nrep) + 2;
+}
+
+// Direct sums of the targets of L against near leaf c.  With one replica
+// (every lane needs the same source at each step) the source index is
+// uniform and the sources come through the scalar cache; otherwise the
+// leaf's sources are staged once in this wave's LDS slice `st` and read from
+// there, so each source crosses the vector memory path once per wave.
+template <bool MIXED>
+__device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, int c, int lane, double4 *st,
+                                 double &ax, double &ay) {
+  const NodeRec C = ldro(a.nodes + c);
+  const int cnt = node_count(C.cnt);
+  const int f = rfl(C.first);
+  const int nrep = L.nrep;
+  if (MIXED) {
+    const float dx = (float)(C.cx - L.NA.cx), dy = (float)(C.cy - L.NA.cy);
+    const float px = L.lx - dx, py = L.ly - dy;
+    // pair (target, source s): u += -f ry, v += f rx with f = z h(y),
+    // y = r^2 s.w (see h_series)
+    auto pair = [&](const float4 s, float &gx, float &gy) {
+      const float rx = px - s.x, ry = py - s.y;
+      const float r2 = fmaf(rx, rx, ry * ry);
+      const float y = r2 * s.w;
+      float h = (1.0f - __builtin_amdgcn_exp2f(-y)) * __builtin_amdgcn_rcpf(y);
+      if (__builtin_amdgcn_ballot_w64(y < kSeriesY)) {
+        if (y < kSeriesY) h = r2 > 1e-20f ? h_series(y) : 0.0f;
+      }
+      const float fac = s.z * h;
+      gx = fmaf(-fac, ry, gx);
+      gy = fmaf(fac, rx, gy);
+    };
+    // groups of four sources: FP32 sum inside a group, FP64 across groups
+    if (nrep == 1) {
+      const float4 *src = a.loc + f;
+      int j = 0;
+      if (cnt >= 4) {
+        // the next group's loads are issued before this group's arithmetic
+        float4 n0 = ldro(src), n1 = ldro(src + 1), n2 = ldro(src + 2), n3 = ldro(src + 3);
+        for (; j + 3 < cnt; j += 4) {
+          const float4 s0 = n0, s1 = n1, s2 = n2, s3 = n3;
+          if (j + 7 < cnt) {
+            n0 = ldro(src + j + 4);
+            n1 = ldro(src + j + 5);
+            n2 = ldro(src + j + 6);
+            n3 = ldro(src + j + 7);
+          }
+          float gx = 0.f, gy = 0.f;
+          pair(s0, gx, gy);
+          pair(s1, gx, gy);
+          pair(s2, gx, gy);
+          pair(s3, gx, gy);
+          ax += (double)gx;
+          ay += (double)gy;
+        }
+      }
+      if (j < cnt) {
+        float gx = 0.f, gy = 0.f;
+        for (; j < cnt; j++) pair(ldro(src + j), gx, gy);
+        ax += (double)gx;
+        ay += (double)gy;
+      }
+    } else {
+      float4 *sf = reinterpret_cast<float4 *>(st);
+      if (lane < cnt) sf[lane] = a.loc[f + lane];
+      wave_sync();
+      if (L.tact) {
+        int j = L.rep;
+        for (; j + 3 * nrep < cnt; j += 4 * nrep) {
+          const float4 s0 = sf[j], s1 = sf[j + nrep], s2 = sf[j + 2 * nrep], s3 = sf[j + 3 * nrep];
+          float gx = 0.f, gy = 0.f;
+          pair(s0, gx, gy);
+          pair(s1, gx, gy);
+          pair(s2, gx, gy);
+          pair(s3, gx, gy);
+          ax += (double)gx;
+          ay += (double)gy;
+        }
+        if (j < cnt) {
+          float gx = 0.f, gy = 0.f;
+          for (; j < cnt; j += nrep) pair(sf[j], gx, gy);
+          ax += (double)gx;
+          ay += (double)gy;
+        }
+      }
+      wave_sync();
+    }
+  } else {
+    const double tx = L.tx, ty = L.ty;
+    auto pair = [&](const double4 s) {
+      const double rx = tx - s.x, ry = ty - s.y;
+      const double r2 = fma(rx, rx, ry * ry);
+      // 1 / r^2: hardware reciprocal refined by two Newton steps
+      double ri = __builtin_amdgcn_rcp(r2);
+      ri = ri * fma(-r2, ri, 2.0);
+      ri = ri * fma(-r2, ri, 2.0);
+      double fac = s.z * (1.0 - exp(-r2 * s.w)) * ri;
+      fac = r2 > 1e-20 ? fac : 0.0;
+      ax = fma(-fac, ry, ax);
+      ay = fma(fac, rx, ay);
+    };
+    if (nrep == 1) {
+      const double4 *src = a.loc64 + f;
+      int j = 0;
+      for (; j + 3 < cnt; j += 4) {
+        const double4 s0 = ldro(src + j), s1 = ldro(src + j + 1), s2 = ldro(src + j + 2),
+                      s3 = ldro(src + j + 3);
+        pair(s0);
+        pair(s1);
+        pair(s2);
+        pair(s3);
+      }
+      for (; j < cnt; j++) pair(ldro(src + j));
+    } else {
+      if (lane < cnt) st[lane] = a.loc64[f + lane];
+      wave_sync();
+      if (L.tact) {
+        int j = L.rep;
+        for (; j + 3 * nrep < cnt; j += 4 * nrep) {
+          const double4 s0 = st[j], s1 = st[j + nrep], s2 = st[j + 2 * nrep], s3 = st[j + 3 * nrep];
+          pair(s0);
+          pair(s1);
+          pair(s2);
+          pair(s3);
+        }
+        for (; j < cnt; j += nrep) pair(st[j]);
+      }
+      wave_sync();
+    }
+  }
+}
+
+// State of the walk: a 4-bit mask of still-open children per level, the
+// current level, the first child of the current group and the current node.
+struct WalkState {
+  Slots2 sl;
+  int d, cur, node;
+  bool expand;
+};
+
+// Wave-uniform walk of the tree for target leaf L: far(c) for every far
+// cell, near(c, count) for every near leaf (false stops the walk).  Climbing
+// uses the group parent array, so the state is a few words and a walk can
+// be saved and resumed.  Node indices are uniform across the wave
+// (readfirstlane), so node records come through scalar loads.
+template <typename FarFn, typename NearFn>
+__device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, FarFn &&far,
+                            NearFn &&near) {
+  for (;;) {
+    if (ws.expand) {
+      const int base = rfl(ldro(a.nodes + ws.node).first);
+      uint32_t m = 0;
+      for (int q = 0; q < 4; q++) {
+        const int c = base + q;
+        const NodeRec C = ldro(a.nodes + c);
+        if (node_count(C.cnt) == 0) continue;
+        const double dx = C.cx - L.NA.cx, dy = C.cy - L.NA.cy;
+        const double dd = sqrt(dx * dx + dy * dy);
+        const double RC = C.region * 0.70710678118654752;
+        if (L.RA + RC <= a.theta * dd && dd - L.RA - RC >= L.rcut)
+          far(c);
+        else
+          m |= 1u << q;
+      }
+      ws.d++;
+      ws.sl.set(ws.d, m);
+      ws.cur = base;
+    }
+    uint32_t m;
+    for (;;) {
+      m = ws.sl.get(ws.d);
+      if (m) break;
+      if (ws.d == 1) return;
+      ws.cur = rfl(ldro(a.gparent + (ws.cur >> 2)));
+      ws.d--;
+    }
+    const int q = 31 - __clz(m);
+    ws.sl.set(ws.d, m & ~(1u << q));
+    ws.node = rfl((ws.cur & ~3) + q);
+    const int32_t cn = ldro(a.nodes + ws.node).cnt;
+    ws.expand = node_internal(cn);
+    if (!ws.expand && !near(ws.node, node_count(cn))) return;
+  }
+}
+
+// Sum over the replicas of each target; the result is valid in the lanes
+// with rep == 0.  red = this wave's LDS slice of 64 entries.
+__device__ inline double2 reduce_replicas(const LeafCtx &L, int lane, double2 *red, double ax,
+                                          double ay) {
+  wave_sync();
+  red[lane] = make_double2(ax, ay);
+  wave_sync();
+  double sx = 0.0, sy = 0.0;
+  if (L.tact && L.rep == 0) {
+    for (int rr = 0; rr < L.nrep; rr++) {
+      const double2 v = red[rr * L.cA + L.ti];
+      sx += v.x;
+      sy += v.y;
+    }
+  }
+  wave_sync();
+  return make_double2(sx, sy);
+}
+
+// Near work of one chunk: resume the walk at checkpoint r (its leaf first)
+// and sum near leaves until the work count reaches `end`; far cells are
+// skipped (the fused pass took them).
+template <bool MIXED>
+__device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const SplitCk &r, int64_t end,
+                                 int lane, double4 *st, double &ax, double &ay) {
+  WalkState ws;
+  ws.sl.lo = rfl64(r.lo);
+  ws.sl.hi = rfl64(r.hi);
+  ws.d = rfl(r.d);
+  ws.cur = rfl(r.cur);
+  ws.node = rfl(r.node);
+  ws.expand = false;
+  int64_t cum = (int64_t)rfl64((uint64_t)r.cum);
+  const int64_t stop = (int64_t)rfl64((uint64_t)end);
+  near_leaf<MIXED>(a, L, ws.node, lane, st, ax, ay);
+  cum += near_units(node_count(ldro(a.nodes + ws.node).cnt), L.nrep);
+  walk(a, L, ws, [](int) {}, [&](int c, int cnt) -> bool {
+    if (cum >= stop) return false;
+    near_leaf<MIXED>(a, L, c, lane, st, ax, ay);
+    cum += near_units(cnt, L.nrep);
+    return true;
+  });
+}
+
+// The fused pass for target leaf li (index in the leaf list), one wave.
 template <bool MIXED>
 __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
                                  double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
-                                 double4 (*s_stage)[64]) {
-  const int A = a.leaves[li];
-  const NodeRec NA = a.nodes[A];
-  const int cA = node_count(NA.cnt);
-  const double RA = NA.region * 0.70710678118654752;
-  const double sA = NA.region;
-  // this leaf's expansion order and near-field radius
-  int pA = a.p;
-  double rcut = a.rcut;
-  if (a.leaf_par) {
-    const float2 lp = a.leaf_par[li];
-    pA = min(a.p, max(2, (int)lp.x));
-    rcut = (double)lp.y * a.smax;
-  }
-  // lanes: (target ti, replica rep)
-  const int nrep = max(1, 64 / cA);
-  const int ti = lane % cA, rep = lane / cA;
-  const bool tact = rep < nrep;
-  const int tj = NA.first + ti;
-  const double tx = a.sx[tj], ty = a.sy[tj];
-  float lx = 0.f, ly = 0.f;
-  if (MIXED) {
-    const float4 q = a.loc[tj];
-    lx = q.x;
-    ly = q.y;
-  }
+                                 double4 (*s_stage)[64], SplitCk (*s_ck)[kMaxCk]) {
+  li = rfl(li);
+  const LeafCtx L = leaf_ctx<MIXED>(a, li, lane);
+  const NodeRec &NA = L.NA;
+  const double sA = L.sA;
+  const int pA = L.pA;
   double ax = 0.0, ay = 0.0;
   // MFMA lane roles: row / column index i, K slot k
   const int mi = lane & 15, mk = lane >> 4;
@@ -331,11 +624,11 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
       const double dr = C.cx - NA.cx, di = C.cy - NA.cy;
       const double inv = 1.0 / (dr * dr + di * di);
       const double2 wv = make_double2(dr * inv, -di * inv);
-      const double2 wi = cpow_u(wv, mi);                         // w^i
+      const double2 wi = cpow_u(wv, mi);                                   // w^i
       const double2 pw = cmul(cmul(wi, wv), make_double2(sApow.x, 0.0));  // (s_A w)^(i+1)
       const double2 ak = a.coef[(size_t)cell * a.p + mi];
-      double2 q = cmul(ak, wi);                                  // a_i w^i
-      if ((mi & 1) == 0) q = make_double2(-q.x, -q.y);           // (-1)^(i+1)
+      double2 q = cmul(ak, wi);                                            // a_i w^i
+      if ((mi & 1) == 0) q = make_double2(-q.x, -q.y);                     // (-1)^(i+1)
       const bool im = (mk & 1) != 0;
       av = im ? pw.y : pw.x;
       b0 = im ? -q.y : q.x;
@@ -344,177 +637,80 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
     acc.add(av, b0, b1);
     acc.tick();
   };
-
-  // Direct sums against one near leaf.  With one replica (every lane needs
-  // the same source at each step) the source index is uniform and the
-  // sources come through the scalar cache; otherwise the leaf's sources are
-  // staged once in this wave's LDS slice and read from there, so each source
-  // crosses the vector memory path once per wave instead of once per lane.
-  auto near_leaf = [&](int c) {
-    const NodeRec C = a.nodes[c];
-    const int cnt = node_count(C.cnt);
-    const int f = __builtin_amdgcn_readfirstlane(C.first);
-    if (MIXED) {
-      const float dx = (float)(C.cx - NA.cx), dy = (float)(C.cy - NA.cy);
-      const float px = lx - dx, py = ly - dy;
-      float sxs = 0.f, sxc = 0.f, sys = 0.f, syc = 0.f;
-      // pair (target, source s): u += -f ry, v += f rx with
-      // f = cir (1 - exp(-r^2/sigma^2)) / r^2, pairs closer than 1e-10 skipped
-      auto pair = [&](const float4 s, float &gx, float &gy) {
-        const float rx = px - s.x, ry = py - s.y;
-        const float r2 = fmaf(rx, rx, ry * ry);
-        float fac = s.z * one_minus_exp_neg(r2 * s.w) * __builtin_amdgcn_rcpf(r2);
-        fac = r2 > 1e-20f ? fac : 0.0f;
-        gx = fmaf(-fac, ry, gx);
-        gy = fmaf(fac, rx, gy);
-      };
-      // groups of four sources: plain FP32 sum inside a group, compensated
-      // (Kahan) accumulation of the group sums
-      if (nrep == 1) {
-        const float4 *src = a.loc + f;
-        int j = 0;
-        for (; j + 3 < cnt; j += 4) {
-          const float4 s0 = src[j], s1 = src[j + 1], s2 = src[j + 2], s3 = src[j + 3];
-          float gx = 0.f, gy = 0.f;
-          pair(s0, gx, gy);
-          pair(s1, gx, gy);
-          pair(s2, gx, gy);
-          pair(s3, gx, gy);
-          kahan_add(sxs, sxc, gx);
-          kahan_add(sys, syc, gy);
-        }
-        float gx = 0.f, gy = 0.f;
-        for (; j < cnt; j++) pair(src[j], gx, gy);
-        kahan_add(sxs, sxc, gx);
-        kahan_add(sys, syc, gy);
-      } else {
-        float4 *st = reinterpret_cast<float4 *>(s_stage[w]);
-        if (lane < cnt) st[lane] = a.loc[f + lane];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (tact) {
-          int j = rep;
-          for (; j + 3 * nrep < cnt; j += 4 * nrep) {
-            const float4 s0 = st[j], s1 = st[j + nrep], s2 = st[j + 2 * nrep],
-                         s3 = st[j + 3 * nrep];
-            float gx = 0.f, gy = 0.f;
-            pair(s0, gx, gy);
-            pair(s1, gx, gy);
-            pair(s2, gx, gy);
-            pair(s3, gx, gy);
-            kahan_add(sxs, sxc, gx);
-            kahan_add(sys, syc, gy);
-          }
-          float gx = 0.f, gy = 0.f;
-          for (; j < cnt; j += nrep) pair(st[j], gx, gy);
-          kahan_add(sxs, sxc, gx);
-          kahan_add(sys, syc, gy);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
-      ax += (double)sxs - (double)sxc;
-      ay += (double)sys - (double)syc;
+  auto far = [&](int c) {
+    if (pending < 0) {
+      pending = c;
     } else {
-      auto pair = [&](const double4 s) {
-        const double rx = tx - s.x, ry = ty - s.y;
-        const double r2 = fma(rx, rx, ry * ry);
-        // 1 / r^2: hardware reciprocal refined by two Newton steps
-        double ri = __builtin_amdgcn_rcp(r2);
-        ri = ri * fma(-r2, ri, 2.0);
-        ri = ri * fma(-r2, ri, 2.0);
-        double fac = s.z * (1.0 - exp(-r2 * s.w)) * ri;
-        fac = r2 > 1e-20 ? fac : 0.0;
-        ax = fma(-fac, ry, ax);
-        ay = fma(fac, rx, ay);
-      };
-      if (nrep == 1) {
-        const double4 *src = a.loc64 + f;
-        int j = 0;
-        for (; j + 3 < cnt; j += 4) {
-          const double4 s0 = src[j], s1 = src[j + 1], s2 = src[j + 2], s3 = src[j + 3];
-          pair(s0);
-          pair(s1);
-          pair(s2);
-          pair(s3);
-        }
-        for (; j < cnt; j++) pair(src[j]);
-      } else {
-        double4 *st = s_stage[w];
-        if (lane < cnt) st[lane] = a.loc64[f + lane];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (tact) {
-          int j = rep;
-          for (; j + 3 * nrep < cnt; j += 4 * nrep) {
-            const double4 s0 = st[j], s1 = st[j + nrep], s2 = st[j + 2 * nrep],
-                          s3 = st[j + 3 * nrep];
-            pair(s0);
-            pair(s1);
-            pair(s2);
-            pair(s3);
-          }
-          for (; j < cnt; j += nrep) pair(st[j]);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
+      far_pair(pending, c);
+      pending = -1;
     }
   };
 
-  // wave-uniform traversal of the whole tree (masks of opened children per level)
-  // (node indices are uniform across the wave: readfirstlane lets the node
-  // records come through scalar loads)
-  Slots2 sl;
-  int d = 0, cur = 0, node = __builtin_amdgcn_readfirstlane(root_of(a, NA.first));
-  bool expand = true;
-  for (;;) {
-    if (expand) {
-      const int base = __builtin_amdgcn_readfirstlane(a.nodes[node].first);
-      uint32_t m = 0;
-      for (int q = 0; q < 4; q++) {
-        const int c = base + q;
-        const NodeRec C = a.nodes[c];
-        if (node_count(C.cnt) == 0) continue;
-        const double dx = C.cx - NA.cx, dy = C.cy - NA.cy;
-        const double dd = sqrt(dx * dx + dy * dy);
-        const double RC = C.region * 0.70710678118654752;
-        if (RA + RC <= a.theta * dd && dd - RA - RC >= rcut) {
-          if (pending < 0) {
-            pending = c;
-          } else {
-            far_pair(pending, c);
-            pending = -1;
-          }
-        } else {
-          m |= 1u << q;
+  // near work: summed here up to the first threshold, then only counted,
+  // with a checkpoint every `step` units (kept in LDS; when the slots run
+  // out, every second checkpoint is dropped and the step doubles)
+  WalkState ws;
+  ws.d = 0;
+  ws.cur = 0;
+  ws.node = rfl(root_of(a, NA.first));
+  ws.expand = true;
+  const bool split = a.split_budget > 0;
+  int64_t cum = 0, step = split ? a.split_budget : 0;
+  int64_t thr = split ? step : INT64_MAX;
+  int nck = 0;
+  auto near = [&](int c, int cnt) -> bool {
+    if (cum >= thr) {
+      if (nck == kMaxCk) {
+        uint64_t lo = 0, hi = 0;
+        int64_t cu = 0;
+        int rd = 0, rc = 0, rn = 0;
+        if (lane < kMaxCk / 2) {
+          const SplitCk &r = s_ck[w][2 * lane];
+          lo = r.lo;
+          hi = r.hi;
+          cu = r.cum;
+          rd = r.d;
+          rc = r.cur;
+          rn = r.node;
         }
+        wave_sync();
+        if (lane < kMaxCk / 2) {
+          SplitCk &r = s_ck[w][lane];
+          r.lo = lo;
+          r.hi = hi;
+          r.cum = cu;
+          r.end = 0;
+          r.li = li;
+          r.d = rd;
+          r.cur = rc;
+          r.node = rn;
+        }
+        wave_sync();
+        nck = kMaxCk / 2;
+        step *= 2;
+        thr = (int64_t)rfl64((uint64_t)s_ck[w][nck - 1].cum) + step;
       }
-      d++;
-      sl.set(d, m);
-      cur = base;
+      if (cum >= thr) {
+        if (lane == 0) {
+          SplitCk &r = s_ck[w][nck];
+          r.lo = ws.sl.lo;
+          r.hi = ws.sl.hi;
+          r.cum = cum;
+          r.end = 0;
+          r.li = li;
+          r.d = ws.d;
+          r.cur = ws.cur;
+          r.node = ws.node;
+        }
+        nck++;
+        thr = cum + step;
+      }
     }
-    uint32_t m;
-    bool done = false;
-    for (;;) {
-      m = sl.get(d);
-      if (m) break;
-      if (d == 1) { done = true; break; }
-      cur = __builtin_amdgcn_readfirstlane(a.gparent[cur >> 2]);
-      d--;
-    }
-    if (done) break;
-    const int q = 31 - __clz(m);
-    sl.set(d, m & ~(1u << q));
-    node = __builtin_amdgcn_readfirstlane((cur & ~3) + q);
-    const int32_t cn = a.nodes[node].cnt;
-    if (node_internal(cn)) {
-      expand = true;
-    } else {
-      near_leaf(node);
-      expand = false;
-    }
-  }
+    if (nck == 0) near_leaf<MIXED>(a, L, c, lane, s_stage[w], ax, ay);
+    cum += near_units(cnt, L.nrep);
+    return true;
+  };
+  walk(a, L, ws, far, near);
   if (pending >= 0) far_pair(pending, -1);
   acc.fold();
 
@@ -534,38 +730,68 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
     }
     if (mi == 0 && row < kFmmMaxP) s_loc[w][row] = make_double2(gr / sA, gi / sA);
   }
-  // s_loc[w] and s_red[w] belong to this wave alone: a wave-level fence orders
-  // the LDS writes of some lanes before the reads of others
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+  // s_loc[w], s_red[w], s_stage[w] and s_ck[w] belong to this wave alone: a
+  // wave-level fence orders the LDS writes of some lanes before the reads of
+  // others
+  wave_sync();
   double phr = 0.0, phi = 0.0;
-  if (tact && rep == 0) {
+  if (L.tact && L.rep == 0) {
     // L2P: sum_l Lt_l u^l, u = (z - c_A)/s_A
-    const double ur = (tx - NA.cx) / sA, ui = (ty - NA.cy) / sA;
+    const double ur = (L.tx - NA.cx) / sA, ui = (L.ty - NA.cy) / sA;
     double2 s = s_loc[w][pA - 1];
     for (int l = pA - 2; l >= 0; l--) {
-      const double2 L = s_loc[w][l];
-      s = make_double2(s.x * ur - s.y * ui + L.x, s.x * ui + s.y * ur + L.y);
+      const double2 Lc = s_loc[w][l];
+      s = make_double2(s.x * ur - s.y * ui + Lc.x, s.x * ui + s.y * ur + Lc.y);
     }
     phr = s.x;
     phi = s.y;
   }
-  s_red[w][lane] = make_double2(ax, ay);
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  if (tact && rep == 0) {
-    double sxa = 0.0, sya = 0.0;
-    for (int rr = 0; rr < nrep; rr++) {
-      const double2 v = s_red[w][rr * cA + ti];
-      sxa += v.x;
-      sya += v.y;
+  const double2 S = reduce_replicas(L, lane, s_red[w], ax, ay);
+
+  // hand the checkpoints to the continuation pass; without room for them,
+  // run the chunks here with the arithmetic of the summing pass
+  bool inline_chunks = false;
+  double2 extra = make_double2(0.0, 0.0);
+  if (split) {
+    int2 rec = make_int2(0, 0);
+    if (nck > 0) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(a.ck_count, (uint32_t)nck);
+      base = (uint32_t)rfl((int)__shfl((int)base, 0, 64));
+      const bool fits = (int64_t)base + nck <= (int64_t)a.ck_cap;
+      if (lane < nck && (int64_t)base + lane < (int64_t)a.ck_cap) {
+        SplitCk r = s_ck[w][lane];
+        r.end = lane + 1 < nck ? s_ck[w][lane + 1].cum : INT64_MAX;
+        if (!fits) r.li = -1;  // the continuation pass skips these
+        a.ck[base + lane] = r;
+      }
+      if (fits) {
+        rec = make_int2((int)base, nck);
+      } else {
+        inline_chunks = true;
+        for (int k = 0; k < nck; k++) {
+          const SplitCk r = s_ck[w][k];
+          const int64_t end = k + 1 < nck ? s_ck[w][k + 1].cum : INT64_MAX;
+          double cx = 0.0, cy = 0.0;
+          run_chunk<MIXED>(a, L, r, end, lane, s_stage[w], cx, cy);
+          const double2 P = reduce_replicas(L, lane, s_red[w], cx, cy);
+          extra.x += P.x;
+          extra.y += P.y;
+        }
+      }
     }
-    const double u = 0.5 * phi / M_PI + 0.5 * sxa / M_PI;
-    const double v = 0.5 * phr / M_PI + 0.5 * sya / M_PI;
-    a.uv[a.perm[tj]] = make_double2(u, v);
+    if (lane == 0) a.leaf_ck[li] = rec;
   }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+  if (L.tact && L.rep == 0) {
+    double u = 0.5 * phi / M_PI + 0.5 * S.x / M_PI;
+    double v = 0.5 * phr / M_PI + 0.5 * S.y / M_PI;
+    if (inline_chunks) {
+      u = u + 0.5 * extra.x / M_PI;
+      v = v + 0.5 * extra.y / M_PI;
+    }
+    a.uv[a.perm[L.tj]] = make_double2(u, v);
+  }
+  wave_sync();
 }
 
 // One wave per target leaf, leaf = 4 * block + wave.
@@ -574,10 +800,11 @@ __global__ __launch_bounds__(256) void fmm_leaf_kernel(FmmArgs a) {
   __shared__ double2 s_loc[4][kFmmMaxP];
   __shared__ double2 s_red[4][64];
   __shared__ double4 s_stage[4][64];
+  __shared__ SplitCk s_ck[4][kMaxCk];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = blockIdx.x * 4 + w;
   if (li >= a.nleaves) return;
-  fmm_process_leaf<MIXED>(a, li, w, lane, s_loc, s_red, s_stage);
+  fmm_process_leaf<MIXED>(a, li, w, lane, s_loc, s_red, s_stage, s_ck);
 }
 
 // Persistent form: a fixed grid whose waves take target leaves from a work
@@ -592,6 +819,7 @@ __global__ __launch_bounds__(256) void fmm_leaf_persistent_kernel(FmmArgs a,
   __shared__ double2 s_loc[4][kFmmMaxP];
   __shared__ double2 s_red[4][64];
   __shared__ double4 s_stage[4][64];
+  __shared__ SplitCk s_ck[4][kMaxCk];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = *nleaves;
   for (;;) {
@@ -599,21 +827,93 @@ __global__ __launch_bounds__(256) void fmm_leaf_persistent_kernel(FmmArgs a,
     if (lane == 0) li = atomicAdd(counter, 1u);
     li = __shfl(li, 0, 64);
     if (li >= total) break;
-    fmm_process_leaf<MIXED>(a, (int)li, w, lane, s_loc, s_red, s_stage);
+    fmm_process_leaf<MIXED>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck);
   }
 }
 
-__global__ void zero_counter_kernel(uint32_t *c) { *c = 0u; }
+// Continuation pass: waves take checkpoint records from a work counter; the
+// partial sums of record j go to part[64 j + target].  Exits when the counter
+// passes the number of records written by the fused pass.
+template <bool MIXED>
+__global__ __launch_bounds__(256) void fmm_chunk_kernel(FmmArgs a, uint32_t *__restrict__ counter) {
+  __shared__ double2 s_red[4][64];
+  __shared__ double4 s_stage[4][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t total = min(*a.ck_count, (uint32_t)a.ck_cap);
+  for (;;) {
+    uint32_t j = 0;
+    if (lane == 0) j = atomicAdd(counter, 1u);
+    j = (uint32_t)rfl((int)__shfl((int)j, 0, 64));
+    if (j >= total) break;
+    const SplitCk r = ldro(a.ck + j);
+    const int li = rfl(r.li);
+    if (li < 0) continue;
+    const LeafCtx L = leaf_ctx<MIXED>(a, li, lane);
+    double ax = 0.0, ay = 0.0;
+    run_chunk<MIXED>(a, L, r, r.end, lane, s_stage[w], ax, ay);
+    const double2 P = reduce_replicas(L, lane, s_red[w], ax, ay);
+    if (L.tact && L.rep == 0) a.part[(size_t)j * 64 + L.ti] = P;
+  }
+}
+
+// Adds the chunk sums of every split leaf to its targets, in chunk order.
+__global__ __launch_bounds__(256) void fmm_split_sum_kernel(FmmArgs a,
+                                                            const uint32_t *__restrict__ nleaves) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t total = *nleaves;
+  const uint32_t nw = gridDim.x * 4;
+  for (uint32_t li = blockIdx.x * 4 + (threadIdx.x >> 6); li < total; li += nw) {
+    const int2 k = a.leaf_ck[li];
+    if (k.y == 0) continue;
+    const NodeRec NA = a.nodes[a.leaves[li]];
+    if (lane < node_count(NA.cnt)) {
+      double sx = 0.0, sy = 0.0;
+      for (int i = 0; i < k.y; i++) {
+        const double2 P = a.part[(size_t)(k.x + i) * 64 + lane];
+        sx += P.x;
+        sy += P.y;
+      }
+      const uint32_t t = a.perm[NA.first + lane];
+      double2 uv = a.uv[t];
+      uv.x = uv.x + 0.5 * sx / M_PI;
+      uv.y = uv.y + 0.5 * sy / M_PI;
+      a.uv[t] = uv;
+    }
+  }
+}
+
+__global__ void zero_counters_kernel(uint32_t *c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+static int launch_split_passes(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
+                               uint32_t *d_chunk_counter, int nblocks, hipStream_t s) {
+  if (mixed)
+    fmm_chunk_kernel<true><<<nblocks, 256, 0, s>>>(args, d_chunk_counter);
+  else
+    fmm_chunk_kernel<false><<<nblocks, 256, 0, s>>>(args, d_chunk_counter);
+  FMM_HIP_TRY(hipGetLastError());
+  fmm_split_sum_kernel<<<nblocks, 256, 0, s>>>(args, d_nleaves);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
 
 int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
-                               uint32_t *d_counter, int nblocks, hipStream_t s) {
-  zero_counter_kernel<<<1, 1, 0, s>>>(d_counter);
+                               uint32_t *d_counter, int nblocks, uint32_t *d_split_ctr,
+                               int split_blocks, hipStream_t s) {
+  zero_counters_kernel<<<1, 64, 0, s>>>(d_counter, 1);
   FMM_HIP_TRY(hipGetLastError());
+  const bool split = args.split_budget > 0;
+  if (split) {
+    zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 2);
+    FMM_HIP_TRY(hipGetLastError());
+  }
   if (mixed)
     fmm_leaf_persistent_kernel<true><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
   else
     fmm_leaf_persistent_kernel<false><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
   FMM_HIP_TRY(hipGetLastError());
+  if (split) return launch_split_passes(args, mixed, d_nleaves, d_split_ctr + 1, split_blocks, s);
   return 0;
 }
 
@@ -680,14 +980,21 @@ int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s) {
   return 0;
 }
 
-int launch_fmm_eval(const FmmArgs &args, bool mixed, hipStream_t s) {
+int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
+                    uint32_t *d_split_ctr, int split_blocks, hipStream_t s) {
   if (args.nleaves <= 0) return 0;
+  const bool split = args.split_budget > 0;
+  if (split) {
+    zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 2);
+    FMM_HIP_TRY(hipGetLastError());
+  }
   const int nb = (args.nleaves + 3) / 4;
   if (mixed)
     fmm_leaf_kernel<true><<<nb, 256, 0, s>>>(args);
   else
     fmm_leaf_kernel<false><<<nb, 256, 0, s>>>(args);
   FMM_HIP_TRY(hipGetLastError());
+  if (split) return launch_split_passes(args, mixed, d_nleaves, d_split_ctr + 1, split_blocks, s);
   return 0;
 }
 

@@ -109,6 +109,13 @@ int fmm_make_forest_device(int nsys, const int32_t *offsets, const double *d_xy,
  * take target leaves from a work counter (0 = one workgroup per four leaves),
  * optionally replayed from a captured HIP graph (use_graph = 1). */
 int fmm_set_persistent(int blocks, int use_graph);
+/* Accurate modes: target leaves with more than `budget` units of near work
+ * (a unit = one wave step over 64 target-source pairs, plus 2 per near
+ * leaf) are split into chunks of at least `budget` units that run on other
+ * waves; their partial sums are added in chunk order, so results are
+ * deterministic.  0 turns splitting off.  Default 8192. */
+int fmm_set_split(int budget);
+int fmm_get_split(void);
 /* Compute units of the current device (for sizing persistent grids). */
 int fmm_device_cus(void);
 
