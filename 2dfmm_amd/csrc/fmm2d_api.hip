@@ -116,6 +116,7 @@ struct Context {
   // splitting of target leaves with much near work (0: off)
   int split_budget = 8192;
   DevBuf<SplitCk> ck;
+  DevBuf<int32_t> nfirst;
   DevBuf<int2> leaf_ck;
   DevBuf<double2> part;
   DevBuf<uint32_t> split_ctr;
@@ -360,6 +361,8 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
                           g.loc.p, g.loc64.p, s))
       return -1;
+    if (g.nfirst.ensure(g.nnodes, s) || launch_node_first_blob(g.nodes.p, g.nnodes, g.nfirst.p, s))
+      return -1;
     // valid already for the fused kernel's arguments used by the learned step
     g.valid = true;
     if (g.learned && apply_learned()) {
@@ -418,6 +421,7 @@ static FmmArgs fmm_args(double *d_uv) {
   a.root_shift = g.root_shift;
   a.leaf_par = g.leaf_par_on ? g.leaf_par.p : nullptr;
   a.smax = g.smax_h;
+  a.nfirst = g.nfirst.p;
   a.split_budget = g.split_budget;
   a.ck_cap = (int)g.ck.cap;
   a.ck = g.ck.p;
@@ -1513,6 +1517,7 @@ void fmm_release(void) {
   g.lflag.release(); g.lpos.release(); g.direct_ws.release();
   g.doff.release(); g.ntl_dev.release(); g.counter.release();
   g.ck.release(); g.leaf_ck.release(); g.part.release(); g.split_ctr.release();
+  g.nfirst.release();
   if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);
   if (g.graph) (void)hipGraphDestroy(g.graph);
   g.graph_exec = nullptr;

@@ -120,12 +120,13 @@ int launch_direct(int ns, const double *d_sxy, const double *d_cir, const double
 
 // ---- accurate fast mode (fmm_kernels.hip) ------------------------------
 // Saved walk of a target leaf with much near work: the continuation pass
-// resumes it at leaf `node` and sums near leaves while the work count is
-// below `end`.
+// resumes it at near item `node` (a leaf, or a whole near subtree from its
+// tile `off`) and sums near steps while the work count is below `end`.
 struct SplitCk {
   uint64_t lo, hi;   // open-child masks, 4 bits per level
   int64_t cum, end;  // near work units before this point; end of the chunk
-  int32_t li, d, cur, node;  // target leaf (-1: skip), level, group, leaf
+  int32_t li, d, cur, node;  // target leaf (-1: skip), level, group, item
+  int32_t off, pad;          // tile of the item
 };
 
 struct FmmArgs {
@@ -149,6 +150,8 @@ struct FmmArgs {
   // null = p and rcut for every leaf.  smax = largest core radius.
   const float2 *leaf_par;
   double smax;
+  // first blob (in leaf order) of every node's subtree
+  const int32_t *nfirst;
   // splitting of target leaves with much near work (split_budget 0: off):
   // checkpoint records (ck_cap of them; *ck_count taken), per target leaf
   // (first record, records), 64 partial sums per record
@@ -168,6 +171,8 @@ int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleav
 int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, uint32_t *d_flag,
                      uint32_t *d_pos, int32_t *d_leaves, uint32_t *d_count, hipStream_t s);
 int launch_max_reduce(const double *d_v, int n, double *d_out, hipStream_t s);
+// First blob of every node's subtree: the first blob of its leftmost leaf.
+int launch_node_first_blob(const NodeRec *d_nodes, int nnodes, int32_t *d_out, hipStream_t s);
 // One wave per target leaf; with splitting on, d_split_ctr holds two counters
 // and the continuation passes run on split_blocks x 256 threads.
 int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,

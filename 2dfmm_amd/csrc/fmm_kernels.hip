@@ -206,6 +206,25 @@ __global__ __launch_bounds__(256) void max_reduce_kernel(const double *__restric
   }
 }
 
+// Child 0 of a node starts at the node's first blob, so following child 0
+// down to a leaf gives the subtree's first blob.
+__global__ __launch_bounds__(256) void node_first_blob_kernel(const NodeRec *__restrict__ nodes,
+                                                              int nnodes,
+                                                              int32_t *__restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nnodes) return;
+  NodeRec c = nodes[i];
+  while (node_internal(c.cnt)) c = nodes[c.first];
+  out[i] = c.first;
+}
+
+int launch_node_first_blob(const NodeRec *d_nodes, int nnodes, int32_t *d_out, hipStream_t s) {
+  if (nnodes <= 0) return 0;
+  node_first_blob_kernel<<<(nnodes + 255) / 256, 256, 0, s>>>(d_nodes, nnodes, d_out);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int launch_max_reduce(const double *d_v, int n, double *d_out, hipStream_t s) {
   FMM_HIP_TRY(hipMemsetAsync(d_out, 0, sizeof(double), s));
   int nb = (n + 255) / 256;
@@ -366,41 +385,95 @@ __device__ inline LeafCtx leaf_ctx(const FmmArgs &a, int li, int lane) {
   return L;
 }
 
-// Work units of one near leaf of cnt sources: wave steps plus a fixed cost.
+// Work units of one near step over cnt sources: wave steps plus a fixed
+// cost for the step's loads and synchronisation.
+constexpr int kStepUnits = 8;
 __device__ inline int64_t near_units(int cnt, int nrep) {
-  return (int64_t)((cnt + nrep - 1) / nrep) + 2;
+  return (int64_t)((cnt + nrep - 1) / nrep) + kStepUnits;
 }
 
-// Direct sums of the targets of L against near leaf c.  With one replica
+// One pair (target, source s) of the mixed form, target at (px, py) and
+// source at (s.x, s.y) in the same FP32 frame: u += -f ry, v += f rx with
+// f = z h(y), y = r^2 s.w (see h_series).
+__device__ inline void pair32(float px, float py, const float4 s, float &gx, float &gy) {
+  const float rx = px - s.x, ry = py - s.y;
+  const float r2 = fmaf(rx, rx, ry * ry);
+  const float y = r2 * s.w;
+  float h = (1.0f - __builtin_amdgcn_exp2f(-y)) * __builtin_amdgcn_rcpf(y);
+  if (__builtin_amdgcn_ballot_w64(y < kSeriesY)) {
+    if (y < kSeriesY) h = r2 > 1e-20f ? h_series(y) : 0.0f;
+  }
+  const float fac = s.z * h;
+  gx = fmaf(-fac, ry, gx);
+  gy = fmaf(fac, rx, gy);
+}
+
+// One pair of the plain form, FP64 (x, y, cir, 1/sigma^2); the direct sum's
+// rule: pairs closer than 1e-10 are skipped.
+__device__ inline void pair64(double tx, double ty, const double4 s, double &ax, double &ay) {
+  const double rx = tx - s.x, ry = ty - s.y;
+  const double r2 = fma(rx, rx, ry * ry);
+  // 1 / r^2: hardware reciprocal refined by two Newton steps
+  double ri = __builtin_amdgcn_rcp(r2);
+  ri = ri * fma(-r2, ri, 2.0);
+  ri = ri * fma(-r2, ri, 2.0);
+  double fac = s.z * (1.0 - exp(-r2 * s.w)) * ri;
+  fac = r2 > 1e-20 ? fac : 0.0;
+  ax = fma(-fac, ry, ax);
+  ay = fma(fac, rx, ay);
+}
+
+// Sources st[rep], st[rep + nrep], ... (< cnt) of an LDS slice; groups of
+// four pairs are summed in FP32 and accumulated in FP64.
+__device__ inline void lds_sum32(float px, float py, const float4 *st, int cnt, int rep, int nrep,
+                                 double &ax, double &ay) {
+  int j = rep;
+  for (; j + 3 * nrep < cnt; j += 4 * nrep) {
+    const float4 s0 = st[j], s1 = st[j + nrep], s2 = st[j + 2 * nrep], s3 = st[j + 3 * nrep];
+    float gx = 0.f, gy = 0.f;
+    pair32(px, py, s0, gx, gy);
+    pair32(px, py, s1, gx, gy);
+    pair32(px, py, s2, gx, gy);
+    pair32(px, py, s3, gx, gy);
+    ax += (double)gx;
+    ay += (double)gy;
+  }
+  if (j < cnt) {
+    float gx = 0.f, gy = 0.f;
+    for (; j < cnt; j += nrep) pair32(px, py, st[j], gx, gy);
+    ax += (double)gx;
+    ay += (double)gy;
+  }
+}
+
+__device__ inline void lds_sum64(double tx, double ty, const double4 *st, int cnt, int rep,
+                                 int nrep, double &ax, double &ay) {
+  int j = rep;
+  for (; j + 3 * nrep < cnt; j += 4 * nrep) {
+    const double4 s0 = st[j], s1 = st[j + nrep], s2 = st[j + 2 * nrep], s3 = st[j + 3 * nrep];
+    pair64(tx, ty, s0, ax, ay);
+    pair64(tx, ty, s1, ax, ay);
+    pair64(tx, ty, s2, ax, ay);
+    pair64(tx, ty, s3, ax, ay);
+  }
+  for (; j < cnt; j += nrep) pair64(tx, ty, st[j], ax, ay);
+}
+
+// Direct sums of the targets of L against near leaf C.  With one replica
 // (every lane needs the same source at each step) the source index is
 // uniform and the sources come through the scalar cache; otherwise the
 // leaf's sources are staged once in this wave's LDS slice `st` and read from
 // there, so each source crosses the vector memory path once per wave.
 template <bool MIXED>
-__device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, int c, int lane, double4 *st,
-                                 double &ax, double &ay) {
-  const NodeRec C = ldro(a.nodes + c);
+__device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeRec &C, int lane,
+                                 double4 *st, double &ax, double &ay) {
   const int cnt = node_count(C.cnt);
   const int f = rfl(C.first);
   const int nrep = L.nrep;
   if (MIXED) {
+    // sources are stored relative to their own leaf's centre
     const float dx = (float)(C.cx - L.NA.cx), dy = (float)(C.cy - L.NA.cy);
     const float px = L.lx - dx, py = L.ly - dy;
-    // pair (target, source s): u += -f ry, v += f rx with f = z h(y),
-    // y = r^2 s.w (see h_series)
-    auto pair = [&](const float4 s, float &gx, float &gy) {
-      const float rx = px - s.x, ry = py - s.y;
-      const float r2 = fmaf(rx, rx, ry * ry);
-      const float y = r2 * s.w;
-      float h = (1.0f - __builtin_amdgcn_exp2f(-y)) * __builtin_amdgcn_rcpf(y);
-      if (__builtin_amdgcn_ballot_w64(y < kSeriesY)) {
-        if (y < kSeriesY) h = r2 > 1e-20f ? h_series(y) : 0.0f;
-      }
-      const float fac = s.z * h;
-      gx = fmaf(-fac, ry, gx);
-      gy = fmaf(fac, rx, gy);
-    };
-    // groups of four sources: FP32 sum inside a group, FP64 across groups
     if (nrep == 1) {
       const float4 *src = a.loc + f;
       int j = 0;
@@ -416,17 +489,17 @@ __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, int c, int 
             n3 = ldro(src + j + 7);
           }
           float gx = 0.f, gy = 0.f;
-          pair(s0, gx, gy);
-          pair(s1, gx, gy);
-          pair(s2, gx, gy);
-          pair(s3, gx, gy);
+          pair32(px, py, s0, gx, gy);
+          pair32(px, py, s1, gx, gy);
+          pair32(px, py, s2, gx, gy);
+          pair32(px, py, s3, gx, gy);
           ax += (double)gx;
           ay += (double)gy;
         }
       }
       if (j < cnt) {
         float gx = 0.f, gy = 0.f;
-        for (; j < cnt; j++) pair(ldro(src + j), gx, gy);
+        for (; j < cnt; j++) pair32(px, py, ldro(src + j), gx, gy);
         ax += (double)gx;
         ay += (double)gy;
       }
@@ -434,70 +507,102 @@ __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, int c, int 
       float4 *sf = reinterpret_cast<float4 *>(st);
       if (lane < cnt) sf[lane] = a.loc[f + lane];
       wave_sync();
-      if (L.tact) {
-        int j = L.rep;
-        for (; j + 3 * nrep < cnt; j += 4 * nrep) {
-          const float4 s0 = sf[j], s1 = sf[j + nrep], s2 = sf[j + 2 * nrep], s3 = sf[j + 3 * nrep];
-          float gx = 0.f, gy = 0.f;
-          pair(s0, gx, gy);
-          pair(s1, gx, gy);
-          pair(s2, gx, gy);
-          pair(s3, gx, gy);
-          ax += (double)gx;
-          ay += (double)gy;
-        }
-        if (j < cnt) {
-          float gx = 0.f, gy = 0.f;
-          for (; j < cnt; j += nrep) pair(sf[j], gx, gy);
-          ax += (double)gx;
-          ay += (double)gy;
-        }
-      }
+      if (L.tact) lds_sum32(px, py, sf, cnt, L.rep, nrep, ax, ay);
       wave_sync();
     }
   } else {
-    const double tx = L.tx, ty = L.ty;
-    auto pair = [&](const double4 s) {
-      const double rx = tx - s.x, ry = ty - s.y;
-      const double r2 = fma(rx, rx, ry * ry);
-      // 1 / r^2: hardware reciprocal refined by two Newton steps
-      double ri = __builtin_amdgcn_rcp(r2);
-      ri = ri * fma(-r2, ri, 2.0);
-      ri = ri * fma(-r2, ri, 2.0);
-      double fac = s.z * (1.0 - exp(-r2 * s.w)) * ri;
-      fac = r2 > 1e-20 ? fac : 0.0;
-      ax = fma(-fac, ry, ax);
-      ay = fma(fac, rx, ay);
-    };
     if (nrep == 1) {
       const double4 *src = a.loc64 + f;
       int j = 0;
       for (; j + 3 < cnt; j += 4) {
         const double4 s0 = ldro(src + j), s1 = ldro(src + j + 1), s2 = ldro(src + j + 2),
                       s3 = ldro(src + j + 3);
-        pair(s0);
-        pair(s1);
-        pair(s2);
-        pair(s3);
+        pair64(L.tx, L.ty, s0, ax, ay);
+        pair64(L.tx, L.ty, s1, ax, ay);
+        pair64(L.tx, L.ty, s2, ax, ay);
+        pair64(L.tx, L.ty, s3, ax, ay);
       }
-      for (; j < cnt; j++) pair(ldro(src + j));
+      for (; j < cnt; j++) pair64(L.tx, L.ty, ldro(src + j), ax, ay);
     } else {
       if (lane < cnt) st[lane] = a.loc64[f + lane];
       wave_sync();
-      if (L.tact) {
-        int j = L.rep;
-        for (; j + 3 * nrep < cnt; j += 4 * nrep) {
-          const double4 s0 = st[j], s1 = st[j + nrep], s2 = st[j + 2 * nrep], s3 = st[j + 3 * nrep];
-          pair(s0);
-          pair(s1);
-          pair(s2);
-          pair(s3);
-        }
-        for (; j < cnt; j += nrep) pair(st[j]);
-      }
+      if (L.tact) lds_sum64(L.tx, L.ty, st, cnt, L.rep, nrep, ax, ay);
       wave_sync();
     }
   }
+}
+
+// Direct sums of the targets of L against tiles ta..tb-1 (64 sources each)
+// of near subtree c, whose sources are one contiguous run spanning many
+// leaves.  Each tile is staged in LDS -- the mixed form converts each source
+// once, to FP32 coordinates relative to the target leaf's centre -- and the
+// next tile's loads are issued before the current tile's sums.
+template <bool MIXED>
+__device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
+                                  int ta, int tb, int lane, double4 *st, double &ax, double &ay) {
+  const int s0 = rfl(ldro(a.nfirst + c));
+  const int cnt = node_count(C.cnt);
+  double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
+  float2 zw = make_float2(0.f, 0.f);
+  auto load = [&](int t) {
+    const int i = 64 * t + lane;
+    if (i < cnt) {
+      q = a.loc64[s0 + i];
+      if (MIXED) {
+        const float4 p = a.loc[s0 + i];
+        zw = make_float2(p.z, p.w);
+      }
+    }
+  };
+  load(ta);
+  for (int t = ta; t < tb; t++) {
+    const int tc = min(64, cnt - 64 * t);
+    if (MIXED) {
+      float4 *sf = reinterpret_cast<float4 *>(st);
+      if (lane < tc)
+        sf[lane] = make_float4((float)(q.x - L.NA.cx), (float)(q.y - L.NA.cy), zw.x, zw.y);
+      wave_sync();
+      if (t + 1 < tb) load(t + 1);
+      if (L.tact) lds_sum32(L.lx, L.ly, sf, tc, L.rep, L.nrep, ax, ay);
+    } else {
+      if (lane < tc) st[lane] = q;
+      wave_sync();
+      if (t + 1 < tb) load(t + 1);
+      if (L.tact) lds_sum64(L.tx, L.ty, st, tc, L.rep, L.nrep, ax, ay);
+    }
+    wave_sync();
+  }
+}
+
+// A node C all of whose descendants fail the far test is summed as one run
+// of sources.  For a descendant C' of C, |c' - c_A| <= d + R_C - R_C' < d + R_C
+// =: reach, so
+//   d' - R_A - R_C' < r_cut      when reach - R_A <= r_cut, and
+//   R_A + R_C' > theta d'         when theta reach <= R_A.
+__device__ inline bool whole_near(const FmmArgs &a, const LeafCtx &L, const NodeRec &C) {
+  const double dx = C.cx - L.NA.cx, dy = C.cy - L.NA.cy;
+  const double reach = sqrt(dx * dx + dy * dy) + C.region * 0.70710678118654752;
+  return reach - L.RA <= L.rcut || a.theta * reach <= L.RA;
+}
+
+// Near items are leaves (one step) and whole near subtrees (one step per
+// tile of 64 sources).
+__device__ inline int item_tiles(const NodeRec &C, bool range) {
+  return range ? (node_count(C.cnt) + 63) >> 6 : 1;
+}
+__device__ inline int tile_count(const NodeRec &C, bool range, int t) {
+  return range ? min(64, node_count(C.cnt) - 64 * t) : node_count(C.cnt);
+}
+
+// Steps ta..tb-1 of near item c.
+template <bool MIXED>
+__device__ inline void near_steps(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
+                                  bool range, int ta, int tb, int lane, double4 *st, double &ax,
+                                  double &ay) {
+  if (range)
+    near_range<MIXED>(a, L, c, C, ta, tb, lane, st, ax, ay);
+  else
+    near_leaf<MIXED>(a, L, C, lane, st, ax, ay);
 }
 
 // State of the walk: a 4-bit mask of still-open children per level, the
@@ -509,10 +614,11 @@ struct WalkState {
 };
 
 // Wave-uniform walk of the tree for target leaf L: far(c) for every far
-// cell, near(c, count) for every near leaf (false stops the walk).  Climbing
-// uses the group parent array, so the state is a few words and a walk can
-// be saved and resumed.  Node indices are uniform across the wave
-// (readfirstlane), so node records come through scalar loads.
+// cell, near(c, C, range) for every near item -- a leaf, or with range set a
+// whole near subtree (false stops the walk).  Climbing uses the group parent
+// array, so the state is a few words and a walk can be saved and resumed.
+// Node indices are uniform across the wave (readfirstlane), so node records
+// come through scalar loads.
 template <typename FarFn, typename NearFn>
 __device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, FarFn &&far,
                             NearFn &&near) {
@@ -547,9 +653,10 @@ __device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, F
     const int q = 31 - __clz(m);
     ws.sl.set(ws.d, m & ~(1u << q));
     ws.node = rfl((ws.cur & ~3) + q);
-    const int32_t cn = ldro(a.nodes + ws.node).cnt;
-    ws.expand = node_internal(cn);
-    if (!ws.expand && !near(ws.node, node_count(cn))) return;
+    const NodeRec C = ldro(a.nodes + ws.node);
+    const bool range = node_internal(C.cnt) && whole_near(a, L, C);
+    ws.expand = node_internal(C.cnt) && !range;
+    if (!ws.expand && !near(ws.node, C, range)) return;
   }
 }
 
@@ -572,9 +679,9 @@ __device__ inline double2 reduce_replicas(const LeafCtx &L, int lane, double2 *r
   return make_double2(sx, sy);
 }
 
-// Near work of one chunk: resume the walk at checkpoint r (its leaf first)
-// and sum near leaves until the work count reaches `end`; far cells are
-// skipped (the fused pass took them).
+// Near work of one chunk: resume the walk at checkpoint r (its item, from
+// tile r.off) and sum near steps until the work count reaches `end`; far
+// cells are skipped (the fused pass took them).
 template <bool MIXED>
 __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const SplitCk &r, int64_t end,
                                  int lane, double4 *st, double &ax, double &ay) {
@@ -587,13 +694,21 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const Split
   ws.expand = false;
   int64_t cum = (int64_t)rfl64((uint64_t)r.cum);
   const int64_t stop = (int64_t)rfl64((uint64_t)end);
-  near_leaf<MIXED>(a, L, ws.node, lane, st, ax, ay);
-  cum += near_units(node_count(ldro(a.nodes + ws.node).cnt), L.nrep);
-  walk(a, L, ws, [](int) {}, [&](int c, int cnt) -> bool {
-    if (cum >= stop) return false;
-    near_leaf<MIXED>(a, L, c, lane, st, ax, ay);
-    cum += near_units(cnt, L.nrep);
-    return true;
+  // steps of an item from tile t0 on that start below `stop`
+  auto item = [&](int c, const NodeRec &C, bool range, int t0) -> bool {
+    const int nt = item_tiles(C, range);
+    int te = t0;
+    int64_t cu = cum;
+    while (te < nt && cu < stop) cu += near_units(tile_count(C, range, te++), L.nrep);
+    if (te > t0) near_steps<MIXED>(a, L, c, C, range, t0, te, lane, st, ax, ay);
+    cum = cu;
+    return te == nt;
+  };
+  // the checkpoint's item is a leaf or a whole near subtree (internal)
+  const NodeRec C0 = ldro(a.nodes + ws.node);
+  if (!item(ws.node, C0, node_internal(C0.cnt), rfl(r.off))) return;
+  walk(a, L, ws, [](int) {}, [&](int c, const NodeRec &C, bool range) -> bool {
+    return item(c, C, range, 0);
   });
 }
 
@@ -658,56 +773,69 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
   int64_t cum = 0, step = split ? a.split_budget : 0;
   int64_t thr = split ? step : INT64_MAX;
   int nck = 0;
-  auto near = [&](int c, int cnt) -> bool {
-    if (cum >= thr) {
-      if (nck == kMaxCk) {
-        uint64_t lo = 0, hi = 0;
-        int64_t cu = 0;
-        int rd = 0, rc = 0, rn = 0;
-        if (lane < kMaxCk / 2) {
-          const SplitCk &r = s_ck[w][2 * lane];
-          lo = r.lo;
-          hi = r.hi;
-          cu = r.cum;
-          rd = r.d;
-          rc = r.cur;
-          rn = r.node;
-        }
-        wave_sync();
-        if (lane < kMaxCk / 2) {
-          SplitCk &r = s_ck[w][lane];
-          r.lo = lo;
-          r.hi = hi;
-          r.cum = cu;
-          r.end = 0;
-          r.li = li;
-          r.d = rd;
-          r.cur = rc;
-          r.node = rn;
-        }
-        wave_sync();
-        nck = kMaxCk / 2;
-        step *= 2;
-        thr = (int64_t)rfl64((uint64_t)s_ck[w][nck - 1].cum) + step;
-      }
-      if (cum >= thr) {
-        if (lane == 0) {
-          SplitCk &r = s_ck[w][nck];
-          r.lo = ws.sl.lo;
-          r.hi = ws.sl.hi;
-          r.cum = cum;
-          r.end = 0;
-          r.li = li;
-          r.d = ws.d;
-          r.cur = ws.cur;
-          r.node = ws.node;
-        }
-        nck++;
-        thr = cum + step;
-      }
+  auto near = [&](int c, const NodeRec &C, bool range) -> bool {
+    const int nt = item_tiles(C, range);
+    int t = 0;
+    if (nck == 0) {
+      // the steps that start below the first threshold are summed here
+      int64_t cu = cum;
+      while (t < nt && cu < thr) cu += near_units(tile_count(C, range, t++), L.nrep);
+      if (t > 0) near_steps<MIXED>(a, L, c, C, range, 0, t, lane, s_stage[w], ax, ay);
+      cum = cu;
     }
-    if (nck == 0) near_leaf<MIXED>(a, L, c, lane, s_stage[w], ax, ay);
-    cum += near_units(cnt, L.nrep);
+    for (; t < nt; t++) {
+      if (cum >= thr) {
+        if (nck == kMaxCk) {
+          uint64_t lo = 0, hi = 0;
+          int64_t cu = 0;
+          int rd = 0, rc = 0, rn = 0, ro = 0;
+          if (lane < kMaxCk / 2) {
+            const SplitCk &r = s_ck[w][2 * lane];
+            lo = r.lo;
+            hi = r.hi;
+            cu = r.cum;
+            rd = r.d;
+            rc = r.cur;
+            rn = r.node;
+            ro = r.off;
+          }
+          wave_sync();
+          if (lane < kMaxCk / 2) {
+            SplitCk &r = s_ck[w][lane];
+            r.lo = lo;
+            r.hi = hi;
+            r.cum = cu;
+            r.end = 0;
+            r.li = li;
+            r.d = rd;
+            r.cur = rc;
+            r.node = rn;
+            r.off = ro;
+          }
+          wave_sync();
+          nck = kMaxCk / 2;
+          step *= 2;
+          thr = (int64_t)rfl64((uint64_t)s_ck[w][nck - 1].cum) + step;
+        }
+        if (cum >= thr) {
+          if (lane == 0) {
+            SplitCk &r = s_ck[w][nck];
+            r.lo = ws.sl.lo;
+            r.hi = ws.sl.hi;
+            r.cum = cum;
+            r.end = 0;
+            r.li = li;
+            r.d = ws.d;
+            r.cur = ws.cur;
+            r.node = ws.node;
+            r.off = t;
+          }
+          nck++;
+          thr = cum + step;
+        }
+      }
+      cum += near_units(tile_count(C, range, t), L.nrep);
+    }
     return true;
   };
   walk(a, L, ws, far, near);
@@ -794,9 +922,11 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
   wave_sync();
 }
 
-// One wave per target leaf, leaf = 4 * block + wave.
+// One wave per target leaf, leaf = 4 * block + wave.  At least three waves
+// per SIMD: the walk is latency-bound and needs the others to hide it.
 template <bool MIXED>
-__global__ __launch_bounds__(256) void fmm_leaf_kernel(FmmArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fmm_leaf_kernel(
+    FmmArgs a) {
   __shared__ double2 s_loc[4][kFmmMaxP];
   __shared__ double2 s_red[4][64];
   __shared__ double4 s_stage[4][64];
@@ -813,7 +943,7 @@ __global__ __launch_bounds__(256) void fmm_leaf_kernel(FmmArgs a) {
 // the launch depends on host-side counts, so it can be captured in a graph
 // and replayed on a rebuilt tree.
 template <bool MIXED>
-__global__ __launch_bounds__(256) void fmm_leaf_persistent_kernel(FmmArgs a,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fmm_leaf_persistent_kernel(FmmArgs a,
                                                                   const uint32_t *__restrict__ nleaves,
                                                                   uint32_t *__restrict__ counter) {
   __shared__ double2 s_loc[4][kFmmMaxP];

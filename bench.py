@@ -7,11 +7,14 @@ at every blob (targets = sources), i.e. make_tree + biot_tree at all N blobs
 Gaussian mixture (centres U[0,1]^2, std log-uniform in [1e-3, 5e-2], core
 radius 5e-4, circulation N(0, 1/N)), seeded.
 
-The headline `value` is the mixed-precision mode (FP32 near field with
-compensated sums, FP32-MFMA far-field contraction, FP64 expansions and
-accumulation); the plain FP64 mode is timed on the same input, and each
-states its error against the FP64 direct sum on sampled targets.  The
-reference-rule treecode is timed beside them.
+The headline `value` is the mixed-precision mode with learned per-leaf
+order and near-field radius (FP32 pair sums accumulated in FP64, FP32-MFMA
+far-field contraction, FP64 expansions) when it meets rel. L2 <= 1e-6, else
+the mixed mode with fixed parameters; the plain FP64 mode is timed on the
+same input, and each states its error against the FP64 direct sum on
+sampled targets.  The reference-rule treecode is timed beside them.  Inputs
+are resident in HBM for `value`; the rate through host arrays (PCIe copies
+included) is reported beside it as `host_inclusive`.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--p P]
 
@@ -39,10 +42,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(xy, cir, cor, nterms, num_p, theta, n_targets, seed, threads):
     """Oracle (reference-semantics C restatement of the treecode): the tree of
     all N blobs built single-threaded as the reference does, then a sample of
-    targets on `threads` host threads; the rate is extrapolated to N targets."""
+    targets on `threads` host threads (and a smaller one on 1 thread); the
+    rates are extrapolated to N targets."""
     from oracle import oracle as O
     wl = importlib.import_module("2dfmm_amd.workloads")
     n = xy.shape[0]
@@ -54,6 +69,11 @@ def cpu_baseline(xy, cir, cor, nterms, num_p, theta, n_targets, seed, threads):
     T.eval(xy[idx], nthreads=threads)
     t_eval = time.perf_counter() - t0
     total = t_build + t_eval * n / len(idx)
+    idx1 = idx[: max(1, len(idx) // 8)]
+    t0 = time.perf_counter()
+    T.eval(xy[idx1], nthreads=1)
+    t_eval1 = time.perf_counter() - t0
+    total1 = t_build + t_eval1 * n / len(idx1)
     return {
         "value": n / total,
         "unit": "target velocities/s",
@@ -63,6 +83,9 @@ def cpu_baseline(xy, cir, cor, nterms, num_p, theta, n_targets, seed, threads):
                    f"theta {theta}): tree of all {n} blobs on 1 thread ({t_build:.1f} s) + "
                    f"{len(idx)} sampled targets on {threads} threads ({t_eval:.2f} s), "
                    f"extrapolated to {n} targets"),
+        "value_1core": n / total1,
+        "sample_1core": f"same tree + {len(idx1)} sampled targets on 1 thread ({t_eval1:.2f} s)",
+        "host": {"cpu_model": _cpu_model(), "nproc": os.cpu_count()},
     }
 
 
@@ -266,7 +289,6 @@ def main():
             f"rel_max={mx:.3e}, far={far} pairs={pairs}, eval {ph['evaluate']:.2f} ms "
             f"= {ach:.2f} TFLOP/s; phases {ph}")
 
-    fmm.set_learned(None)
     # headline: mixed + learned when it meets the 1e-6 bar, else mixed alone
     if "learned" in results and results["learned"]["rel_L2"] <= 1e-6:
         head_mode = "learned"
@@ -276,6 +298,33 @@ def main():
         head_mode = next(iter(results))
     head = results[head_mode]
     plain = results.get("fmm64")
+
+    # the same step through host arrays: make_tree copies the blobs in, the
+    # velocities come back to host memory (PCIe both ways); never `value`
+    host = None
+    if head_mode in ("learned", "mixed", "fmm64"):
+        fmm.set_mode(fmm.MODE_FMM64 if head_mode == "fmm64" else fmm.MODE_MIXED)
+        fmm.set_fmm_params(args.p, args.leaf, args.theta, args.cut)
+        if head_mode == "learned":
+            with np.load(wpath) as z:
+                fmm.set_learned(z["weights"], float(z["p_margin"]), float(z["cut_margin"]))
+        else:
+            fmm.set_learned(None)
+        # one evaluation per step: no velocity prefetch inside make_tree
+        fmm.set_prefetch(False)
+        fmm.make_tree(xy, cir, cor)
+        fmm.eval_sources()
+        t0 = time.perf_counter()
+        reps = max(1, min(args.steps, 3))
+        for _ in range(reps):
+            fmm.make_tree(xy, cir, cor)
+            fmm.eval_sources()
+        hms = (time.perf_counter() - t0) * 1e3 / reps
+        fmm.set_prefetch(True)
+        host = {"ms_per_step": hms, "value": n / (hms * 1e-3),
+                "note": "make_tree from host arrays + velocities copied back (PCIe included)"}
+        log(f"[bench] {head_mode} through host arrays: {hms:.2f} ms/step")
+    fmm.set_learned(None)
 
     cpu = None
     if not args.no_cpu:
@@ -319,6 +368,7 @@ def main():
                      "traffic": traffic, "flops_per_launch": head["flops"],
                      "avg_launch_ms": head["phases_ms"]["evaluate"]},
         "cpu_baseline": cpu,
+        "host_inclusive": host,
     }
     print(json.dumps(out), flush=True)
 
