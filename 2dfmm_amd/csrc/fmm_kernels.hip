@@ -344,12 +344,27 @@ __device__ inline uint64_t rfl64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// LDS stage of one wave: up to 64 sources of a near step, replica-major
+// (replica r reads its sources contiguously from r * stride), so rows of
+// up to 128 entries; FP32 entries for the mixed form, FP64 for the plain one.
+template <bool MIXED>
+struct StageOf {
+  typedef double4 T;
+};
+template <>
+struct StageOf<true> {
+  typedef float4 T;
+};
+constexpr int kStageSlots = 128;
+
 // Per-target-leaf values; lanes are (target ti, replica rep) pairs.
+// Source j of a staged step goes to LDS slot (j mod nrep) stride + j / nrep
+// (spos for j = lane), stride = ceil(64 / nrep).
 struct LeafCtx {
   NodeRec NA;
-  int cA, pA, nrep, ti, rep, tj;
+  int cA, pA, nrep, ti, rep, tj, stride, spos;
   bool tact;
-  double RA, sA, rcut, tx, ty;
+  double RA, sA, rcut, inv_theta, tx, ty;
   float lx, ly;
 };
 
@@ -360,6 +375,7 @@ __device__ inline LeafCtx leaf_ctx(const FmmArgs &a, int li, int lane) {
   L.cA = node_count(L.NA.cnt);
   L.RA = L.NA.region * 0.70710678118654752;
   L.sA = L.NA.region;
+  L.inv_theta = 1.0 / a.theta;
   // this leaf's expansion order and near-field radius
   L.pA = a.p;
   L.rcut = a.rcut;
@@ -371,6 +387,8 @@ __device__ inline LeafCtx leaf_ctx(const FmmArgs &a, int li, int lane) {
   L.nrep = max(1, 64 / L.cA);
   L.ti = lane % L.cA;
   L.rep = lane / L.cA;
+  L.stride = (64 + L.nrep - 1) / L.nrep;
+  L.spos = (lane % L.nrep) * L.stride + lane / L.nrep;
   L.tact = L.rep < L.nrep;
   L.tj = L.NA.first + L.ti;
   L.tx = a.sx[L.tj];
@@ -399,11 +417,11 @@ __device__ inline void pair32(float px, float py, const float4 s, float &gx, flo
   const float rx = px - s.x, ry = py - s.y;
   const float r2 = fmaf(rx, rx, ry * ry);
   const float y = r2 * s.w;
-  float h = (1.0f - __builtin_amdgcn_exp2f(-y)) * __builtin_amdgcn_rcpf(y);
-  if (__builtin_amdgcn_ballot_w64(y < kSeriesY)) {
-    if (y < kSeriesY) h = r2 > 1e-20f ? h_series(y) : 0.0f;
+  // z (1 - 2^-y) / y with z - z 2^-y in one fused multiply-add
+  float fac = fmaf(-s.z, __builtin_amdgcn_exp2f(-y), s.z) * __builtin_amdgcn_rcpf(y);
+  if (__builtin_amdgcn_fcmpf(y, kSeriesY, 4 /* ordered less-than, wave mask */)) {
+    if (y < kSeriesY) fac = r2 > 1e-20f ? s.z * h_series(y) : 0.0f;
   }
-  const float fac = s.z * h;
   gx = fmaf(-fac, ry, gx);
   gy = fmaf(fac, rx, gy);
 }
@@ -423,40 +441,40 @@ __device__ inline void pair64(double tx, double ty, const double4 s, double &ax,
   ay = fma(fac, rx, ay);
 }
 
-// Sources st[rep], st[rep + nrep], ... (< cnt) of an LDS slice; groups of
-// four pairs are summed in FP32 and accumulated in FP64.
-__device__ inline void lds_sum32(float px, float py, const float4 *st, int cnt, int rep, int nrep,
+// The sources of one replica: source j = rep + k nrep (< cnt) at row[k] of
+// the replica-major stage.  Groups of eight pairs are summed in FP32 and
+// accumulated in FP64; the row is read with immediate offsets.
+__device__ inline void lds_sum32(float px, float py, const float4 *row, int cnt, int rep, int nrep,
                                  double &ax, double &ay) {
   int j = rep;
-  for (; j + 3 * nrep < cnt; j += 4 * nrep) {
-    const float4 s0 = st[j], s1 = st[j + nrep], s2 = st[j + 2 * nrep], s3 = st[j + 3 * nrep];
+  for (; j + 7 * nrep < cnt; j += 8 * nrep, row += 8) {
+    // all eight reads issued before the first pair (the series branch would
+    // otherwise keep them apart)
+    float4 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = row[k];
     float gx = 0.f, gy = 0.f;
-    pair32(px, py, s0, gx, gy);
-    pair32(px, py, s1, gx, gy);
-    pair32(px, py, s2, gx, gy);
-    pair32(px, py, s3, gx, gy);
+#pragma unroll
+    for (int k = 0; k < 8; k++) pair32(px, py, v[k], gx, gy);
     ax += (double)gx;
     ay += (double)gy;
   }
   if (j < cnt) {
     float gx = 0.f, gy = 0.f;
-    for (; j < cnt; j += nrep) pair32(px, py, st[j], gx, gy);
+    for (; j < cnt; j += nrep) pair32(px, py, *row++, gx, gy);
     ax += (double)gx;
     ay += (double)gy;
   }
 }
 
-__device__ inline void lds_sum64(double tx, double ty, const double4 *st, int cnt, int rep,
+__device__ inline void lds_sum64(double tx, double ty, const double4 *row, int cnt, int rep,
                                  int nrep, double &ax, double &ay) {
   int j = rep;
-  for (; j + 3 * nrep < cnt; j += 4 * nrep) {
-    const double4 s0 = st[j], s1 = st[j + nrep], s2 = st[j + 2 * nrep], s3 = st[j + 3 * nrep];
-    pair64(tx, ty, s0, ax, ay);
-    pair64(tx, ty, s1, ax, ay);
-    pair64(tx, ty, s2, ax, ay);
-    pair64(tx, ty, s3, ax, ay);
+  for (; j + 3 * nrep < cnt; j += 4 * nrep, row += 4) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) pair64(tx, ty, row[k], ax, ay);
   }
-  for (; j < cnt; j += nrep) pair64(tx, ty, st[j], ax, ay);
+  for (; j < cnt; j += nrep) pair64(tx, ty, *row++, ax, ay);
 }
 
 // Direct sums of the targets of L against near leaf C.  With one replica
@@ -466,7 +484,7 @@ __device__ inline void lds_sum64(double tx, double ty, const double4 *st, int cn
 // there, so each source crosses the vector memory path once per wave.
 template <bool MIXED>
 __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeRec &C, int lane,
-                                 double4 *st, double &ax, double &ay) {
+                                 typename StageOf<MIXED>::T *st, double &ax, double &ay) {
   const int cnt = node_count(C.cnt);
   const int f = rfl(C.first);
   const int nrep = L.nrep;
@@ -505,9 +523,9 @@ __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeR
       }
     } else {
       float4 *sf = reinterpret_cast<float4 *>(st);
-      if (lane < cnt) sf[lane] = a.loc[f + lane];
+      if (lane < cnt) sf[L.spos] = a.loc[f + lane];
       wave_sync();
-      if (L.tact) lds_sum32(px, py, sf, cnt, L.rep, nrep, ax, ay);
+      if (L.tact) lds_sum32(px, py, sf + L.rep * L.stride, cnt, L.rep, nrep, ax, ay);
       wave_sync();
     }
   } else {
@@ -524,9 +542,10 @@ __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeR
       }
       for (; j < cnt; j++) pair64(L.tx, L.ty, ldro(src + j), ax, ay);
     } else {
-      if (lane < cnt) st[lane] = a.loc64[f + lane];
+      double4 *sd = reinterpret_cast<double4 *>(st);
+      if (lane < cnt) sd[L.spos] = a.loc64[f + lane];
       wave_sync();
-      if (L.tact) lds_sum64(L.tx, L.ty, st, cnt, L.rep, nrep, ax, ay);
+      if (L.tact) lds_sum64(L.tx, L.ty, sd + L.rep * L.stride, cnt, L.rep, nrep, ax, ay);
       wave_sync();
     }
   }
@@ -539,7 +558,8 @@ __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeR
 // next tile's loads are issued before the current tile's sums.
 template <bool MIXED>
 __device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
-                                  int ta, int tb, int lane, double4 *st, double &ax, double &ay) {
+                                  int ta, int tb, int lane, typename StageOf<MIXED>::T *st,
+                                  double &ax, double &ay) {
   const int s0 = rfl(ldro(a.nfirst + c));
   const int cnt = node_count(C.cnt);
   double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -560,15 +580,16 @@ __device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int c, con
     if (MIXED) {
       float4 *sf = reinterpret_cast<float4 *>(st);
       if (lane < tc)
-        sf[lane] = make_float4((float)(q.x - L.NA.cx), (float)(q.y - L.NA.cy), zw.x, zw.y);
+        sf[L.spos] = make_float4((float)(q.x - L.NA.cx), (float)(q.y - L.NA.cy), zw.x, zw.y);
       wave_sync();
       if (t + 1 < tb) load(t + 1);
-      if (L.tact) lds_sum32(L.lx, L.ly, sf, tc, L.rep, L.nrep, ax, ay);
+      if (L.tact) lds_sum32(L.lx, L.ly, sf + L.rep * L.stride, tc, L.rep, L.nrep, ax, ay);
     } else {
-      if (lane < tc) st[lane] = q;
+      double4 *sd = reinterpret_cast<double4 *>(st);
+      if (lane < tc) sd[L.spos] = q;
       wave_sync();
       if (t + 1 < tb) load(t + 1);
-      if (L.tact) lds_sum64(L.tx, L.ty, st, tc, L.rep, L.nrep, ax, ay);
+      if (L.tact) lds_sum64(L.tx, L.ty, sd + L.rep * L.stride, tc, L.rep, L.nrep, ax, ay);
     }
     wave_sync();
   }
@@ -578,11 +599,14 @@ __device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int c, con
 // of sources.  For a descendant C' of C, |c' - c_A| <= d + R_C - R_C' < d + R_C
 // =: reach, so
 //   d' - R_A - R_C' < r_cut      when reach - R_A <= r_cut, and
-//   R_A + R_C' > theta d'         when theta reach <= R_A.
-__device__ inline bool whole_near(const FmmArgs &a, const LeafCtx &L, const NodeRec &C) {
-  const double dx = C.cx - L.NA.cx, dy = C.cy - L.NA.cy;
-  const double reach = sqrt(dx * dx + dy * dy) + C.region * 0.70710678118654752;
-  return reach - L.RA <= L.rcut || a.theta * reach <= L.RA;
+//   R_A + R_C' > theta d'         when theta reach <= R_A,
+// i.e. when d <= max(r_cut + R_A - R_C, R_A / theta - R_C) (compared squared).
+__device__ inline bool whole_near(double inv_theta, double RA, double rcut, const NodeRec &NA,
+                                  const NodeRec &C) {
+  const double dx = C.cx - NA.cx, dy = C.cy - NA.cy;
+  const double RC = C.region * 0.70710678118654752;
+  const double m = fmax(rcut + RA - RC, RA * inv_theta - RC);
+  return m >= 0.0 && dx * dx + dy * dy <= m * m;
 }
 
 // Near items are leaves (one step) and whole near subtrees (one step per
@@ -597,12 +621,25 @@ __device__ inline int tile_count(const NodeRec &C, bool range, int t) {
 // Steps ta..tb-1 of near item c.
 template <bool MIXED>
 __device__ inline void near_steps(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
-                                  bool range, int ta, int tb, int lane, double4 *st, double &ax,
+                                  bool range, int ta, int tb, int lane,
+                                  typename StageOf<MIXED>::T *st, double &ax,
                                   double &ay) {
   if (range)
     near_range<MIXED>(a, L, c, C, ta, tb, lane, st, ax, ay);
   else
     near_leaf<MIXED>(a, L, C, lane, st, ax, ay);
+}
+
+// The far test of a cell C for target leaf A: R_A + R_C <= theta d and
+// d - R_A - R_C >= r_cut, i.e. d >= max((R_A + R_C) / theta, R_A + R_C + r_cut),
+// compared squared (no square root).  Every walk -- fused pass, chunks,
+// counting -- calls this one function, so they all make the same decisions.
+__device__ inline bool far_cell(double inv_theta, double RA, double rcut, const NodeRec &NA,
+                                const NodeRec &C) {
+  const double dx = C.cx - NA.cx, dy = C.cy - NA.cy;
+  const double sr = RA + C.region * 0.70710678118654752;
+  const double lim = fmax(sr * inv_theta, sr + rcut);
+  return dx * dx + dy * dy >= lim * lim;
 }
 
 // State of the walk: a 4-bit mask of still-open children per level, the
@@ -630,10 +667,7 @@ __device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, F
         const int c = base + q;
         const NodeRec C = ldro(a.nodes + c);
         if (node_count(C.cnt) == 0) continue;
-        const double dx = C.cx - L.NA.cx, dy = C.cy - L.NA.cy;
-        const double dd = sqrt(dx * dx + dy * dy);
-        const double RC = C.region * 0.70710678118654752;
-        if (L.RA + RC <= a.theta * dd && dd - L.RA - RC >= L.rcut)
+        if (far_cell(L.inv_theta, L.RA, L.rcut, L.NA, C))
           far(c);
         else
           m |= 1u << q;
@@ -654,7 +688,7 @@ __device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, F
     ws.sl.set(ws.d, m & ~(1u << q));
     ws.node = rfl((ws.cur & ~3) + q);
     const NodeRec C = ldro(a.nodes + ws.node);
-    const bool range = node_internal(C.cnt) && whole_near(a, L, C);
+    const bool range = node_internal(C.cnt) && whole_near(L.inv_theta, L.RA, L.rcut, L.NA, C);
     ws.expand = node_internal(C.cnt) && !range;
     if (!ws.expand && !near(ws.node, C, range)) return;
   }
@@ -684,7 +718,8 @@ __device__ inline double2 reduce_replicas(const LeafCtx &L, int lane, double2 *r
 // cells are skipped (the fused pass took them).
 template <bool MIXED>
 __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const SplitCk &r, int64_t end,
-                                 int lane, double4 *st, double &ax, double &ay) {
+                                 int lane, typename StageOf<MIXED>::T *st, double &ax,
+                                 double &ay) {
   WalkState ws;
   ws.sl.lo = rfl64(r.lo);
   ws.sl.hi = rfl64(r.hi);
@@ -716,7 +751,8 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const Split
 template <bool MIXED>
 __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
                                  double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
-                                 double4 (*s_stage)[64], SplitCk (*s_ck)[kMaxCk]) {
+                                 typename StageOf<MIXED>::T (*s_stage)[kStageSlots],
+                                 SplitCk (*s_ck)[kMaxCk]) {
   li = rfl(li);
   const LeafCtx L = leaf_ctx<MIXED>(a, li, lane);
   const NodeRec &NA = L.NA;
@@ -929,7 +965,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
     FmmArgs a) {
   __shared__ double2 s_loc[4][kFmmMaxP];
   __shared__ double2 s_red[4][64];
-  __shared__ double4 s_stage[4][64];
+  __shared__ typename StageOf<MIXED>::T s_stage[4][kStageSlots];
   __shared__ SplitCk s_ck[4][kMaxCk];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = blockIdx.x * 4 + w;
@@ -948,7 +984,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
                                                                   uint32_t *__restrict__ counter) {
   __shared__ double2 s_loc[4][kFmmMaxP];
   __shared__ double2 s_red[4][64];
-  __shared__ double4 s_stage[4][64];
+  __shared__ typename StageOf<MIXED>::T s_stage[4][kStageSlots];
   __shared__ SplitCk s_ck[4][kMaxCk];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = *nleaves;
@@ -967,7 +1003,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
 template <bool MIXED>
 __global__ __launch_bounds__(256) void fmm_chunk_kernel(FmmArgs a, uint32_t *__restrict__ counter) {
   __shared__ double2 s_red[4][64];
-  __shared__ double4 s_stage[4][64];
+  __shared__ typename StageOf<MIXED>::T s_stage[4][kStageSlots];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = min(*a.ck_count, (uint32_t)a.ck_cap);
   for (;;) {
@@ -1057,6 +1093,7 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
   const int cA = node_count(NA.cnt);
   const double RA = NA.region * 0.70710678118654752;
   const double rcut = a.leaf_par ? (double)a.leaf_par[li].y * a.smax : a.rcut;
+  const double inv_theta = 1.0 / a.theta;
   int64_t far = 0, pairs = 0;
   Slots2 sl;
   int d = 0, cur = 0, node = root_of(a, NA.first);
@@ -1068,10 +1105,7 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
       for (int q = 0; q < 4; q++) {
         const NodeRec C = a.nodes[base + q];
         if (node_count(C.cnt) == 0) continue;
-        const double dx = C.cx - NA.cx, dy = C.cy - NA.cy;
-        const double dd = sqrt(dx * dx + dy * dy);
-        const double RC = C.region * 0.70710678118654752;
-        if (RA + RC <= a.theta * dd && dd - RA - RC >= rcut) far++;
+        if (far_cell(inv_theta, RA, rcut, NA, C)) far++;
         else m |= 1u << q;
       }
       d++;
