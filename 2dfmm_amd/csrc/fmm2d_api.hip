@@ -96,6 +96,7 @@ struct Context {
   DevBuf<uint32_t> lflag, lpos;
   DevBuf<double2> direct_ws;  // partial sums of the direct judge
   double smax_h = 0.0;        // largest core radius of the tree's blobs
+  double wmax_h = 0.0;        // log2(e) / (smallest core radius)^2, capped
   // learned per-leaf parameters
   bool learned = false;
   float p_margin = 0.f, cut_margin = 0.f;
@@ -344,20 +345,22 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   if (g.t_mode != FMM2D_MODE_TREECODE) {
     // target leaves, leaf-relative FP32 sources, near-field radius
     if (g.leaves.ensure(n, s) || g.lat.ensure(n, s) || g.lflag.ensure(n, s) || g.lpos.ensure(n, s) ||
-        g.loc.ensure(n, s) || g.loc64.ensure(n, s) || g.smax.ensure(1, s))
+        g.loc.ensure(n, s) || g.loc64.ensure(n, s) || g.smax.ensure(2, s))
       return -1;
     if (g.ntl_dev.ensure(1, s) || g.counter.ensure(1, s)) return -1;
     if (launch_leaf_list(g.nodes.p, g.nnodes, n, g.lat.p, g.lflag.p, g.lpos.p, g.leaves.p,
                          g.ntl_dev.p, s))
       return -1;
-    if (launch_max_reduce(dcor, n, g.smax.p, s)) return -1;
-    double smax = 0.0;
+    if (launch_minmax_reduce(dcor, n, g.smax.p, s)) return -1;
+    double smm[2] = {0.0, 0.0};
     FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.ntl_dev.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    FMM_HIP_TRY(hipMemcpyAsync(&smax, g.smax.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    FMM_HIP_TRY(hipMemcpyAsync(smm, g.smax.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     FMM_HIP_TRY(hipStreamSynchronize(s));
     g.ntleaves = (int)g.h_total[0];
-    g.smax_h = smax;
-    g.rcut = g.fmm_cut * smax;
+    g.smax_h = smm[0];
+    g.rcut = g.fmm_cut * smm[0];
+    // the FP32 pair kernel's w = log2(e) / sigma^2 is capped at 7.2e19
+    g.wmax_h = smm[1] > 0.0 ? std::min(1.4426950408889634 / (smm[1] * smm[1]), 7.2e19) : 7.2e19;
     if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
                           g.loc.p, g.loc64.p, s))
       return -1;
@@ -421,6 +424,7 @@ static FmmArgs fmm_args(double *d_uv) {
   a.root_shift = g.root_shift;
   a.leaf_par = g.leaf_par_on ? g.leaf_par.p : nullptr;
   a.smax = g.smax_h;
+  a.wmax = (float)g.wmax_h;
   a.nfirst = g.nfirst.p;
   a.split_budget = g.split_budget;
   a.ck_cap = (int)g.ck.cap;

@@ -150,6 +150,8 @@ struct FmmArgs {
   // null = p and rcut for every leaf.  smax = largest core radius.
   const float2 *leaf_par;
   double smax;
+  // largest w = log2(e) / sigma^2 of the sources (mixed form)
+  float wmax;
   // first blob (in leaf order) of every node's subtree
   const int32_t *nfirst;
   // splitting of target leaves with much near work (split_budget 0: off):
@@ -170,7 +172,8 @@ int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleav
 // Non-empty leaves in blob (Morton) order; their number lands in *d_count.
 int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, uint32_t *d_flag,
                      uint32_t *d_pos, int32_t *d_leaves, uint32_t *d_count, hipStream_t s);
-int launch_max_reduce(const double *d_v, int n, double *d_out, hipStream_t s);
+// d_out[0] = max, d_out[1] = min of n values >= 0.
+int launch_minmax_reduce(const double *d_v, int n, double *d_out, hipStream_t s);
 // First blob of every node's subtree: the first blob of its leftmost leaf.
 int launch_node_first_blob(const NodeRec *d_nodes, int nnodes, int32_t *d_out, hipStream_t s);
 // One wave per target leaf; with splitting on, d_split_ctr holds two counters

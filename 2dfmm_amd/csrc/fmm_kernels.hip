@@ -190,20 +190,38 @@ int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, u
   return 0;
 }
 
-__global__ __launch_bounds__(256) void max_reduce_kernel(const double *__restrict__ v, int n,
-                                                         double *__restrict__ out) {
-  double m = 0.0;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = fmax(m, v[i]);
+// out[0] = max, out[1] = min of n values >= 0 (their bit patterns order like
+// unsigned integers, so integer atomics combine the blocks).
+__global__ __launch_bounds__(256) void minmax_reduce_kernel(const double *__restrict__ v, int n,
+                                                            double *__restrict__ out) {
+  double hi = 0.0, lo = 1.0e300;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    hi = fmax(hi, v[i]);
+    lo = fmin(lo, v[i]);
+  }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-  __shared__ double s[4];
-  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+  for (int o = 32; o > 0; o >>= 1) {
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+  }
+  __shared__ double s[2][4];
+  if ((threadIdx.x & 63) == 0) {
+    s[0][threadIdx.x >> 6] = hi;
+    s[1][threadIdx.x >> 6] = lo;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    m = fmax(fmax(s[0], s[1]), fmax(s[2], s[3]));
-    // values are >= 0: their bit patterns order like unsigned integers
-    atomicMax(reinterpret_cast<unsigned long long *>(out), (unsigned long long)__double_as_longlong(m));
+    hi = fmax(fmax(s[0][0], s[0][1]), fmax(s[0][2], s[0][3]));
+    lo = fmin(fmin(s[1][0], s[1][1]), fmin(s[1][2], s[1][3]));
+    atomicMax(reinterpret_cast<unsigned long long *>(out), (unsigned long long)__double_as_longlong(hi));
+    atomicMin(reinterpret_cast<unsigned long long *>(out + 1),
+              (unsigned long long)__double_as_longlong(lo));
   }
+}
+
+__global__ void minmax_init_kernel(double *out) {
+  out[0] = 0.0;
+  out[1] = 1.0e300;
 }
 
 // Child 0 of a node starts at the node's first blob, so following child 0
@@ -225,12 +243,13 @@ int launch_node_first_blob(const NodeRec *d_nodes, int nnodes, int32_t *d_out, h
   return 0;
 }
 
-int launch_max_reduce(const double *d_v, int n, double *d_out, hipStream_t s) {
-  FMM_HIP_TRY(hipMemsetAsync(d_out, 0, sizeof(double), s));
+int launch_minmax_reduce(const double *d_v, int n, double *d_out, hipStream_t s) {
+  minmax_init_kernel<<<1, 1, 0, s>>>(d_out);
+  FMM_HIP_TRY(hipGetLastError());
   int nb = (n + 255) / 256;
   if (nb > 1024) nb = 1024;
   if (nb < 1) nb = 1;
-  max_reduce_kernel<<<nb, 256, 0, s>>>(d_v, n, d_out);
+  minmax_reduce_kernel<<<nb, 256, 0, s>>>(d_v, n, d_out);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -412,15 +431,19 @@ __device__ inline int64_t near_units(int cnt, int nrep) {
 
 // One pair (target, source s) of the mixed form, target at (px, py) and
 // source at (s.x, s.y) in the same FP32 frame: u += -f ry, v += f rx with
-// f = z h(y), y = r^2 s.w (see h_series).
+// f = z h(y), y = r^2 s.w (see h_series).  SERIES = false for near items
+// whose pairs all have y >= kSeriesY (needs_series): no check at all.
+template <bool SERIES>
 __device__ inline void pair32(float px, float py, const float4 s, float &gx, float &gy) {
   const float rx = px - s.x, ry = py - s.y;
   const float r2 = fmaf(rx, rx, ry * ry);
   const float y = r2 * s.w;
   // z (1 - 2^-y) / y with z - z 2^-y in one fused multiply-add
   float fac = fmaf(-s.z, __builtin_amdgcn_exp2f(-y), s.z) * __builtin_amdgcn_rcpf(y);
-  if (__builtin_amdgcn_fcmpf(y, kSeriesY, 4 /* ordered less-than, wave mask */)) {
-    if (y < kSeriesY) fac = r2 > 1e-20f ? s.z * h_series(y) : 0.0f;
+  if (SERIES) {
+    if (__builtin_amdgcn_fcmpf(y, kSeriesY, 4 /* ordered less-than, wave mask */)) {
+      if (y < kSeriesY) fac = r2 > 1e-20f ? s.z * h_series(y) : 0.0f;
+    }
   }
   gx = fmaf(-fac, ry, gx);
   gy = fmaf(fac, rx, gy);
@@ -444,6 +467,7 @@ __device__ inline void pair64(double tx, double ty, const double4 s, double &ax,
 // The sources of one replica: source j = rep + k nrep (< cnt) at row[k] of
 // the replica-major stage.  Groups of eight pairs are summed in FP32 and
 // accumulated in FP64; the row is read with immediate offsets.
+template <bool SERIES>
 __device__ inline void lds_sum32(float px, float py, const float4 *row, int cnt, int rep, int nrep,
                                  double &ax, double &ay) {
   int j = rep;
@@ -455,13 +479,13 @@ __device__ inline void lds_sum32(float px, float py, const float4 *row, int cnt,
     for (int k = 0; k < 8; k++) v[k] = row[k];
     float gx = 0.f, gy = 0.f;
 #pragma unroll
-    for (int k = 0; k < 8; k++) pair32(px, py, v[k], gx, gy);
+    for (int k = 0; k < 8; k++) pair32<SERIES>(px, py, v[k], gx, gy);
     ax += (double)gx;
     ay += (double)gy;
   }
   if (j < cnt) {
     float gx = 0.f, gy = 0.f;
-    for (; j < cnt; j += nrep) pair32(px, py, *row++, gx, gy);
+    for (; j < cnt; j += nrep) pair32<SERIES>(px, py, *row++, gx, gy);
     ax += (double)gx;
     ay += (double)gy;
   }
@@ -475,6 +499,50 @@ __device__ inline void lds_sum64(double tx, double ty, const double4 *row, int c
     for (int k = 0; k < 4; k++) pair64(tx, ty, row[k], ax, ay);
   }
   for (; j < cnt; j += nrep) pair64(tx, ty, *row++, ax, ay);
+}
+
+// All sources of a near leaf for one target, read at wave-uniform indices
+// (scalar loads), groups of four pairs summed in FP32, accumulated in FP64.
+template <bool SERIES>
+__device__ inline void scalar_sum32(float px, float py, const float4 *src, int cnt, double &ax,
+                                    double &ay) {
+  int j = 0;
+  if (cnt >= 4) {
+    // the next group's loads are issued before this group's arithmetic
+    float4 n0 = ldro(src), n1 = ldro(src + 1), n2 = ldro(src + 2), n3 = ldro(src + 3);
+    for (; j + 3 < cnt; j += 4) {
+      const float4 s0 = n0, s1 = n1, s2 = n2, s3 = n3;
+      if (j + 7 < cnt) {
+        n0 = ldro(src + j + 4);
+        n1 = ldro(src + j + 5);
+        n2 = ldro(src + j + 6);
+        n3 = ldro(src + j + 7);
+      }
+      float gx = 0.f, gy = 0.f;
+      pair32<SERIES>(px, py, s0, gx, gy);
+      pair32<SERIES>(px, py, s1, gx, gy);
+      pair32<SERIES>(px, py, s2, gx, gy);
+      pair32<SERIES>(px, py, s3, gx, gy);
+      ax += (double)gx;
+      ay += (double)gy;
+    }
+  }
+  if (j < cnt) {
+    float gx = 0.f, gy = 0.f;
+    for (; j < cnt; j++) pair32<SERIES>(px, py, ldro(src + j), gx, gy);
+    ax += (double)gx;
+    ay += (double)gy;
+  }
+}
+
+// Whether a pair of target leaf A and near item C can have y < kSeriesY:
+// y = r^2 w >= gap^2 wmax is false only if the gap between the two boxes
+// (half side region/2) is below sqrt(kSeriesY / wmax); 1% margin for the FP32
+// coordinates.  With one core radius this is exact up to the margin.
+__device__ inline bool needs_series(const FmmArgs &a, const LeafCtx &L, const NodeRec &C) {
+  const double h = 0.5 * (L.NA.region + C.region);
+  const double gx = fmax(fabs(C.cx - L.NA.cx) - h, 0.0), gy = fmax(fabs(C.cy - L.NA.cy) - h, 0.0);
+  return (gx * gx + gy * gy) * (double)a.wmax < 1.01 * (double)kSeriesY;
 }
 
 // Direct sums of the targets of L against near leaf C.  With one replica
@@ -492,40 +560,22 @@ __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeR
     // sources are stored relative to their own leaf's centre
     const float dx = (float)(C.cx - L.NA.cx), dy = (float)(C.cy - L.NA.cy);
     const float px = L.lx - dx, py = L.ly - dy;
+    const bool ser = needs_series(a, L, C);
     if (nrep == 1) {
-      const float4 *src = a.loc + f;
-      int j = 0;
-      if (cnt >= 4) {
-        // the next group's loads are issued before this group's arithmetic
-        float4 n0 = ldro(src), n1 = ldro(src + 1), n2 = ldro(src + 2), n3 = ldro(src + 3);
-        for (; j + 3 < cnt; j += 4) {
-          const float4 s0 = n0, s1 = n1, s2 = n2, s3 = n3;
-          if (j + 7 < cnt) {
-            n0 = ldro(src + j + 4);
-            n1 = ldro(src + j + 5);
-            n2 = ldro(src + j + 6);
-            n3 = ldro(src + j + 7);
-          }
-          float gx = 0.f, gy = 0.f;
-          pair32(px, py, s0, gx, gy);
-          pair32(px, py, s1, gx, gy);
-          pair32(px, py, s2, gx, gy);
-          pair32(px, py, s3, gx, gy);
-          ax += (double)gx;
-          ay += (double)gy;
-        }
-      }
-      if (j < cnt) {
-        float gx = 0.f, gy = 0.f;
-        for (; j < cnt; j++) pair32(px, py, ldro(src + j), gx, gy);
-        ax += (double)gx;
-        ay += (double)gy;
-      }
+      if (ser)
+        scalar_sum32<true>(px, py, a.loc + f, cnt, ax, ay);
+      else
+        scalar_sum32<false>(px, py, a.loc + f, cnt, ax, ay);
     } else {
       float4 *sf = reinterpret_cast<float4 *>(st);
       if (lane < cnt) sf[L.spos] = a.loc[f + lane];
       wave_sync();
-      if (L.tact) lds_sum32(px, py, sf + L.rep * L.stride, cnt, L.rep, nrep, ax, ay);
+      if (L.tact) {
+        if (ser)
+          lds_sum32<true>(px, py, sf + L.rep * L.stride, cnt, L.rep, nrep, ax, ay);
+        else
+          lds_sum32<false>(px, py, sf + L.rep * L.stride, cnt, L.rep, nrep, ax, ay);
+      }
       wave_sync();
     }
   } else {
@@ -562,6 +612,7 @@ __device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int c, con
                                   double &ax, double &ay) {
   const int s0 = rfl(ldro(a.nfirst + c));
   const int cnt = node_count(C.cnt);
+  const bool ser = MIXED && needs_series(a, L, C);
   double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
   float2 zw = make_float2(0.f, 0.f);
   auto load = [&](int t) {
@@ -583,7 +634,12 @@ __device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int c, con
         sf[L.spos] = make_float4((float)(q.x - L.NA.cx), (float)(q.y - L.NA.cy), zw.x, zw.y);
       wave_sync();
       if (t + 1 < tb) load(t + 1);
-      if (L.tact) lds_sum32(L.lx, L.ly, sf + L.rep * L.stride, tc, L.rep, L.nrep, ax, ay);
+      if (L.tact) {
+        if (ser)
+          lds_sum32<true>(L.lx, L.ly, sf + L.rep * L.stride, tc, L.rep, L.nrep, ax, ay);
+        else
+          lds_sum32<false>(L.lx, L.ly, sf + L.rep * L.stride, tc, L.rep, L.nrep, ax, ay);
+      }
     } else {
       double4 *sd = reinterpret_cast<double4 *>(st);
       if (lane < tc) sd[L.spos] = q;

@@ -183,6 +183,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device to device (a kernel of known traffic for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--persistent", type=int, default=0,
+                    help="accurate modes: persistent fused pass with this many workgroups per CU "
+                         "(0: one wave per target leaf over a sized grid)")
     ap.add_argument("--traffic", default=None,
                     help="JSON file with measured HBM bytes per launch of the dominant kernel")
     args = ap.parse_args()
@@ -220,6 +223,8 @@ def main():
     d_uv = torch.empty((n, 2), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     fmm.set_timing(True)
+    if args.persistent > 0:
+        fmm.set_persistent(args.persistent * fmm.device_cus(), False)
 
     # FP64 direct sum on sampled targets (the judge)
     idx = wl.sample_targets(n, args.judge_targets, seed=7)
@@ -357,7 +362,8 @@ def main():
         "data": "synthetic (seeded Gaussian mixture)",
         "config": {"workload": "gaussian_mixture_8", "n_blobs": n, "targets": n,
                    "core_radius": 5e-4, "mode": head_mode, "p": args.p, "leaf": args.leaf,
-                   "theta": args.theta, "cut_core_radii": args.cut, "parallelism": "single GPU"},
+                   "theta": args.theta, "cut_core_radii": args.cut, "parallelism": "single GPU",
+                   "persistent_blocks_per_cu": args.persistent},
         "accuracy": {"vs": "FP64 direct sum (fmm.c:121-142 rule)", "targets": len(idx),
                      "rel_L2": head["rel_L2"], "rel_max": head["rel_max"],
                      "meets_1e-6": head["rel_L2"] <= 1e-6},
