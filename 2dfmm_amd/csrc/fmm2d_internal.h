@@ -124,6 +124,7 @@ int launch_direct(int ns, const double *d_sxy, const double *d_cir, const double
 // tile `off`) and sums near steps while the work count is below `end`.
 struct SplitCk {
   uint64_t lo, hi;   // open-child masks, 4 bits per level
+  uint64_t rlo, rhi; // whole-near-subtree masks, 4 bits per level
   int64_t cum, end;  // near work units before this point; end of the chunk
   int32_t li, d, cur, node;  // target leaf (-1: skip), level, group, item
   int32_t off, pad;          // tile of the item

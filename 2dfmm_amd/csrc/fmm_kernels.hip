@@ -333,7 +333,7 @@ __device__ inline int root_of(const FmmArgs &a, int first) {
 // the checkpoints on other waves, and a last pass adds their partial sums in
 // chunk order (deterministic).
 // ---------------------------------------------------------------------------
-constexpr int kMaxCk = 64;  // checkpoints per leaf kept in LDS
+constexpr int kMaxCk = 32;  // checkpoints per leaf kept in LDS
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -701,7 +701,8 @@ __device__ inline bool far_cell(double inv_theta, double RA, double rcut, const 
 // State of the walk: a 4-bit mask of still-open children per level, the
 // current level, the first child of the current group and the current node.
 struct WalkState {
-  Slots2 sl;
+  Slots2 sl;  // open children per level
+  Slots2 rl;  // of those, whole near subtrees (summed as one run)
   int d, cur, node;
   bool expand;
 };
@@ -711,25 +712,35 @@ struct WalkState {
 // whole near subtree (false stops the walk).  Climbing uses the group parent
 // array, so the state is a few words and a walk can be saved and resumed.
 // Node indices are uniform across the wave (readfirstlane), so node records
-// come through scalar loads.
+// come through scalar loads.  An expansion tests its four children on lanes
+// 0-3 at once (far, open, whole near), combined with ballots.
 template <typename FarFn, typename NearFn>
-__device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, FarFn &&far,
-                            NearFn &&near) {
+__device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, int lane,
+                            FarFn &&far, NearFn &&near) {
   for (;;) {
     if (ws.expand) {
       const int base = rfl(ldro(a.nodes + ws.node).first);
-      uint32_t m = 0;
-      for (int q = 0; q < 4; q++) {
-        const int c = base + q;
-        const NodeRec C = ldro(a.nodes + c);
-        if (node_count(C.cnt) == 0) continue;
-        if (far_cell(L.inv_theta, L.RA, L.rcut, L.NA, C))
-          far(c);
-        else
-          m |= 1u << q;
+      bool is_far = false, is_open = false, is_run = false;
+      if (lane < 4) {
+        const NodeRec C = a.nodes[base + lane];
+        if (node_count(C.cnt) > 0) {
+          is_far = far_cell(L.inv_theta, L.RA, L.rcut, L.NA, C);
+          is_open = !is_far;
+          is_run = is_open && node_internal(C.cnt) &&
+                   whole_near(L.inv_theta, L.RA, L.rcut, L.NA, C);
+        }
+      }
+      uint32_t fm = (uint32_t)__builtin_amdgcn_ballot_w64(is_far);
+      const uint32_t om = (uint32_t)__builtin_amdgcn_ballot_w64(is_open);
+      const uint32_t rm = (uint32_t)__builtin_amdgcn_ballot_w64(is_run);
+      while (fm) {
+        const int q = __builtin_ctz(fm);
+        fm &= fm - 1;
+        far(base + q);
       }
       ws.d++;
-      ws.sl.set(ws.d, m);
+      ws.sl.set(ws.d, om);
+      ws.rl.set(ws.d, rm);
       ws.cur = base;
     }
     uint32_t m;
@@ -744,7 +755,7 @@ __device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, F
     ws.sl.set(ws.d, m & ~(1u << q));
     ws.node = rfl((ws.cur & ~3) + q);
     const NodeRec C = ldro(a.nodes + ws.node);
-    const bool range = node_internal(C.cnt) && whole_near(L.inv_theta, L.RA, L.rcut, L.NA, C);
+    const bool range = (ws.rl.get(ws.d) >> q) & 1u;
     ws.expand = node_internal(C.cnt) && !range;
     if (!ws.expand && !near(ws.node, C, range)) return;
   }
@@ -779,6 +790,8 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const Split
   WalkState ws;
   ws.sl.lo = rfl64(r.lo);
   ws.sl.hi = rfl64(r.hi);
+  ws.rl.lo = rfl64(r.rlo);
+  ws.rl.hi = rfl64(r.rhi);
   ws.d = rfl(r.d);
   ws.cur = rfl(r.cur);
   ws.node = rfl(r.node);
@@ -798,7 +811,7 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const Split
   // the checkpoint's item is a leaf or a whole near subtree (internal)
   const NodeRec C0 = ldro(a.nodes + ws.node);
   if (!item(ws.node, C0, node_internal(C0.cnt), rfl(r.off))) return;
-  walk(a, L, ws, [](int) {}, [&](int c, const NodeRec &C, bool range) -> bool {
+  walk(a, L, ws, lane, [](int) {}, [&](int c, const NodeRec &C, bool range) -> bool {
     return item(c, C, range, 0);
   });
 }
@@ -878,13 +891,15 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
     for (; t < nt; t++) {
       if (cum >= thr) {
         if (nck == kMaxCk) {
-          uint64_t lo = 0, hi = 0;
+          uint64_t lo = 0, hi = 0, rlo = 0, rhi = 0;
           int64_t cu = 0;
           int rd = 0, rc = 0, rn = 0, ro = 0;
           if (lane < kMaxCk / 2) {
             const SplitCk &r = s_ck[w][2 * lane];
             lo = r.lo;
             hi = r.hi;
+            rlo = r.rlo;
+            rhi = r.rhi;
             cu = r.cum;
             rd = r.d;
             rc = r.cur;
@@ -896,6 +911,8 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
             SplitCk &r = s_ck[w][lane];
             r.lo = lo;
             r.hi = hi;
+            r.rlo = rlo;
+            r.rhi = rhi;
             r.cum = cu;
             r.end = 0;
             r.li = li;
@@ -914,6 +931,8 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
             SplitCk &r = s_ck[w][nck];
             r.lo = ws.sl.lo;
             r.hi = ws.sl.hi;
+            r.rlo = ws.rl.lo;
+            r.rhi = ws.rl.hi;
             r.cum = cum;
             r.end = 0;
             r.li = li;
@@ -930,7 +949,7 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
     }
     return true;
   };
-  walk(a, L, ws, far, near);
+  walk(a, L, ws, lane, far, near);
   if (pending >= 0) far_pair(pending, -1);
   acc.fold();
 
