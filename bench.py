@@ -183,11 +183,15 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device to device (a kernel of known traffic for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--split", type=int, default=8192,
+                    help="accurate modes: near work units per chunk of a split target leaf (0: off)")
     ap.add_argument("--persistent", type=int, default=0,
                     help="accurate modes: persistent fused pass with this many workgroups per CU "
                          "(0: one wave per target leaf over a sized grid)")
-    ap.add_argument("--traffic", default=None,
-                    help="JSON file with measured HBM bytes per launch of the dominant kernel")
+    ap.add_argument("--traffic",
+                    default=os.path.join(ROOT, "profiles", "fmm_eval_gaussian_mixture_4M_traffic.txt"),
+                    help="measured HBM bytes per evaluation (rocprofv3 FETCH_SIZE / WRITE_SIZE, "
+                         "copy-calibrated), JSON keyed by precision")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -225,6 +229,7 @@ def main():
     fmm.set_timing(True)
     if args.persistent > 0:
         fmm.set_persistent(args.persistent * fmm.device_cus(), False)
+    fmm.set_split(args.split)
 
     # FP64 direct sum on sampled targets (the judge)
     idx = wl.sample_targets(n, args.judge_targets, seed=7)
@@ -258,7 +263,8 @@ def main():
             fmm.set_mode(fmm.MODE_MIXED if mixed else fmm.MODE_FMM64)
             fmm.set_fmm_params(args.p, args.leaf, args.theta, args.cut)
             prec = "f32" if mixed else "f64"
-            kernel = "fmm_leaf_kernel<%s>" % ("true" if mixed else "false")
+            t = "true" if mixed else "false"
+            kernel = f"fmm_leaf_kernel<{t}> + fmm_chunk_kernel<{t}> + fmm_split_sum_kernel"
             if mode == "learned":
                 with np.load(wpath) as z:
                     fmm.set_learned(z["weights"], float(z["p_margin"]), float(z["cut_margin"]))
@@ -344,9 +350,11 @@ def main():
 
     peak = PEAK[head["precision"]]
     traffic = None
-    if args.traffic and os.path.exists(args.traffic):
+    if args.traffic and os.path.exists(args.traffic) and head_mode != "treecode":
         with open(args.traffic) as f:
-            traffic = json.load(f).get(head["kernel"])
+            t = json.load(f).get(head["precision"])
+        if t:
+            traffic = t["bytes"]
     out = {
         "metric": METRIC,
         "value": head["value"],
@@ -363,7 +371,7 @@ def main():
         "config": {"workload": "gaussian_mixture_8", "n_blobs": n, "targets": n,
                    "core_radius": 5e-4, "mode": head_mode, "p": args.p, "leaf": args.leaf,
                    "theta": args.theta, "cut_core_radii": args.cut, "parallelism": "single GPU",
-                   "persistent_blocks_per_cu": args.persistent},
+                   "persistent_blocks_per_cu": args.persistent, "split_budget": args.split},
         "accuracy": {"vs": "FP64 direct sum (fmm.c:121-142 rule)", "targets": len(idx),
                      "rel_L2": head["rel_L2"], "rel_max": head["rel_max"],
                      "meets_1e-6": head["rel_L2"] <= 1e-6},

@@ -66,8 +66,9 @@ int fmm_get_params(int *nterms, int *num_p, double *theta, double *r_min);
  * the target leaf AND farther than cut_sigmas * (largest core radius), far
  * cells enter one local expansion per target leaf through a dense complex
  * contraction on MFMA, near leaves are summed directly.  MIXED runs the near
- * sums in FP32 (leaf-relative coordinates, Kahan sums) and the contraction on
- * FP32 MFMA; expansions and the final accumulation stay FP64.  These modes
+ * pairs in FP32 (coordinates relative to a leaf centre; groups of pairs
+ * summed in FP32 and accumulated in FP64) and the contraction on FP32 MFMA;
+ * expansions and the final accumulation stay FP64.  These modes
  * apply to the all-blob evaluation (make_tree's prefetch, fmm_eval_sources*);
  * other points are evaluated by the treecode rule on the same tree. */
 #define FMM2D_MODE_TREECODE 0

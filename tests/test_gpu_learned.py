@@ -126,3 +126,35 @@ def test_learned_constant_network_and_fallback(fmm):
     fmm.make_tree(xy, cir, cor)
     assert np.array_equal(fmm.eval_sources().view(np.uint64), ref.view(np.uint64))
     fmm.set_learned(None)
+
+
+def test_shipped_weights_ab_on_unseen_layout(fmm):
+    """A/B of the shipped network (2dfmm_amd/learned_weights.npz) against the
+    fixed order and radius (16, 4 core radii) on a 4M Gaussian mixture whose
+    seed (11) is used neither in training, nor for the margins, nor by the
+    benchmark: the learned parameters meet rel. L2 <= 1e-6 against the FP64
+    direct sum with at least 10% fewer near pairs."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "2dfmm_amd", "learned_weights.npz")
+    with np.load(path) as z:
+        w, pm, cm = z["weights"], float(z["p_margin"]), float(z["cut_margin"])
+    wl = workloads()
+    n = 4_000_000
+    xy, cir, cor = wl.gaussian_mixture(n, seed=11)
+    idx = wl.sample_targets(n, 3000, seed=11)
+    ref = fmm.direct(xy, cir, cor, xy[idx], style=1)
+    fmm.set_mode(fmm.MODE_MIXED)
+    fmm.set_fmm_params(16, 64, 0.5, 4.0)
+    out = {}
+    for name, weights in (("fixed", None), ("learned", w)):
+        fmm.set_learned(weights, pm, cm)
+        fmm.make_tree(xy, cir, cor)
+        uv = fmm.eval_sources()
+        l2, _ = rel_err(uv[idx], ref)
+        _, pairs = fmm.count_interactions()
+        out[name] = (l2, pairs)
+        _say(f"[learned A/B] {name}: rel_L2={l2:.2e} near pairs={pairs}")
+    fmm.set_learned(None)
+    assert out["learned"][0] <= 1e-6
+    assert out["learned"][1] <= 0.9 * out["fixed"][1]
