@@ -833,37 +833,43 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
   const bool row_ok = mi < pA;
   const double2 sApow = cpow_u(make_double2(sA, 0.0), mi + 1);  // s_A^(i+1), real
   FarAcc<MIXED> acc;
-  int pending = -1;
 
-  // one far cell pair (c0, c1; c1 may be -1) -> two MFMA steps
-  auto far_pair = [&](int c0, int c1) {
-    const int cell = mk < 2 ? c0 : c1;
-    double av = 0.0, b0 = 0.0, b1 = 0.0;
+  // Four far cells (c[k] < 0: none) -> up to two MFMA steps.  Lane group mk
+  // (16 lanes, row i = mi) forms (s_A w)^(i+1) and (-1)^(i+1) a_i w^i of
+  // cell c[mk] once; each MFMA step then takes the real and imaginary parts
+  // of two cells (K slots cell re, cell im, cell' re, cell' im) from the
+  // groups that formed them (lane permutes).
+  int pend[4] = {-1, -1, -1, -1};
+  int npend = 0;
+  auto far_quad = [&]() {
+    const int cell = pend[mk];
+    double2 pw = make_double2(0.0, 0.0), q = make_double2(0.0, 0.0);
     if (cell >= 0 && row_ok) {
       const NodeRec C = a.nodes[cell];
       const double dr = C.cx - NA.cx, di = C.cy - NA.cy;
       const double inv = 1.0 / (dr * dr + di * di);
       const double2 wv = make_double2(dr * inv, -di * inv);
-      const double2 wi = cpow_u(wv, mi);                                   // w^i
-      const double2 pw = cmul(cmul(wi, wv), make_double2(sApow.x, 0.0));  // (s_A w)^(i+1)
+      const double2 wi = cpow_u(wv, mi);                          // w^i
+      pw = cmul(cmul(wi, wv), make_double2(sApow.x, 0.0));        // (s_A w)^(i+1)
       const double2 ak = a.coef[(size_t)cell * a.p + mi];
-      double2 q = cmul(ak, wi);                                            // a_i w^i
-      if ((mi & 1) == 0) q = make_double2(-q.x, -q.y);                     // (-1)^(i+1)
-      const bool im = (mk & 1) != 0;
-      av = im ? pw.y : pw.x;
-      b0 = im ? -q.y : q.x;
-      b1 = im ? q.x : q.y;
+      q = cmul(ak, wi);                                           // a_i w^i
+      if ((mi & 1) == 0) q = make_double2(-q.x, -q.y);            // (-1)^(i+1)
     }
-    acc.add(av, b0, b1);
-    acc.tick();
+    const bool im = (mk & 1) != 0;
+    for (int st = 0; st < 2; st++) {
+      if (pend[2 * st] < 0 && pend[2 * st + 1] < 0) continue;  // wave-uniform
+      const int src = mi + 16 * (2 * st + (mk >> 1));
+      const double px = __shfl(pw.x, src, 64), py = __shfl(pw.y, src, 64);
+      const double qx = __shfl(q.x, src, 64), qy = __shfl(q.y, src, 64);
+      acc.add(im ? py : px, im ? -qy : qx, im ? qx : qy);
+      acc.tick();
+    }
+    pend[0] = pend[1] = pend[2] = pend[3] = -1;
+    npend = 0;
   };
   auto far = [&](int c) {
-    if (pending < 0) {
-      pending = c;
-    } else {
-      far_pair(pending, c);
-      pending = -1;
-    }
+    pend[npend++] = c;
+    if (npend == 4) far_quad();
   };
 
   // near work: summed here up to the first threshold, then only counted,
@@ -950,7 +956,7 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
     return true;
   };
   walk(a, L, ws, lane, far, near);
-  if (pending >= 0) far_pair(pending, -1);
+  if (npend > 0) far_quad();
   acc.fold();
 
   // local expansion: Lt_l = (1/s_A) sum_k C(l+k,k) G[l][k]; lane holds rows
@@ -1033,19 +1039,20 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
   wave_sync();
 }
 
-// One wave per target leaf, leaf = 4 * block + wave.  At least three waves
-// per SIMD: the walk is latency-bound and needs the others to hide it.
+// One wave per target leaf and one wave per workgroup: a finished wave's
+// slot is refilled at once (with several waves per workgroup the slot waits
+// for the workgroup's slowest leaf).  At least four waves per SIMD: the walk
+// is latency-bound and needs the others to hide it.
 template <bool MIXED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fmm_leaf_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fmm_leaf_kernel(
     FmmArgs a) {
-  __shared__ double2 s_loc[4][kFmmMaxP];
-  __shared__ double2 s_red[4][64];
-  __shared__ typename StageOf<MIXED>::T s_stage[4][kStageSlots];
-  __shared__ SplitCk s_ck[4][kMaxCk];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int li = blockIdx.x * 4 + w;
+  __shared__ double2 s_loc[1][kFmmMaxP];
+  __shared__ double2 s_red[1][64];
+  __shared__ typename StageOf<MIXED>::T s_stage[1][kStageSlots];
+  __shared__ SplitCk s_ck[1][kMaxCk];
+  const int li = blockIdx.x;
   if (li >= a.nleaves) return;
-  fmm_process_leaf<MIXED>(a, li, w, lane, s_loc, s_red, s_stage, s_ck);
+  fmm_process_leaf<MIXED>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck);
 }
 
 // Persistent form: a fixed grid whose waves take target leaves from a work
@@ -1227,11 +1234,10 @@ int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
     zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 2);
     FMM_HIP_TRY(hipGetLastError());
   }
-  const int nb = (args.nleaves + 3) / 4;
   if (mixed)
-    fmm_leaf_kernel<true><<<nb, 256, 0, s>>>(args);
+    fmm_leaf_kernel<true><<<args.nleaves, 64, 0, s>>>(args);
   else
-    fmm_leaf_kernel<false><<<nb, 256, 0, s>>>(args);
+    fmm_leaf_kernel<false><<<args.nleaves, 64, 0, s>>>(args);
   FMM_HIP_TRY(hipGetLastError());
   if (split) return launch_split_passes(args, mixed, d_nleaves, d_split_ctr + 1, split_blocks, s);
   return 0;
