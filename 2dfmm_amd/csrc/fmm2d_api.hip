@@ -97,6 +97,7 @@ struct Context {
   DevBuf<double2> direct_ws;  // partial sums of the direct judge
   double smax_h = 0.0;        // largest core radius of the tree's blobs
   double wmax_h = 0.0;        // log2(e) / (smallest core radius)^2, capped
+  double sw_h = 0.0;          // sqrt(wmax_h) when all core radii are equal, else 0
   // learned per-leaf parameters
   bool learned = false;
   float p_margin = 0.f, cut_margin = 0.f;
@@ -361,8 +362,9 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     g.rcut = g.fmm_cut * smm[0];
     // the FP32 pair kernel's w = log2(e) / sigma^2 is capped at 7.2e19
     g.wmax_h = smm[1] > 0.0 ? std::min(1.4426950408889634 / (smm[1] * smm[1]), 7.2e19) : 7.2e19;
+    g.sw_h = smm[0] == smm[1] ? std::sqrt(g.wmax_h) : 0.0;
     if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
-                          g.loc.p, g.loc64.p, s))
+                          g.loc.p, g.loc64.p, g.sw_h, s))
       return -1;
     if (g.nfirst.ensure(g.nnodes, s) || launch_node_first_blob(g.nodes.p, g.nnodes, g.nfirst.p, s))
       return -1;
@@ -425,6 +427,7 @@ static FmmArgs fmm_args(double *d_uv) {
   a.leaf_par = g.leaf_par_on ? g.leaf_par.p : nullptr;
   a.smax = g.smax_h;
   a.wmax = (float)g.wmax_h;
+  a.sw = g.sw_h;
   a.nfirst = g.nfirst.p;
   a.split_budget = g.split_budget;
   a.ck_cap = (int)g.ck.cap;

@@ -151,8 +151,10 @@ struct FmmArgs {
   // null = p and rcut for every leaf.  smax = largest core radius.
   const float2 *leaf_par;
   double smax;
-  // largest w = log2(e) / sigma^2 of the sources (mixed form)
+  // largest w = log2(e) / sigma^2 of the sources (mixed form); with one core
+  // radius sw = sqrt(w) scales the FP32 coordinates (0: radii differ)
   float wmax;
+  double sw;
   // first blob (in leaf order) of every node's subtree
   const int32_t *nfirst;
   // splitting of target leaves with much near work (split_budget 0: off):
@@ -167,9 +169,11 @@ struct FmmArgs {
 };
 int launch_mfma_selftest(const double *a, const double *b, double *out, double *ref, int fp32,
                          hipStream_t s);
+// sw > 0 (all core radii equal, sw = sqrt(log2(e) / sigma^2)): FP32 sources as
+// (x', y', cir sw, 1) with coordinates relative to the leaf centre times sw.
 int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleaves,
                       const double *sx, const double *sy, const double *sg, const double *ss,
-                      float4 *loc, double4 *loc64, hipStream_t s);
+                      float4 *loc, double4 *loc64, double sw, hipStream_t s);
 // Non-empty leaves in blob (Morton) order; their number lands in *d_count.
 int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, uint32_t *d_flag,
                      uint32_t *d_pos, int32_t *d_leaves, uint32_t *d_count, hipStream_t s);

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void leaf_local_kernel(const NodeRec *__restri
                                                          const double *__restrict__ sg,
                                                          const double *__restrict__ ss,
                                                          float4 *__restrict__ loc,
-                                                         double4 *__restrict__ loc64) {
+                                                         double4 *__restrict__ loc64, double sw) {
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (w >= nleaves) return;
   const NodeRec r = nodes[leaves[w]];
@@ -143,18 +143,24 @@ __global__ __launch_bounds__(256) void leaf_local_kernel(const NodeRec *__restri
     const int j = r.first + e;
     const double s2 = ss[j] * ss[j];
     loc64[j] = make_double4(sx[j], sy[j], sg[j], 1.0 / s2);
-    const double wj = fmin(1.4426950408889634 / s2, kMaxW);
-    loc[j] = make_float4((float)(sx[j] - r.cx), (float)(sy[j] - r.cy), (float)(sg[j] * wj),
-                         (float)wj);
+    if (sw > 0.0) {
+      // one core radius: coordinates times sqrt(w), z = cir sqrt(w), y = r'^2
+      loc[j] = make_float4((float)((sx[j] - r.cx) * sw), (float)((sy[j] - r.cy) * sw),
+                           (float)(sg[j] * sw), 1.0f);
+    } else {
+      const double wj = fmin(1.4426950408889634 / s2, kMaxW);
+      loc[j] = make_float4((float)(sx[j] - r.cx), (float)(sy[j] - r.cy), (float)(sg[j] * wj),
+                           (float)wj);
+    }
   }
 }
 
 int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleaves,
                       const double *sx, const double *sy, const double *sg, const double *ss,
-                      float4 *loc, double4 *loc64, hipStream_t s) {
+                      float4 *loc, double4 *loc64, double sw, hipStream_t s) {
   if (nleaves <= 0) return 0;
   leaf_local_kernel<<<(nleaves + 3) / 4, 256, 0, s>>>(d_nodes, d_leaves, nleaves, sx, sy, sg, ss,
-                                                      loc, loc64);
+                                                      loc, loc64, sw);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -387,7 +393,7 @@ struct LeafCtx {
   float lx, ly;
 };
 
-template <bool MIXED>
+template <bool MIXED, bool UNI>
 __device__ inline LeafCtx leaf_ctx(const FmmArgs &a, int li, int lane) {
   LeafCtx L;
   L.NA = ldro(a.nodes + rfl(ldro(a.leaves + li)));
@@ -433,11 +439,11 @@ __device__ inline int64_t near_units(int cnt, int nrep) {
 // source at (s.x, s.y) in the same FP32 frame: u += -f ry, v += f rx with
 // f = z h(y), y = r^2 s.w (see h_series).  SERIES = false for near items
 // whose pairs all have y >= kSeriesY (needs_series): no check at all.
-template <bool SERIES>
+template <bool SERIES, bool UNI>
 __device__ inline void pair32(float px, float py, const float4 s, float &gx, float &gy) {
   const float rx = px - s.x, ry = py - s.y;
   const float r2 = fmaf(rx, rx, ry * ry);
-  const float y = r2 * s.w;
+  const float y = UNI ? r2 : r2 * s.w;
   // z (1 - 2^-y) / y with z - z 2^-y in one fused multiply-add
   float fac = fmaf(-s.z, __builtin_amdgcn_exp2f(-y), s.z) * __builtin_amdgcn_rcpf(y);
   if (SERIES) {
@@ -467,7 +473,7 @@ __device__ inline void pair64(double tx, double ty, const double4 s, double &ax,
 // The sources of one replica: source j = rep + k nrep (< cnt) at row[k] of
 // the replica-major stage.  Groups of eight pairs are summed in FP32 and
 // accumulated in FP64; the row is read with immediate offsets.
-template <bool SERIES>
+template <bool SERIES, bool UNI>
 __device__ inline void lds_sum32(float px, float py, const float4 *row, int cnt, int rep, int nrep,
                                  double &ax, double &ay) {
   int j = rep;
@@ -479,13 +485,13 @@ __device__ inline void lds_sum32(float px, float py, const float4 *row, int cnt,
     for (int k = 0; k < 8; k++) v[k] = row[k];
     float gx = 0.f, gy = 0.f;
 #pragma unroll
-    for (int k = 0; k < 8; k++) pair32<SERIES>(px, py, v[k], gx, gy);
+    for (int k = 0; k < 8; k++) pair32<SERIES, UNI>(px, py, v[k], gx, gy);
     ax += (double)gx;
     ay += (double)gy;
   }
   if (j < cnt) {
     float gx = 0.f, gy = 0.f;
-    for (; j < cnt; j += nrep) pair32<SERIES>(px, py, *row++, gx, gy);
+    for (; j < cnt; j += nrep) pair32<SERIES, UNI>(px, py, *row++, gx, gy);
     ax += (double)gx;
     ay += (double)gy;
   }
@@ -503,7 +509,7 @@ __device__ inline void lds_sum64(double tx, double ty, const double4 *row, int c
 
 // All sources of a near leaf for one target, read at wave-uniform indices
 // (scalar loads), groups of four pairs summed in FP32, accumulated in FP64.
-template <bool SERIES>
+template <bool SERIES, bool UNI>
 __device__ inline void scalar_sum32(float px, float py, const float4 *src, int cnt, double &ax,
                                     double &ay) {
   int j = 0;
@@ -519,17 +525,17 @@ __device__ inline void scalar_sum32(float px, float py, const float4 *src, int c
         n3 = ldro(src + j + 7);
       }
       float gx = 0.f, gy = 0.f;
-      pair32<SERIES>(px, py, s0, gx, gy);
-      pair32<SERIES>(px, py, s1, gx, gy);
-      pair32<SERIES>(px, py, s2, gx, gy);
-      pair32<SERIES>(px, py, s3, gx, gy);
+      pair32<SERIES, UNI>(px, py, s0, gx, gy);
+      pair32<SERIES, UNI>(px, py, s1, gx, gy);
+      pair32<SERIES, UNI>(px, py, s2, gx, gy);
+      pair32<SERIES, UNI>(px, py, s3, gx, gy);
       ax += (double)gx;
       ay += (double)gy;
     }
   }
   if (j < cnt) {
     float gx = 0.f, gy = 0.f;
-    for (; j < cnt; j++) pair32<SERIES>(px, py, ldro(src + j), gx, gy);
+    for (; j < cnt; j++) pair32<SERIES, UNI>(px, py, ldro(src + j), gx, gy);
     ax += (double)gx;
     ay += (double)gy;
   }
@@ -550,31 +556,33 @@ __device__ inline bool needs_series(const FmmArgs &a, const LeafCtx &L, const No
 // uniform and the sources come through the scalar cache; otherwise the
 // leaf's sources are staged once in this wave's LDS slice `st` and read from
 // there, so each source crosses the vector memory path once per wave.
-template <bool MIXED>
+template <bool MIXED, bool UNI>
 __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeRec &C, int lane,
                                  typename StageOf<MIXED>::T *st, double &ax, double &ay) {
   const int cnt = node_count(C.cnt);
   const int f = rfl(C.first);
   const int nrep = L.nrep;
   if (MIXED) {
-    // sources are stored relative to their own leaf's centre
-    const float dx = (float)(C.cx - L.NA.cx), dy = (float)(C.cy - L.NA.cy);
+    // sources are stored relative to their own leaf's centre (times sqrt(w)
+    // with one core radius)
+    const float dx = UNI ? (float)((C.cx - L.NA.cx) * a.sw) : (float)(C.cx - L.NA.cx);
+    const float dy = UNI ? (float)((C.cy - L.NA.cy) * a.sw) : (float)(C.cy - L.NA.cy);
     const float px = L.lx - dx, py = L.ly - dy;
     const bool ser = needs_series(a, L, C);
     if (nrep == 1) {
       if (ser)
-        scalar_sum32<true>(px, py, a.loc + f, cnt, ax, ay);
+        scalar_sum32<true, UNI>(px, py, a.loc + f, cnt, ax, ay);
       else
-        scalar_sum32<false>(px, py, a.loc + f, cnt, ax, ay);
+        scalar_sum32<false, UNI>(px, py, a.loc + f, cnt, ax, ay);
     } else {
       float4 *sf = reinterpret_cast<float4 *>(st);
       if (lane < cnt) sf[L.spos] = a.loc[f + lane];
       wave_sync();
       if (L.tact) {
         if (ser)
-          lds_sum32<true>(px, py, sf + L.rep * L.stride, cnt, L.rep, nrep, ax, ay);
+          lds_sum32<true, UNI>(px, py, sf + L.rep * L.stride, cnt, L.rep, nrep, ax, ay);
         else
-          lds_sum32<false>(px, py, sf + L.rep * L.stride, cnt, L.rep, nrep, ax, ay);
+          lds_sum32<false, UNI>(px, py, sf + L.rep * L.stride, cnt, L.rep, nrep, ax, ay);
       }
       wave_sync();
     }
@@ -606,7 +614,7 @@ __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeR
 // leaves.  Each tile is staged in LDS -- the mixed form converts each source
 // once, to FP32 coordinates relative to the target leaf's centre -- and the
 // next tile's loads are issued before the current tile's sums.
-template <bool MIXED>
+template <bool MIXED, bool UNI>
 __device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
                                   int ta, int tb, int lane, typename StageOf<MIXED>::T *st,
                                   double &ax, double &ay) {
@@ -631,14 +639,16 @@ __device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int c, con
     if (MIXED) {
       float4 *sf = reinterpret_cast<float4 *>(st);
       if (lane < tc)
-        sf[L.spos] = make_float4((float)(q.x - L.NA.cx), (float)(q.y - L.NA.cy), zw.x, zw.y);
+        sf[L.spos] = UNI ? make_float4((float)((q.x - L.NA.cx) * a.sw),
+                                       (float)((q.y - L.NA.cy) * a.sw), zw.x, zw.y)
+                         : make_float4((float)(q.x - L.NA.cx), (float)(q.y - L.NA.cy), zw.x, zw.y);
       wave_sync();
       if (t + 1 < tb) load(t + 1);
       if (L.tact) {
         if (ser)
-          lds_sum32<true>(L.lx, L.ly, sf + L.rep * L.stride, tc, L.rep, L.nrep, ax, ay);
+          lds_sum32<true, UNI>(L.lx, L.ly, sf + L.rep * L.stride, tc, L.rep, L.nrep, ax, ay);
         else
-          lds_sum32<false>(L.lx, L.ly, sf + L.rep * L.stride, tc, L.rep, L.nrep, ax, ay);
+          lds_sum32<false, UNI>(L.lx, L.ly, sf + L.rep * L.stride, tc, L.rep, L.nrep, ax, ay);
       }
     } else {
       double4 *sd = reinterpret_cast<double4 *>(st);
@@ -675,15 +685,15 @@ __device__ inline int tile_count(const NodeRec &C, bool range, int t) {
 }
 
 // Steps ta..tb-1 of near item c.
-template <bool MIXED>
+template <bool MIXED, bool UNI>
 __device__ inline void near_steps(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
                                   bool range, int ta, int tb, int lane,
                                   typename StageOf<MIXED>::T *st, double &ax,
                                   double &ay) {
   if (range)
-    near_range<MIXED>(a, L, c, C, ta, tb, lane, st, ax, ay);
+    near_range<MIXED, UNI>(a, L, c, C, ta, tb, lane, st, ax, ay);
   else
-    near_leaf<MIXED>(a, L, C, lane, st, ax, ay);
+    near_leaf<MIXED, UNI>(a, L, C, lane, st, ax, ay);
 }
 
 // The far test of a cell C for target leaf A: R_A + R_C <= theta d and
@@ -783,7 +793,7 @@ __device__ inline double2 reduce_replicas(const LeafCtx &L, int lane, double2 *r
 // Near work of one chunk: resume the walk at checkpoint r (its item, from
 // tile r.off) and sum near steps until the work count reaches `end`; far
 // cells are skipped (the fused pass took them).
-template <bool MIXED>
+template <bool MIXED, bool UNI>
 __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const SplitCk &r, int64_t end,
                                  int lane, typename StageOf<MIXED>::T *st, double &ax,
                                  double &ay) {
@@ -804,7 +814,7 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const Split
     int te = t0;
     int64_t cu = cum;
     while (te < nt && cu < stop) cu += near_units(tile_count(C, range, te++), L.nrep);
-    if (te > t0) near_steps<MIXED>(a, L, c, C, range, t0, te, lane, st, ax, ay);
+    if (te > t0) near_steps<MIXED, UNI>(a, L, c, C, range, t0, te, lane, st, ax, ay);
     cum = cu;
     return te == nt;
   };
@@ -817,13 +827,13 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const Split
 }
 
 // The fused pass for target leaf li (index in the leaf list), one wave.
-template <bool MIXED>
+template <bool MIXED, bool UNI>
 __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
                                  double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
                                  typename StageOf<MIXED>::T (*s_stage)[kStageSlots],
                                  SplitCk (*s_ck)[kMaxCk]) {
   li = rfl(li);
-  const LeafCtx L = leaf_ctx<MIXED>(a, li, lane);
+  const LeafCtx L = leaf_ctx<MIXED, UNI>(a, li, lane);
   const NodeRec &NA = L.NA;
   const double sA = L.sA;
   const int pA = L.pA;
@@ -891,7 +901,7 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
       // the steps that start below the first threshold are summed here
       int64_t cu = cum;
       while (t < nt && cu < thr) cu += near_units(tile_count(C, range, t++), L.nrep);
-      if (t > 0) near_steps<MIXED>(a, L, c, C, range, 0, t, lane, s_stage[w], ax, ay);
+      if (t > 0) near_steps<MIXED, UNI>(a, L, c, C, range, 0, t, lane, s_stage[w], ax, ay);
       cum = cu;
     }
     for (; t < nt; t++) {
@@ -1018,7 +1028,7 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
           const SplitCk r = s_ck[w][k];
           const int64_t end = k + 1 < nck ? s_ck[w][k + 1].cum : INT64_MAX;
           double cx = 0.0, cy = 0.0;
-          run_chunk<MIXED>(a, L, r, end, lane, s_stage[w], cx, cy);
+          run_chunk<MIXED, UNI>(a, L, r, end, lane, s_stage[w], cx, cy);
           const double2 P = reduce_replicas(L, lane, s_red[w], cx, cy);
           extra.x += P.x;
           extra.y += P.y;
@@ -1043,7 +1053,7 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
 // slot is refilled at once (with several waves per workgroup the slot waits
 // for the workgroup's slowest leaf).  At least four waves per SIMD: the walk
 // is latency-bound and needs the others to hide it.
-template <bool MIXED>
+template <bool MIXED, bool UNI>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fmm_leaf_kernel(
     FmmArgs a) {
   __shared__ double2 s_loc[1][kFmmMaxP];
@@ -1052,7 +1062,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fm
   __shared__ SplitCk s_ck[1][kMaxCk];
   const int li = blockIdx.x;
   if (li >= a.nleaves) return;
-  fmm_process_leaf<MIXED>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck);
+  fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck);
 }
 
 // Persistent form: a fixed grid whose waves take target leaves from a work
@@ -1060,7 +1070,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fm
 // once the counter passes the count, so the grid always drains.  Nothing in
 // the launch depends on host-side counts, so it can be captured in a graph
 // and replayed on a rebuilt tree.
-template <bool MIXED>
+template <bool MIXED, bool UNI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fmm_leaf_persistent_kernel(FmmArgs a,
                                                                   const uint32_t *__restrict__ nleaves,
                                                                   uint32_t *__restrict__ counter) {
@@ -1075,14 +1085,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
     if (lane == 0) li = atomicAdd(counter, 1u);
     li = __shfl(li, 0, 64);
     if (li >= total) break;
-    fmm_process_leaf<MIXED>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck);
+    fmm_process_leaf<MIXED, UNI>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck);
   }
 }
 
 // Continuation pass: waves take checkpoint records from a work counter; the
 // partial sums of record j go to part[64 j + target].  Exits when the counter
 // passes the number of records written by the fused pass.
-template <bool MIXED>
+template <bool MIXED, bool UNI>
 __global__ __launch_bounds__(256) void fmm_chunk_kernel(FmmArgs a, uint32_t *__restrict__ counter) {
   __shared__ double2 s_red[4][64];
   __shared__ typename StageOf<MIXED>::T s_stage[4][kStageSlots];
@@ -1096,9 +1106,9 @@ __global__ __launch_bounds__(256) void fmm_chunk_kernel(FmmArgs a, uint32_t *__r
     const SplitCk r = ldro(a.ck + j);
     const int li = rfl(r.li);
     if (li < 0) continue;
-    const LeafCtx L = leaf_ctx<MIXED>(a, li, lane);
+    const LeafCtx L = leaf_ctx<MIXED, UNI>(a, li, lane);
     double ax = 0.0, ay = 0.0;
-    run_chunk<MIXED>(a, L, r, r.end, lane, s_stage[w], ax, ay);
+    run_chunk<MIXED, UNI>(a, L, r, r.end, lane, s_stage[w], ax, ay);
     const double2 P = reduce_replicas(L, lane, s_red[w], ax, ay);
     if (L.tact && L.rep == 0) a.part[(size_t)j * 64 + L.ti] = P;
   }
@@ -1136,10 +1146,12 @@ __global__ void zero_counters_kernel(uint32_t *c, int n) {
 
 static int launch_split_passes(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
                                uint32_t *d_chunk_counter, int nblocks, hipStream_t s) {
-  if (mixed)
-    fmm_chunk_kernel<true><<<nblocks, 256, 0, s>>>(args, d_chunk_counter);
+  if (mixed && args.sw > 0.f)
+    fmm_chunk_kernel<true, true><<<nblocks, 256, 0, s>>>(args, d_chunk_counter);
+  else if (mixed)
+    fmm_chunk_kernel<true, false><<<nblocks, 256, 0, s>>>(args, d_chunk_counter);
   else
-    fmm_chunk_kernel<false><<<nblocks, 256, 0, s>>>(args, d_chunk_counter);
+    fmm_chunk_kernel<false, false><<<nblocks, 256, 0, s>>>(args, d_chunk_counter);
   FMM_HIP_TRY(hipGetLastError());
   fmm_split_sum_kernel<<<nblocks, 256, 0, s>>>(args, d_nleaves);
   FMM_HIP_TRY(hipGetLastError());
@@ -1156,10 +1168,12 @@ int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *
     zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 2);
     FMM_HIP_TRY(hipGetLastError());
   }
-  if (mixed)
-    fmm_leaf_persistent_kernel<true><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
+  if (mixed && args.sw > 0.f)
+    fmm_leaf_persistent_kernel<true, true><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
+  else if (mixed)
+    fmm_leaf_persistent_kernel<true, false><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
   else
-    fmm_leaf_persistent_kernel<false><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
+    fmm_leaf_persistent_kernel<false, false><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
   FMM_HIP_TRY(hipGetLastError());
   if (split) return launch_split_passes(args, mixed, d_nleaves, d_split_ctr + 1, split_blocks, s);
   return 0;
@@ -1234,10 +1248,12 @@ int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
     zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 2);
     FMM_HIP_TRY(hipGetLastError());
   }
-  if (mixed)
-    fmm_leaf_kernel<true><<<args.nleaves, 64, 0, s>>>(args);
+  if (mixed && args.sw > 0.f)
+    fmm_leaf_kernel<true, true><<<args.nleaves, 64, 0, s>>>(args);
+  else if (mixed)
+    fmm_leaf_kernel<true, false><<<args.nleaves, 64, 0, s>>>(args);
   else
-    fmm_leaf_kernel<false><<<args.nleaves, 64, 0, s>>>(args);
+    fmm_leaf_kernel<false, false><<<args.nleaves, 64, 0, s>>>(args);
   FMM_HIP_TRY(hipGetLastError());
   if (split) return launch_split_passes(args, mixed, d_nleaves, d_split_ctr + 1, split_blocks, s);
   return 0;

@@ -110,3 +110,32 @@ def test_fmm_mode_biot_tree(fmm):
     for i in range(0, 5000, 499):
         assert fmm.biot_tree(*xy[i]) == tuple(uv[i])
     fmm.set_mode(fmm.MODE_TREECODE)
+
+
+def test_fmm_varying_core_radii(fmm, oracle):
+    """Core radii that differ between blobs use the general FP32 encoding
+    (w = log2(e)/sigma^2 per source); equal radii use coordinates scaled by
+    sqrt(w).  Both against the direct sum, and the two encodings against each
+    other on one layout (one radius nudged by 1e-12 selects the general one)."""
+    wl = workloads()
+    n = 30000
+    xy, cir, _ = wl.gaussian_mixture(n, seed=58)
+    rng = np.random.default_rng(58)
+    cor = 5e-4 + 1.5e-3 * rng.random(n)
+    idx = wl.sample_targets(n, 1200, seed=58)
+    d = oracle.direct(xy, cir, cor, xy[idx], style=1)
+    for mode, tol in ((fmm.MODE_FMM64, 5e-7), (fmm.MODE_MIXED, 1e-6)):
+        l2, mx = rel_err(_run(fmm, mode, xy, cir, cor)[idx], d)
+        _say(f"  varying radii mode {mode}: rel_L2={l2:.2e} rel_max={mx:.2e}")
+        assert l2 <= tol
+    cu = np.full(n, 1e-3)
+    cn = cu.copy()
+    cn[0] *= 1.0 + 1e-12
+    uu = _run(fmm, fmm.MODE_MIXED, xy, cir, cu)
+    un = _run(fmm, fmm.MODE_MIXED, xy, cir, cn)
+    diff = np.abs(uu - un).max() / np.abs(uu).max()
+    _say(f"  equal-radius vs general FP32 encoding: max diff {diff:.2e}")
+    assert diff <= 1e-5
+    du = oracle.direct(xy, cir, cu, xy[idx], style=1)
+    assert rel_err(uu[idx], du)[0] <= 1e-6
+    fmm.set_mode(fmm.MODE_TREECODE)
