@@ -68,6 +68,9 @@ lib.fmm_make_forest_device.argtypes = [C.c_int, _ip, _vp, _vp, _vp]
 lib.fmm_set_persistent.argtypes = [C.c_int, C.c_int]
 lib.fmm_set_split.argtypes = [C.c_int]
 lib.fmm_get_split.argtypes = []
+lib.fmm_set_smooth.argtypes = [C.c_double, C.c_int]
+lib.fmm_get_smooth.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
+lib.fmm_count_work.argtypes = [C.POINTER(C.c_longlong)] * 3
 lib.fmm_device_cus.argtypes = []
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 lib.fmm_set_learned.argtypes = [C.c_void_p, C.c_float, C.c_float]
@@ -319,6 +322,19 @@ def get_split() -> int:
     return int(lib.fmm_get_split())
 
 
+def set_smooth(eta: float, min_blobs: int = 24) -> None:
+    """Smooth cells of the mixed mode (one core radius): near nodes of half side
+    <= eta * sigma with >= min_blobs blobs are summed through a Taylor expansion
+    of the regularised kernel (eta 0: off).  Takes effect at the next build."""
+    _check(lib.fmm_set_smooth(float(eta), int(min_blobs)), "set_smooth")
+
+
+def get_smooth():
+    e, m = C.c_double(), C.c_int()
+    _check(lib.fmm_get_smooth(C.byref(e), C.byref(m)), "get_smooth")
+    return e.value, m.value
+
+
 def device_cus() -> int:
     return int(lib.fmm_device_cus())
 
@@ -426,6 +442,13 @@ def count_interactions():
     a, b = C.c_longlong(), C.c_longlong()
     _check(lib.fmm_count_interactions(C.byref(a), C.byref(b)), "count_interactions")
     return a.value, b.value
+
+
+def count_work():
+    """(far cells, pairs summed directly, (target, smooth cell) evaluations)."""
+    a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()
+    _check(lib.fmm_count_work(C.byref(a), C.byref(b), C.byref(c)), "count_work")
+    return a.value, b.value, c.value
 
 
 def synchronize() -> None:

@@ -166,4 +166,193 @@ int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int npare
   return 0;
 }
 
+// ---- smooth cells ----------------------------------------------------------
+// A cell small against the core radius sigma is summed for every target by a
+// Taylor expansion of the regularised kernel about the cell centre c.  In core
+// radii (z = (x - x_j) / sigma, complex) blob j adds cir_j f(z) / sigma to
+// F, the velocity being (u, v) = (Im F, Re F) / (2 pi), with f(z) = (1 - e^{-z zb}) / z,
+// zb = conj z, an entire function of (z, zb).  With Z = (x - c) / sigma and
+// d_j = (c - x_j) / sigma,
+//   sum_j cir_j f(Z + d_j) = sum_{m+n<=q} T_mn(Z) mu_mn,
+//   T_mn = d^m/dz^m d^n/dzb^n f,   mu_mn = sum_j cir_j d_j^m conj(d_j)^n / (m! n!),
+// the error falling like (|d| / sigma)^(q+1).  Closed forms (E = e^{-z zb}):
+//   T_m0 = zb^(m+1) phi^(m)(z zb),  phi(w) = (1 - e^-w) / w    (f = zb phi),
+//   T_mn = E H_{m,n-1} for n >= 1,   since d/dzb f = E, and
+//   H_mk E = d^m/dz^m d^k/dzb^k E,  H_mk = sum_j C(m,j) k!/(k-j)! (-1)^(m+k-j) z^(k-j) zb^(m-j),
+// so the n >= 1 part is E P(z, zb) with P = sum_{i+j<=q-1} c_ij z^i zb^j,
+//   c_ij = sum_l mu_{j+l, i+l+1} C(j+l, l) (i+l)!/i! (-1)^(i+j+l).
+// The coefficients are stored for the scaled coordinates of the evaluation,
+// Z = lam Z' with lam = sqrt(ln 2) (Z' in units of sigma / sqrt(log2 e), where
+// e^{-|Z|^2} = 2^{-|Z'|^2}), and divided by sigma: nu_m lam^(m+1) / sigma for
+// m = q .. 0, then c_ij lam^(i+j) / sigma in Horner order.
+__device__ inline int smooth_tri(int m) { return m * (kSmoothQ + 1) - m * (m - 1) / 2; }
+
+// One wave per node; lane t < (q+1)(q+2)/2 sums term (m, n) over the node's
+// blobs (FP64), then lane k < kSmoothC forms output coefficient k.
+__global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restrict__ nodes,
+                                                         int nnodes,
+                                                         const int32_t *__restrict__ nfirst,
+                                                         const double *__restrict__ sx,
+                                                         const double *__restrict__ sy,
+                                                         const double *__restrict__ sg,
+                                                         double inv_sigma, double region,
+                                                         int smin, float4 *__restrict__ out) {
+  constexpr int Q = kSmoothQ;
+  constexpr int NT = (Q + 1) * (Q + 2) / 2;
+  __shared__ double2 mu[NT];
+  const int node = blockIdx.x;
+  if (node >= nnodes) return;
+  const NodeRec C = nodes[node];
+  const int cnt = node_count(C.cnt);
+  if (!(C.region <= region && 2.0 * C.region > region && cnt >= smin)) return;
+  const int lane = threadIdx.x;
+  double fact[2 * Q + 2];
+  fact[0] = 1.0;
+  for (int k = 1; k < 2 * Q + 2; k++) fact[k] = fact[k - 1] * (double)k;
+  if (lane < NT) {
+    int m = 0;
+    while (lane >= smooth_tri(m + 1)) m++;
+    const int n = lane - smooth_tri(m);
+    const int f = nfirst[node];
+    double ar = 0.0, ai = 0.0;
+    for (int j = f; j < f + cnt; j++) {
+      const double dr = (C.cx - sx[j]) * inv_sigma, di = (C.cy - sy[j]) * inv_sigma;
+      double pr = sg[j], pi = 0.0;
+      for (int k = 0; k < m; k++) {
+        const double t = pr * dr - pi * di;
+        pi = pr * di + pi * dr;
+        pr = t;
+      }
+      for (int k = 0; k < n; k++) {  // times conj(d)
+        const double t = pr * dr + pi * di;
+        pi = pi * dr - pr * di;
+        pr = t;
+      }
+      ar += pr;
+      ai += pi;
+    }
+    const double s = 1.0 / (fact[m] * fact[n]);
+    mu[lane] = make_double2(ar * s, ai * s);
+  }
+  __syncthreads();
+  if (lane <= 2 * (kSmoothF4 - 1) + 1) {
+    const double lam = 0.83255461115769775635;  // sqrt(ln 2)
+    double vr = 0.0, vi = 0.0;
+    if (lane <= Q) {
+      const int m = Q - lane;
+      const double2 v = mu[smooth_tri(m)];
+      const double sc = pow(lam, (double)(m + 1));
+      vr = v.x * sc;
+      vi = v.y * sc;
+    } else if (lane < kSmoothC) {
+      int k = lane - (Q + 1), j = Q - 1;
+      while (k >= Q - j) {
+        k -= Q - j;
+        j--;
+      }
+      const int i = Q - 1 - j - k;
+      for (int l = 0; i + j + 2 * l + 1 <= Q; l++) {
+        const int m = j + l, n = i + l + 1;
+        const double2 v = mu[smooth_tri(m) + n];
+        double c = fact[j + l] / (fact[l] * fact[j]) * fact[i + l] / fact[i];
+        if ((i + j + l) & 1) c = -c;
+        vr += v.x * c;
+        vi += v.y * c;
+      }
+      const double sc = pow(lam, (double)(i + j));
+      vr *= sc;
+      vi *= sc;
+    }
+    reinterpret_cast<float2 *>(out + (size_t)node * kSmoothF4)[lane] =
+        make_float2((float)(vr * inv_sigma), (float)(vi * inv_sigma));
+  }
+}
+
+int launch_smooth_coef(const NodeRec *d_nodes, int nnodes, const int32_t *d_nfirst,
+                       const double *sx, const double *sy, const double *sg, double sigma,
+                       double region, int smin, float4 *d_out, hipStream_t s) {
+  if (nnodes <= 0) return 0;
+  smooth_coef_kernel<<<nnodes, 64, 0, s>>>(d_nodes, nnodes, d_nfirst, sx, sy, sg, 1.0 / sigma,
+                                           region, smin, d_out);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// flag[n] = 1: a leaf above the smooth level holds blobs of node n's
+// subtree; flag[nnodes + n] = 1: a smooth cell (smooth level, >= smin blobs)
+// lies in it.  Every such node marks its ancestors (each writer stores 1).
+__global__ __launch_bounds__(256) void smooth_mark_kernel(const NodeRec *__restrict__ nodes,
+                                                          const int32_t *__restrict__ gparent,
+                                                          int nnodes, int nroot_slots, double region,
+                                                          int smin, uint8_t *__restrict__ flag) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nnodes) return;
+  const NodeRec C = nodes[i];
+  const int cnt = node_count(C.cnt);
+  uint8_t *f = nullptr;
+  if (!node_internal(C.cnt) && cnt > 0 && C.region > region)
+    f = flag;
+  else if (C.region <= region && 2.0 * C.region > region && cnt >= smin)
+    f = flag + nnodes;
+  if (!f) return;
+  int n = i;
+  while (n >= nroot_slots) {
+    n = gparent[n >> 2];
+    if (n < 0) break;
+    f[n] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void smooth_range_kernel(const NodeRec *__restrict__ nodes,
+                                                           int nnodes, double region,
+                                                           const uint8_t *__restrict__ flag,
+                                                           int2 *__restrict__ range) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nnodes) return;
+  const NodeRec C = nodes[i];
+  int2 r = make_int2(-1, -1);
+  if (node_internal(C.cnt) && C.region > region && flag[nnodes + i] && flag[i])
+    r = make_int2(-2, -2);
+  else if (node_internal(C.cnt) && C.region > region && flag[nnodes + i]) {
+    // leftmost and rightmost non-empty paths down to the smooth level (all
+    // internal above it: no leaf there holds blobs); a path that meets a
+    // leaf leaves the node to be opened
+    int lo = i, hi = i;
+    NodeRec a = C, b = C;
+    bool ok = true;
+    while (ok && a.region > region) {
+      ok = node_internal(a.cnt) && a.first >= 0 && a.first + 3 < nnodes;
+      if (!ok) break;
+      int k = 0;
+      while (k < 3 && node_count(nodes[a.first + k].cnt) == 0) k++;
+      lo = a.first + k;
+      a = nodes[lo];
+    }
+    while (ok && b.region > region) {
+      ok = node_internal(b.cnt) && b.first >= 0 && b.first + 3 < nnodes;
+      if (!ok) break;
+      int k = 3;
+      while (k > 0 && node_count(nodes[b.first + k].cnt) == 0) k--;
+      hi = b.first + k;
+      b = nodes[hi];
+    }
+    r = ok ? make_int2(lo, hi + 1) : make_int2(-2, -2);
+  }
+  range[i] = r;
+}
+
+int launch_smooth_ranges(const NodeRec *d_nodes, const int32_t *d_gparent, int nnodes,
+                         int nroot_slots, double region, int smin, uint8_t *d_flag, int2 *d_range,
+                         hipStream_t s) {
+  if (nnodes <= 0) return 0;
+  const int nb = (nnodes + 255) / 256;
+  FMM_HIP_TRY(hipMemsetAsync(d_flag, 0, 2 * (size_t)nnodes, s));
+  smooth_mark_kernel<<<nb, 256, 0, s>>>(d_nodes, d_gparent, nnodes, nroot_slots, region, smin,
+                                        d_flag);
+  FMM_HIP_TRY(hipGetLastError());
+  smooth_range_kernel<<<nb, 256, 0, s>>>(d_nodes, nnodes, region, d_flag, d_range);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 }  // namespace fmm2d

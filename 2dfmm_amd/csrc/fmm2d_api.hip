@@ -123,6 +123,16 @@ struct Context {
   DevBuf<double2> part;
   DevBuf<uint32_t> split_ctr;
   int ncu = 0;
+  // smooth cells of the mixed form (one core radius): half side <= eta sigma,
+  // at least smooth_min blobs; smooth_region_h = 2 eta sigma of the current
+  // tree (0: none built)
+  double smooth_eta = 0.3;
+  int smooth_min = 24;
+  double smooth_region_h = 0.0;
+  int t_smooth_min = 0;  // smooth_min the current tree's coefficients used
+  DevBuf<float4> scoef;
+  DevBuf<uint8_t> sflag;
+  DevBuf<int2> srange;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -235,6 +245,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   g.nsys = nsys;
   g.root_shift = -1;
   g.leaf_par_on = false;
+  g.smooth_region_h = 0.0;
   g.n_real = -1;
   g.t_mode = g.mode;
   if (g.mode == FMM2D_MODE_TREECODE) {
@@ -368,6 +379,22 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
       return -1;
     if (g.nfirst.ensure(g.nnodes, s) || launch_node_first_blob(g.nodes.p, g.nnodes, g.nfirst.p, s))
       return -1;
+    // smooth cells: mixed form with one core radius only
+    if (g.t_mode == FMM2D_MODE_MIXED && g.sw_h > 0.0 && g.smooth_eta > 0.0) {
+      const double region = 2.0 * g.smooth_eta * smm[0];
+      // every internal node (>= num_p blobs) of the smooth level is a smooth cell
+      const int smin = std::min(g.smooth_min, g.t_num_p);
+      if (g.scoef.ensure((size_t)g.nnodes * kSmoothF4, s) ||
+          launch_smooth_coef(g.nodes.p, g.nnodes, g.nfirst.p, g.sx.p, g.sy.p, g.sg.p, smm[0],
+                             region, smin, g.scoef.p, s))
+        return -1;
+      if (g.sflag.ensure(2 * (size_t)g.nnodes, s) || g.srange.ensure(g.nnodes, s) ||
+          launch_smooth_ranges(g.nodes.p, g.gparent.p, g.nnodes, nroot_slots, region, smin,
+                               g.sflag.p, g.srange.p, s))
+        return -1;
+      g.smooth_region_h = region;
+      g.t_smooth_min = smin;
+    }
     // valid already for the fused kernel's arguments used by the learned step
     g.valid = true;
     if (g.learned && apply_learned()) {
@@ -435,6 +462,10 @@ static FmmArgs fmm_args(double *d_uv) {
   a.ck_count = g.split_ctr.p;
   a.leaf_ck = g.leaf_ck.p;
   a.part = g.part.p;
+  a.smooth_region = g.smooth_region_h;
+  a.smooth_min = g.t_smooth_min;
+  a.scoef = g.scoef.p;
+  a.srange = g.srange.p;
   return a;
 }
 
@@ -979,6 +1010,25 @@ int fmm_get_split(void) {
   return g.split_budget;
 }
 
+int fmm_set_smooth(double eta, int min_blobs) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!(eta >= 0.0) || eta > 0.35 || min_blobs < 1) {
+    set_error("fmm_set_smooth: eta=%g (0 = off, at most 0.35), min_blobs=%d (>= 1)", eta,
+              min_blobs);
+    return -1;
+  }
+  g.smooth_eta = eta;
+  g.smooth_min = min_blobs;
+  return 0;
+}
+
+int fmm_get_smooth(double *eta, int *min_blobs) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (eta) *eta = g.smooth_eta;
+  if (min_blobs) *min_blobs = g.smooth_min;
+  return 0;
+}
+
 // ---- learned per-leaf parameters -------------------------------------------
 int fmm_set_learned(const float *weights, float p_margin, float cut_margin) {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -1453,16 +1503,17 @@ int fmm_get_timing(double *ms, int n) {
   return 0;
 }
 
-int fmm_count_interactions(long long *m2s, long long *pairs) {
-  std::lock_guard<std::mutex> lk(g_mu);
+// Work of the evaluation with all blobs as targets (see fmm_count_work).
+static int count_work(long long *m2s, long long *pairs, long long *smooth) {
   if (need_tree()) return -1;
   *m2s = 0;
   *pairs = 0;
+  *smooth = 0;
   if (g.n == 0) return 0;
   if (g.t_mode != FMM2D_MODE_TREECODE) {
     DevBuf<int64_t> c;
-    if (c.ensure(2 * (size_t)std::max(g.ntleaves, 1), g.stream)) return -1;
-    std::vector<int64_t> h(2 * (size_t)g.ntleaves);
+    if (c.ensure(3 * (size_t)std::max(g.ntleaves, 1), g.stream)) return -1;
+    std::vector<int64_t> h(3 * (size_t)g.ntleaves);
     int rc = launch_fmm_count(fmm_args(nullptr), c.p, g.stream);
     if (!rc && (hipMemcpyAsync(h.data(), c.p, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost,
                                g.stream) != hipSuccess ||
@@ -1472,13 +1523,15 @@ int fmm_count_interactions(long long *m2s, long long *pairs) {
     }
     c.release();
     if (rc) return rc;
-    long long a = 0, b = 0;
+    long long a = 0, b = 0, sm = 0;
     for (int i = 0; i < g.ntleaves; i++) {
-      a += h[2 * i];
-      b += h[2 * i + 1];
+      a += h[3 * i];
+      b += h[3 * i + 1];
+      sm += h[3 * i + 2];
     }
     *m2s = a;
     *pairs = b;
+    *smooth = sm;
     return 0;
   }
   DevBuf<int32_t> c;
@@ -1503,6 +1556,17 @@ int fmm_count_interactions(long long *m2s, long long *pairs) {
   return 0;
 }
 
+int fmm_count_interactions(long long *m2s, long long *pairs) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  long long sm = 0;
+  return count_work(m2s, pairs, &sm);
+}
+
+int fmm_count_work(long long *m2s, long long *pairs, long long *smooth) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return count_work(m2s, pairs, smooth);
+}
+
 int fmm_synchronize(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g.init) return 0;
@@ -1525,6 +1589,9 @@ void fmm_release(void) {
   g.doff.release(); g.ntl_dev.release(); g.counter.release();
   g.ck.release(); g.leaf_ck.release(); g.part.release(); g.split_ctr.release();
   g.nfirst.release();
+  g.scoef.release();
+  g.sflag.release();
+  g.srange.release();
   if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);
   if (g.graph) (void)hipGraphDestroy(g.graph);
   g.graph_exec = nullptr;

@@ -88,6 +88,33 @@ int launch_s2m(const NodeRec *d_nodes, int nnodes, const double *sx, const doubl
 int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int nparents, int nterms,
                      double2 *d_coef, hipStream_t s);
 
+// Smooth cells (mixed form, one core radius sigma): a cell small against the
+// core (half side <= eta sigma) is summed for every target through a Taylor
+// expansion of the regularised kernel f(z) = (1 - e^{-|z|^2}) / z (z in core
+// radii) about its centre, of total degree kSmoothQ in (d, conj d).  Per
+// cell kSmoothC complex FP32 coefficients: nu_q .. nu_0 (the n = 0 column),
+// then c_ij of the n >= 1 part in Horner order (j = q-1 .. 0, i = q-1-j .. 0);
+// see smooth_coef_kernel.  Coordinates of the evaluation are the FP32
+// leaf-relative ones scaled by sw = sqrt(log2 e) / sigma.
+constexpr int kSmoothQ = 8;
+constexpr int kSmoothC = (kSmoothQ + 1) + kSmoothQ * (kSmoothQ + 1) / 2;
+constexpr int kSmoothF4 = (kSmoothC + 1) / 2;  // float4 per cell
+// Coefficients of every node of region in (region/2, region] with at least
+// smin blobs (those of node k are sx..sg[nfirst[k] ...]), at
+// d_out[node * kSmoothF4]; other nodes are left as they are.
+int launch_smooth_coef(const NodeRec *d_nodes, int nnodes, const int32_t *d_nfirst,
+                       const double *sx, const double *sy, const double *sg, double sigma,
+                       double region, int smin, float4 *d_out, hipStream_t s);
+// Smooth runs: for every internal node above the smooth level (region >
+// `region`) that holds a smooth cell (smooth level, >= smin blobs): the index
+// range [x, y) of its smooth-level descendants (consecutive: each level is
+// stored in Morton order) if all its blobs lie in them, else (-2, -2);
+// (-1, -1) for every other node.  d_flag: 2 nnodes bytes of scratch.  The
+// first nroot_slots nodes are roots.
+int launch_smooth_ranges(const NodeRec *d_nodes, const int32_t *d_gparent, int nnodes,
+                         int nroot_slots, double region, int smin, uint8_t *d_flag, int2 *d_range,
+                         hipStream_t s);
+
 // ---- evaluation (eval_kernels.hip) ---------------------------------------
 struct TreeView {
   const NodeRec *nodes;
@@ -166,6 +193,16 @@ struct FmmArgs {
   uint32_t *ck_count;
   int2 *leaf_ck;
   double2 *part;
+  // smooth cells (smooth_region 0: off): a near node of region in
+  // (smooth_region/2, smooth_region] with at least smooth_min blobs is summed
+  // through its coefficients scoef[node * kSmoothF4 ...]
+  double smooth_region;
+  int smooth_min;
+  const float4 *scoef;
+  // per node: smooth-level descendants [x, y) of a node above that level
+  // whose blobs all lie in them; x = -2: holds smooth cells and other blobs;
+  // x = -1: no smooth cell below (see launch_smooth_ranges)
+  const int2 *srange;
 };
 int launch_mfma_selftest(const double *a, const double *b, double *out, double *ref, int fp32,
                          hipStream_t s);

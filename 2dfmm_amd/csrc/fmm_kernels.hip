@@ -368,6 +368,9 @@ __device__ inline uint64_t rfl64(uint64_t v) {
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
   return ((uint64_t)hi << 32) | lo;
 }
+__device__ inline double rfl64d(double v) {
+  return __builtin_bit_cast(double, rfl64(__builtin_bit_cast(uint64_t, v)));
+}
 
 // LDS stage of one wave: up to 64 sources of a near step, replica-major
 // (replica r reads its sources contiguously from r * stride), so rows of
@@ -675,13 +678,237 @@ __device__ inline bool whole_near(double inv_theta, double RA, double rcut, cons
   return m >= 0.0 && dx * dx + dy * dy <= m * m;
 }
 
-// Near items are leaves (one step) and whole near subtrees (one step per
-// tile of 64 sources).
-__device__ inline int item_tiles(const NodeRec &C, bool range) {
-  return range ? (node_count(C.cnt) + 63) >> 6 : 1;
+// Near items are leaves (one step), whole near subtrees (one step per tile
+// of 64 sources), smooth cells (one step; summed in batches, see
+// smooth_batch) and smooth runs: a whole near subtree whose blobs all lie in
+// smooth-level nodes, one step per tile of 64 consecutive smooth-level nodes
+// (smooth cells and small leaves).
+enum ItemKind { kItemLeaf = 0, kItemRange = 1, kItemSmooth = 2, kItemSmoothRun = 3 };
+__device__ inline int item_tiles(const FmmArgs &a, int c, const NodeRec &C, int kind) {
+  if (kind == kItemSmoothRun) {
+    const int2 r = ldro(a.srange + c);
+    return (rfl(r.y) - rfl(r.x) + 63) >> 6;
+  }
+  return kind == kItemRange ? (node_count(C.cnt) + 63) >> 6 : 1;
 }
 __device__ inline int tile_count(const NodeRec &C, bool range, int t) {
   return range ? min(64, node_count(C.cnt) - 64 * t) : node_count(C.cnt);
+}
+
+// ---- smooth cells ----------------------------------------------------------
+// A near node of the smooth level (region in (smooth_region/2, smooth_region])
+// holding at least smooth_min blobs is summed for all targets through the
+// Taylor coefficients of smooth_coef_kernel (expansion_kernels.hip).  Only
+// the mixed form with one core radius builds them (smooth_region 0 otherwise).
+__device__ inline bool smooth_cell(const FmmArgs &a, const NodeRec &C) {
+  return a.smooth_region > 0.0 && C.region <= a.smooth_region &&
+         2.0 * C.region > a.smooth_region && node_count(C.cnt) >= a.smooth_min;
+}
+
+// Coefficients of phi^(q)(w) = (-1)^q sum_k (-w)^k / (k! (k + q + 1)) below
+// kSmoothW0 (14 terms: the next is below 2e-10 of the sum there).
+constexpr float kSmoothW0 = 1.5f;
+constexpr int kSmoothSeries = 14;
+struct SmoothSeries {
+  float v[kSmoothSeries];
+};
+constexpr SmoothSeries make_smooth_series() {
+  SmoothSeries t{};
+  double f = 1.0;
+  for (int k = 0; k < kSmoothSeries; k++) {
+    if (k > 0) f *= (double)k;
+    const double c = 1.0 / (f * (double)(k + kSmoothQ + 1));
+    t.v[k] = (float)((k & 1) ? -c : c);
+  }
+  return t;
+}
+constexpr SmoothSeries kSmoothSer = make_smooth_series();
+
+// One smooth cell at one target, FP32: Z' = (zx, zy) is the target relative
+// to the cell centre in the scaled coordinates (sigma / sqrt(log2 e) units),
+// row[] the cell's coefficients.  With w = ln2 |Z'|^2 and E = e^-w = 2^-|Z'|^2,
+//   F = zb sum_m phi^(m)(w) nu_m zb^m + E sum_j zb^j sum_i c_ij z^i    (Horner),
+// phi^(m)(w) = (-1)^m int_0^1 t^m e^{-wt} dt from w phi^(m) + m phi^(m-1) =
+// (-1)^(m+1) E: upward from phi = (1 - E)/w for w >= kSmoothW0, downward from
+// the series of phi^(q) below it (each direction only where it is stable).
+// Adds (Im F, Re F) to (gx, gy).
+__device__ inline void smooth_eval(float zx, float zy, const float4 *row, float &gx, float &gy) {
+  constexpr int Q = kSmoothQ;
+  auto co = [&](int k) -> float2 {
+    const float4 v = row[k >> 1];
+    return (k & 1) ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
+  };
+  const float wp = fmaf(zx, zx, zy * zy);
+  const float E = __builtin_amdgcn_exp2f(-wp);
+  const float w = wp * 0.693147180559945309f;
+  float ph[Q + 1];
+  if (w >= kSmoothW0) {
+    const float rw = __builtin_amdgcn_rcpf(w);
+    ph[0] = (1.0f - E) * rw;
+#pragma unroll
+    for (int m = 1; m <= Q; m++) ph[m] = fmaf(-(float)m, ph[m - 1], (m & 1) ? E : -E) * rw;
+  } else {
+    float sr = kSmoothSer.v[kSmoothSeries - 1];
+#pragma unroll
+    for (int k = kSmoothSeries - 2; k >= 0; k--) sr = fmaf(sr, w, kSmoothSer.v[k]);
+    ph[Q] = (Q & 1) ? -sr : sr;
+#pragma unroll
+    for (int m = Q; m >= 1; m--) ph[m - 1] = fmaf(-w, ph[m], (m & 1) ? E : -E) * (1.0f / (float)m);
+  }
+  // n = 0 column: acc = sum_m nu_m phi^(m) zb^m, S0 = acc zb
+  float2 c = co(0);
+  float ar = c.x * ph[Q], ai = c.y * ph[Q];
+#pragma unroll
+  for (int m = Q - 1; m >= 0; m--) {
+    c = co(Q - m);
+    float tr = fmaf(ar, zx, fmaf(ai, zy, c.x * ph[m]));
+    float ti = fmaf(ai, zx, fmaf(-ar, zy, c.y * ph[m]));
+    if ((Q - m) & 1) asm volatile("" : "+v"(tr), "+v"(ti) : : "memory");
+    ar = tr;
+    ai = ti;
+  }
+  float s0r = fmaf(ar, zx, ai * zy), s0i = fmaf(ai, zx, -ar * zy);
+  asm volatile("" : "+v"(s0r), "+v"(s0i) : : "memory");
+  // n >= 1: P = sum_j zb^j Q_j(z), Q_j(z) = sum_i c_ij z^i
+  float pr = 0.f, pi = 0.f;
+  int k = Q + 1;
+#pragma unroll
+  for (int j = Q - 1; j >= 0; j--) {
+    c = co(k++);
+    float qr = c.x, qi = c.y;
+#pragma unroll
+    for (int i = Q - 2 - j; i >= 0; i--) {
+      c = co(k++);
+      const float tr = fmaf(qr, zx, fmaf(-qi, zy, c.x));
+      const float ti = fmaf(qr, zy, fmaf(qi, zx, c.y));
+      qr = tr;
+      qi = ti;
+    }
+    const float tr = fmaf(pr, zx, fmaf(pi, zy, qr));
+    const float ti = fmaf(pi, zx, fmaf(-pr, zy, qi));
+    pr = tr;
+    pi = ti;
+    // the next row's coefficient reads stay after this row's arithmetic (all
+    // rows' reads at once would hold ~90 registers): an empty asm statement
+    // that takes the running sum and orders memory
+    asm volatile("" : "+v"(pr), "+v"(pi) : : "memory");
+  }
+  gx += fmaf(E, pi, s0i);
+  gy += fmaf(E, pr, s0r);
+}
+
+// Smooth cells are summed kSmoothBatch at a time: replica r of every target
+// takes cell r of the batch.  Work units of one cell: kSmoothUnits (about the
+// cost of one evaluation in pair steps) shared by the cells of a batch.
+constexpr int kSmoothBatch = 5;
+constexpr int kSmoothRow = kSmoothF4 + 1;  // coefficients + the cell's offset
+constexpr int kSmoothUnits = 32;
+static_assert(kSmoothBatch * kSmoothRow <= kStageSlots, "smooth batch exceeds the stage");
+__device__ inline int smooth_batch_size(const LeafCtx &L) { return min(L.nrep, kSmoothBatch); }
+__device__ inline int64_t smooth_units(const LeafCtx &L) {
+  const int b = smooth_batch_size(L);
+  return (kSmoothUnits + b - 1) / b;
+}
+
+// Node k (lane k) of tile t of smooth run c: its record (cnt 0 past the end).
+__device__ inline NodeRec srun_node(const FmmArgs &a, int c, int t, int lane, int &idx) {
+  const int2 r = ldro(a.srange + c);
+  idx = rfl(r.x) + 64 * t + lane;
+  NodeRec C = {};
+  if (idx < rfl(r.y)) C = a.nodes[idx];
+  return C;
+}
+
+__device__ inline int64_t item_units(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
+                                     int kind, int t, int lane) {
+  if (kind == kItemSmooth) return smooth_units(L);
+  if (kind == kItemSmoothRun) {
+    // a smooth cell, or a small leaf summed directly, per lane; wave sum
+    int idx;
+    const NodeRec N = srun_node(a, c, t, lane, idx);
+    const int cnt = node_count(N.cnt);
+    int u = cnt >= a.smooth_min ? (int)smooth_units(L) : (cnt > 0 ? (int)near_units(cnt, L.nrep) : 0);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) u += __shfl_xor(u, o, 64);
+    return (int64_t)rfl(u);
+  }
+  return near_units(tile_count(C, kind == kItemRange, t), L.nrep);
+}
+
+// The nb cells held by lanes 0..nb-1 in `cell`: their coefficients and offsets
+// from the target leaf (FP64 difference, then FP32 scaled) are staged in the
+// wave's LDS slice st, kSmoothRow float4 per cell; then every (target,
+// replica r < nb) lane sums cell r.
+__device__ inline void smooth_batch(const FmmArgs &a, const LeafCtx &L, int cell, int nb, int lane,
+                                    float4 *st, double &ax, double &ay) {
+  const int k0 = lane / kSmoothRow, k1 = (lane + 64) / kSmoothRow;
+  const int c0 = __shfl(cell, k0, 64), c1 = __shfl(cell, k1, 64);
+  auto stage = [&](int e, int k, int c) {
+    if (e < nb * kSmoothRow) {
+      const int j = e - k * kSmoothRow;
+      if (j < kSmoothF4) {
+        st[e] = a.scoef[(size_t)c * kSmoothF4 + j];
+      } else {
+        const NodeRec C = a.nodes[c];
+        st[e] = make_float4((float)((C.cx - L.NA.cx) * a.sw), (float)((C.cy - L.NA.cy) * a.sw),
+                            0.f, 0.f);
+      }
+    }
+  };
+  stage(lane, k0, c0);
+  stage(lane + 64, k1, c1);
+  wave_sync();
+  float gx = 0.f, gy = 0.f;
+  if (L.tact && L.rep < nb) {
+    const float4 *row = st + L.rep * kSmoothRow;
+    const float4 o = row[kSmoothF4];
+    smooth_eval(L.lx - o.x, L.ly - o.y, row, gx, gy);
+  }
+  wave_sync();
+  ax += (double)gx;
+  ay += (double)gy;
+}
+
+// Steps ta..tb-1 of near item c of any kind; smooth cells go to push(cell)
+// (the caller's batch), the small leaves of a smooth run are summed here.
+template <bool MIXED, bool UNI, typename Push>
+__device__ inline void item_steps(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
+                                  int kind, int ta, int tb, int lane,
+                                  typename StageOf<MIXED>::T *st, double &ax, double &ay,
+                                  Push &&push) {
+  if constexpr (MIXED && UNI) {
+    if (kind == kItemSmooth) {
+      push(c);
+      return;
+    }
+    if (kind == kItemSmoothRun) {
+      for (int t = ta; t < tb; t++) {
+        int idx;
+        const NodeRec N = srun_node(a, c, t, lane, idx);
+        const int cnt = node_count(N.cnt);
+        uint64_t m = __builtin_amdgcn_ballot_w64(cnt > 0);
+        const uint64_t sm = __builtin_amdgcn_ballot_w64(cnt >= a.smooth_min);
+        while (m) {
+          const int k = __builtin_ctzll(m);
+          m &= m - 1;
+          const int node = rfl(__shfl(idx, k, 64));
+          if ((sm >> k) & 1) {
+            push(node);
+          } else {
+            NodeRec S;
+            S.cx = rfl64d(__shfl(N.cx, k, 64));
+            S.cy = rfl64d(__shfl(N.cy, k, 64));
+            S.region = 0.0;
+            S.first = rfl(__shfl(N.first, k, 64));
+            S.cnt = rfl(__shfl(N.cnt, k, 64));
+            near_leaf<MIXED, UNI>(a, L, S, lane, st, ax, ay);
+          }
+        }
+      }
+      return;
+    }
+  }
+  near_steps<MIXED, UNI>(a, L, c, C, kind == kItemRange, ta, tb, lane, st, ax, ay);
 }
 
 // Steps ta..tb-1 of near item c.
@@ -718,15 +945,16 @@ struct WalkState {
 };
 
 // Wave-uniform walk of the tree for target leaf L: far(c) for every far
-// cell, near(c, C, range) for every near item -- a leaf, or with range set a
-// whole near subtree (false stops the walk).  Climbing uses the group parent
+// cell, near(c, C, kind) for every near item -- a leaf, a whole near subtree
+// or (SMOOTH, with smooth cells built) a smooth cell (false stops the walk).  Climbing uses the group parent
 // array, so the state is a few words and a walk can be saved and resumed.
 // Node indices are uniform across the wave (readfirstlane), so node records
 // come through scalar loads.  An expansion tests its four children on lanes
 // 0-3 at once (far, open, whole near), combined with ballots.
-template <typename FarFn, typename NearFn>
+template <bool SMOOTH, typename FarFn, typename NearFn>
 __device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, int lane,
                             FarFn &&far, NearFn &&near) {
+  const bool smooth_on = SMOOTH && a.smooth_region > 0.0;
   for (;;) {
     if (ws.expand) {
       const int base = rfl(ldro(a.nodes + ws.node).first);
@@ -736,7 +964,12 @@ __device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, i
         if (node_count(C.cnt) > 0) {
           is_far = far_cell(L.inv_theta, L.RA, L.rcut, L.NA, C);
           is_open = !is_far;
+          // whole near subtrees: summed as one run of sources or, if all
+          // their blobs lie in smooth-level nodes, as one run of those;
+          // with smooth cells among other blobs they are opened
           is_run = is_open && node_internal(C.cnt) &&
+                   (!smooth_on || C.region <= a.smooth_region ||
+                    a.srange[base + lane].x != -2) &&
                    whole_near(L.inv_theta, L.RA, L.rcut, L.NA, C);
         }
       }
@@ -766,8 +999,13 @@ __device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, i
     ws.node = rfl((ws.cur & ~3) + q);
     const NodeRec C = ldro(a.nodes + ws.node);
     const bool range = (ws.rl.get(ws.d) >> q) & 1u;
-    ws.expand = node_internal(C.cnt) && !range;
-    if (!ws.expand && !near(ws.node, C, range)) return;
+    const bool sm = smooth_on && smooth_cell(a, C);
+    ws.expand = node_internal(C.cnt) && !range && !sm;
+    const int kind = sm      ? kItemSmooth
+                     : !range ? kItemLeaf
+                     : (smooth_on && rfl(ldro(a.srange + ws.node).x) >= 0) ? kItemSmoothRun
+                                                                           : kItemRange;
+    if (!ws.expand && !near(ws.node, C, kind)) return;
   }
 }
 
@@ -808,27 +1046,50 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const Split
   ws.expand = false;
   int64_t cum = (int64_t)rfl64((uint64_t)r.cum);
   const int64_t stop = (int64_t)rfl64((uint64_t)end);
+  constexpr bool SM = MIXED && UNI;
+  // pending smooth cells (lane k holds cell k of the batch)
+  int scell = 0, snb = 0;
+  const int sbsz = smooth_batch_size(L);
   // steps of an item from tile t0 on that start below `stop`
-  auto item = [&](int c, const NodeRec &C, bool range, int t0) -> bool {
-    const int nt = item_tiles(C, range);
+  auto push = [&](int cell) {
+    if constexpr (SM) {
+      if (lane == snb) scell = cell;
+      if (++snb == sbsz) {
+        smooth_batch(a, L, scell, snb, lane, st, ax, ay);
+        snb = 0;
+      }
+    }
+  };
+  auto item = [&](int c, const NodeRec &C, int kind, int t0) -> bool {
+    const int nt = item_tiles(a, c, C, kind);
     int te = t0;
     int64_t cu = cum;
-    while (te < nt && cu < stop) cu += near_units(tile_count(C, range, te++), L.nrep);
-    if (te > t0) near_steps<MIXED, UNI>(a, L, c, C, range, t0, te, lane, st, ax, ay);
+    while (te < nt && cu < stop) cu += item_units(a, L, c, C, kind, te++, lane);
+    if (te > t0) item_steps<MIXED, UNI>(a, L, c, C, kind, t0, te, lane, st, ax, ay, push);
     cum = cu;
     return te == nt;
   };
-  // the checkpoint's item is a leaf or a whole near subtree (internal)
+  // the checkpoint's item is a leaf, a whole near subtree (internal: a run
+  // of sources, or with smooth cells a run of smooth-level nodes) or a
+  // smooth cell
   const NodeRec C0 = ldro(a.nodes + ws.node);
-  if (!item(ws.node, C0, node_internal(C0.cnt), rfl(r.off))) return;
-  walk(a, L, ws, lane, [](int) {}, [&](int c, const NodeRec &C, bool range) -> bool {
-    return item(c, C, range, 0);
-  });
+  const bool smooth_on = SM && a.smooth_region > 0.0;
+  const int kind0 = (smooth_on && smooth_cell(a, C0)) ? kItemSmooth
+                    : !node_internal(C0.cnt) ? kItemLeaf
+                    : (smooth_on && rfl(ldro(a.srange + ws.node).x) >= 0) ? kItemSmoothRun
+                                                                          : kItemRange;
+  if (item(ws.node, C0, kind0, rfl(r.off)))
+    walk<SM>(a, L, ws, lane, [](int) {}, [&](int c, const NodeRec &C, int kind) -> bool {
+      return item(c, C, kind, 0);
+    });
+  if constexpr (SM) {
+    if (snb > 0) smooth_batch(a, L, scell, snb, lane, st, ax, ay);
+  }
 }
 
 // The fused pass for target leaf li (index in the leaf list), one wave.
 template <bool MIXED, bool UNI>
-__device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
+__device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
                                  double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
                                  typename StageOf<MIXED>::T (*s_stage)[kStageSlots],
                                  SplitCk (*s_ck)[kMaxCk]) {
@@ -894,14 +1155,28 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
   int64_t cum = 0, step = split ? a.split_budget : 0;
   int64_t thr = split ? step : INT64_MAX;
   int nck = 0;
-  auto near = [&](int c, const NodeRec &C, bool range) -> bool {
-    const int nt = item_tiles(C, range);
+  constexpr bool SM = MIXED && UNI;
+  // pending smooth cells (lane k holds cell k of the batch); only the summing
+  // part of the walk adds cells, so the last batch is summed after the walk
+  int scell = 0, snb = 0;
+  const int sbsz = smooth_batch_size(L);
+  auto push = [&](int cell) {
+    if constexpr (SM) {
+      if (lane == snb) scell = cell;
+      if (++snb == sbsz) {
+        smooth_batch(a, L, scell, snb, lane, s_stage[w], ax, ay);
+        snb = 0;
+      }
+    }
+  };
+  auto near = [&](int c, const NodeRec &C, int kind) -> bool {
+    const int nt = item_tiles(a, c, C, kind);
     int t = 0;
     if (nck == 0) {
       // the steps that start below the first threshold are summed here
       int64_t cu = cum;
-      while (t < nt && cu < thr) cu += near_units(tile_count(C, range, t++), L.nrep);
-      if (t > 0) near_steps<MIXED, UNI>(a, L, c, C, range, 0, t, lane, s_stage[w], ax, ay);
+      while (t < nt && cu < thr) cu += item_units(a, L, c, C, kind, t++, lane);
+      if (t > 0) item_steps<MIXED, UNI>(a, L, c, C, kind, 0, t, lane, s_stage[w], ax, ay, push);
       cum = cu;
     }
     for (; t < nt; t++) {
@@ -961,11 +1236,14 @@ __device__ void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
           thr = cum + step;
         }
       }
-      cum += near_units(tile_count(C, range, t), L.nrep);
+      cum += item_units(a, L, c, C, kind, t, lane);
     }
     return true;
   };
-  walk(a, L, ws, lane, far, near);
+  walk<SM>(a, L, ws, lane, far, near);
+  if constexpr (SM) {
+    if (snb > 0) smooth_batch(a, L, scell, snb, lane, s_stage[w], ax, ay);
+  }
   if (npend > 0) far_quad();
   acc.fold();
 
@@ -1180,8 +1458,9 @@ int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *
 }
 
 // Work of the fused kernel, one thread per target leaf with the same walk:
-// out[2*leaf] = far cells (each one M2L into the leaf), out[2*leaf+1] = near
-// source-target pairs.
+// out[3*leaf] = far cells (each one M2L into the leaf), out[3*leaf+1] = near
+// source-target pairs summed directly, out[3*leaf+2] = (target, smooth cell)
+// evaluations.
 __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__restrict__ out) {
   const int li = blockIdx.x * 256 + threadIdx.x;
   if (li >= a.nleaves) return;
@@ -1190,7 +1469,7 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
   const double RA = NA.region * 0.70710678118654752;
   const double rcut = a.leaf_par ? (double)a.leaf_par[li].y * a.smax : a.rcut;
   const double inv_theta = 1.0 / a.theta;
-  int64_t far = 0, pairs = 0;
+  int64_t far = 0, pairs = 0, smooth = 0;
   Slots2 sl;
   int d = 0, cur = 0, node = root_of(a, NA.first);
   bool expand = true;
@@ -1221,16 +1500,20 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
     const int q = 31 - __clz(m);
     sl.set(d, m & ~(1u << q));
     node = (cur & ~3) + q;
-    const int32_t cn = a.nodes[node].cnt;
-    if (node_internal(cn)) {
+    const NodeRec Cn = a.nodes[node];
+    if (smooth_cell(a, Cn)) {
+      smooth += cA;
+      expand = false;
+    } else if (node_internal(Cn.cnt)) {
       expand = true;
     } else {
-      pairs += (int64_t)cA * node_count(cn);
+      pairs += (int64_t)cA * node_count(Cn.cnt);
       expand = false;
     }
   }
-  out[2 * li] = far;
-  out[2 * li + 1] = pairs;
+  out[3 * li] = far;
+  out[3 * li + 1] = pairs;
+  out[3 * li + 2] = smooth;
 }
 
 int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s) {

@@ -32,6 +32,8 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# complex coefficients of a smooth cell's degree-8 expansion (9 + 36)
+SMOOTH_COEFS = 45
 sys.path.insert(0, ROOT)
 
 METRIC = "target velocities/s at N=4M, rel.err≤1e-6 vs FP64 direct: mixed+learned vs plain"
@@ -188,6 +190,10 @@ def main():
     ap.add_argument("--persistent", type=int, default=0,
                     help="accurate modes: persistent fused pass with this many workgroups per CU "
                          "(0: one wave per target leaf over a sized grid)")
+    ap.add_argument("--smooth", type=float, default=0.3,
+                    help="mixed modes: smooth cells of half side <= SMOOTH core radii (0: off)")
+    ap.add_argument("--smooth-min", type=int, default=24,
+                    help="mixed modes: fewest blobs of a smooth cell")
     ap.add_argument("--traffic",
                     default=os.path.join(ROOT, "profiles", "fmm_eval_gaussian_mixture_4M_traffic.txt"),
                     help="measured HBM bytes per evaluation (rocprofv3 FETCH_SIZE / WRITE_SIZE, "
@@ -230,6 +236,7 @@ def main():
     if args.persistent > 0:
         fmm.set_persistent(args.persistent * fmm.device_cus(), False)
     fmm.set_split(args.split)
+    fmm.set_smooth(args.smooth, args.smooth_min)
 
     # FP64 direct sum on sampled targets (the judge)
     idx = wl.sample_targets(n, args.judge_targets, seed=7)
@@ -285,19 +292,23 @@ def main():
         got = d_uv.cpu().numpy()[idx]
         l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
         mx = float(np.abs(got - ref).max() / np.abs(ref).max())
-        far, pairs = fmm.count_interactions()
         p = args.p
         if mode == "treecode":
+            far, pairs = fmm.count_interactions()
+            smooth = 0
             flops = far * (8 * (p - 1) + 10) + pairs * 20
         else:
-            flops = far * 8 * p * p + pairs * 20 + n * 8 * p
+            far, pairs, smooth = fmm.count_work()
+            # smooth cell at one target: 8 flops per complex coefficient
+            flops = far * 8 * p * p + pairs * 20 + n * 8 * p + smooth * 8 * SMOOTH_COEFS
         ach = flops / (ph["evaluate"] * 1e-3) / 1e12
         results[mode] = {"ms_per_step": ms, "value": n / (ms * 1e-3), "rel_L2": l2, "rel_max": mx,
                          "phases_ms": ph, "tree": fmm.tree_size(), "far": far, "pairs": pairs,
+                         "smooth": smooth,
                          "flops": flops, "achieved_tflops": ach, "precision": prec,
                          "kernel": kernel}
         log(f"[bench] {mode}: {ms:.2f} ms/step ({n / (ms * 1e-3):.4g} vel/s), rel_L2={l2:.3e} "
-            f"rel_max={mx:.3e}, far={far} pairs={pairs}, eval {ph['evaluate']:.2f} ms "
+            f"rel_max={mx:.3e}, far={far} pairs={pairs} smooth={smooth}, eval {ph['evaluate']:.2f} ms "
             f"= {ach:.2f} TFLOP/s; phases {ph}")
 
     # headline: mixed + learned when it meets the 1e-6 bar, else mixed alone

@@ -117,6 +117,14 @@ int fmm_set_persistent(int blocks, int use_graph);
  * deterministic.  0 turns splitting off.  Default 8192. */
 int fmm_set_split(int budget);
 int fmm_get_split(void);
+/* Smooth cells (no reference counterpart; mixed mode with one core radius
+ * sigma).  A near node whose half side is at most eta sigma and that holds at
+ * least min_blobs blobs is summed for every target through a degree-8 Taylor
+ * expansion of the regularised kernel about its centre (coefficients computed
+ * at make_tree) instead of pair by pair.  eta in [0, 0.35]; 0 turns it off.
+ * Takes effect at the next make_tree.  Default eta 0.3, min_blobs 24. */
+int fmm_set_smooth(double eta, int min_blobs);
+int fmm_get_smooth(double *eta, int *min_blobs);
 /* Compute units of the current device (for sizing persistent grids). */
 int fmm_device_cus(void);
 
@@ -214,6 +222,9 @@ int fmm_get_timing(double *ms, int n);
 /* Treecode interactions of the last tree with all blobs as targets: far cells
  * evaluated by M2S (cells holding blobs) and near source-target pairs. */
 int fmm_count_interactions(long long *m2s, long long *pairs);
+/* Same walk, with the smooth cells of the mixed form counted apart: far
+ * cells, pairs summed directly, and (target, smooth cell) evaluations. */
+int fmm_count_work(long long *m2s, long long *pairs, long long *smooth);
 
 /* Wait for all work of the library; release all device memory. */
 int fmm_synchronize(void);
