@@ -1331,6 +1331,19 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
 // slot is refilled at once (with several waves per workgroup the slot waits
 // for the workgroup's slowest leaf).  At least four waves per SIMD: the walk
 // is latency-bound and needs the others to hide it.
+//
+// Workgroups are dealt to the 8 XCDs in turn (workgroup b runs on XCD b mod
+// 8); XCD x is given blocks x, x + 8, x + 16, ... of kXcdBlock consecutive
+// leaves (neighbours in space), so the tree nodes and sources of one block's
+// walks meet in one L2 while the blocks still spread the clustered work
+// evenly over the XCDs.  Grid: a multiple of 8 kXcdBlock workgroups.
+constexpr int kXcds = 8;
+constexpr int kXcdBlock = 64;
+__device__ inline int xcd_leaf(int b, int nleaves) {
+  const int x = b % kXcds, k = b / kXcds;
+  return ((k / kXcdBlock) * kXcds + x) * kXcdBlock + k % kXcdBlock;
+}
+
 template <bool MIXED, bool UNI>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fmm_leaf_kernel(
     FmmArgs a) {
@@ -1338,7 +1351,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fm
   __shared__ double2 s_red[1][64];
   __shared__ typename StageOf<MIXED>::T s_stage[1][kStageSlots];
   __shared__ SplitCk s_ck[1][kMaxCk];
-  const int li = blockIdx.x;
+  const int li = xcd_leaf(blockIdx.x, a.nleaves);
   if (li >= a.nleaves) return;
   fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck);
 }
@@ -1531,12 +1544,13 @@ int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
     zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 2);
     FMM_HIP_TRY(hipGetLastError());
   }
+  const int grid = kXcds * kXcdBlock * ((args.nleaves + kXcds * kXcdBlock - 1) / (kXcds * kXcdBlock));
   if (mixed && args.sw > 0.f)
-    fmm_leaf_kernel<true, true><<<args.nleaves, 64, 0, s>>>(args);
+    fmm_leaf_kernel<true, true><<<grid, 64, 0, s>>>(args);
   else if (mixed)
-    fmm_leaf_kernel<true, false><<<args.nleaves, 64, 0, s>>>(args);
+    fmm_leaf_kernel<true, false><<<grid, 64, 0, s>>>(args);
   else
-    fmm_leaf_kernel<false, false><<<args.nleaves, 64, 0, s>>>(args);
+    fmm_leaf_kernel<false, false><<<grid, 64, 0, s>>>(args);
   FMM_HIP_TRY(hipGetLastError());
   if (split) return launch_split_passes(args, mixed, d_nleaves, d_split_ctr + 1, split_blocks, s);
   return 0;
