@@ -355,4 +355,27 @@ int launch_smooth_ranges(const NodeRec *d_nodes, const int32_t *d_gparent, int n
   return 0;
 }
 
+// Work units of the nodes of the smooth level (the others 0): a smooth cell
+// kSmoothUnits, a small leaf its blobs + kStepUnits; then their prefix sums.
+__global__ __launch_bounds__(256) void smooth_units_kernel(const NodeRec *__restrict__ nodes,
+                                                           int nnodes, double region, int smin,
+                                                           uint32_t *__restrict__ u) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nnodes) return;
+  const NodeRec C = nodes[i];
+  const int cnt = node_count(C.cnt);
+  uint32_t v = 0;
+  if (C.region <= region && 2.0 * C.region > region && cnt > 0)
+    v = cnt >= smin ? (uint32_t)kSmoothUnits : (uint32_t)(cnt + kStepUnits);
+  u[i] = v;
+}
+
+int launch_smooth_units(const NodeRec *d_nodes, int nnodes, double region, int smin,
+                        uint32_t *d_tmp, uint32_t *d_out, hipStream_t s) {
+  if (nnodes <= 0) return 0;
+  smooth_units_kernel<<<(nnodes + 255) / 256, 256, 0, s>>>(d_nodes, nnodes, region, smin, d_tmp);
+  FMM_HIP_TRY(hipGetLastError());
+  return scan_u32(d_tmp, d_out, nnodes, d_out + nnodes, s);
+}
+
 }  // namespace fmm2d

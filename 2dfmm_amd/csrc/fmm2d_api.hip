@@ -133,6 +133,7 @@ struct Context {
   DevBuf<float4> scoef;
   DevBuf<uint8_t> sflag;
   DevBuf<int2> srange;
+  DevBuf<uint32_t> sunits, sutmp;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -392,6 +393,9 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
           launch_smooth_ranges(g.nodes.p, g.gparent.p, g.nnodes, nroot_slots, region, smin,
                                g.sflag.p, g.srange.p, s))
         return -1;
+      if (g.sunits.ensure((size_t)g.nnodes + 1, s) || g.sutmp.ensure(g.nnodes, s) ||
+          launch_smooth_units(g.nodes.p, g.nnodes, region, smin, g.sutmp.p, g.sunits.p, s))
+        return -1;
       g.smooth_region_h = region;
       g.t_smooth_min = smin;
     }
@@ -466,6 +470,7 @@ static FmmArgs fmm_args(double *d_uv) {
   a.smooth_min = g.t_smooth_min;
   a.scoef = g.scoef.p;
   a.srange = g.srange.p;
+  a.sunits = g.sunits.p;
   return a;
 }
 
@@ -1592,6 +1597,8 @@ void fmm_release(void) {
   g.scoef.release();
   g.sflag.release();
   g.srange.release();
+  g.sunits.release();
+  g.sutmp.release();
   if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);
   if (g.graph) (void)hipGraphDestroy(g.graph);
   g.graph_exec = nullptr;

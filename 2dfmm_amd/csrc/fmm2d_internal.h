@@ -146,15 +146,12 @@ int launch_direct(int ns, const double *d_sxy, const double *d_cir, const double
                   hipStream_t s);
 
 // ---- accurate fast mode (fmm_kernels.hip) ------------------------------
-// Saved walk of a target leaf with much near work: the continuation pass
-// resumes it at near item `node` (a leaf, or a whole near subtree from its
-// tile `off`) and sums near steps while the work count is below `end`.
+// One chunk of the near work of a target leaf with much of it: the tiles of
+// its near items whose work before them (in walk order) lies in [cum, end);
+// the continuation pass walks the leaf again and sums them.
 struct SplitCk {
-  uint64_t lo, hi;   // open-child masks, 4 bits per level
-  uint64_t rlo, rhi; // whole-near-subtree masks, 4 bits per level
-  int64_t cum, end;  // near work units before this point; end of the chunk
-  int32_t li, d, cur, node;  // target leaf (-1: skip), level, group, item
-  int32_t off, pad;          // tile of the item
+  int64_t cum, end;
+  int32_t li, pad;  // target leaf (-1: skip)
 };
 
 struct FmmArgs {
@@ -203,7 +200,17 @@ struct FmmArgs {
   // whose blobs all lie in them; x = -2: holds smooth cells and other blobs;
   // x = -1: no smooth cell below (see launch_smooth_ranges)
   const int2 *srange;
+  // prefix sums over the nodes of the work units of smooth-level nodes (a
+  // smooth cell kSmoothUnits, a small leaf its blobs + kStepUnits; nnodes + 1)
+  const uint32_t *sunits;
 };
+// Work units of a smooth cell, and of a step's fixed cost (fmm_kernels.hip).
+constexpr int kSmoothUnits = 32;
+constexpr int kStepUnits = 8;
+// Per-node work units of the smooth level summed into d_out[0..nnodes] (exclusive
+// prefix, total last); d_tmp: nnodes u32 of scratch.
+int launch_smooth_units(const NodeRec *d_nodes, int nnodes, double region, int smin,
+                        uint32_t *d_tmp, uint32_t *d_out, hipStream_t s);
 int launch_mfma_selftest(const double *a, const double *b, double *out, double *ref, int fp32,
                          hipStream_t s);
 // sw > 0 (all core radii equal, sw = sqrt(log2(e) / sigma^2)): FP32 sources as

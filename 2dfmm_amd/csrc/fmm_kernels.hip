@@ -339,7 +339,7 @@ __device__ inline int root_of(const FmmArgs &a, int first) {
 // the checkpoints on other waves, and a last pass adds their partial sums in
 // chunk order (deterministic).
 // ---------------------------------------------------------------------------
-constexpr int kMaxCk = 32;  // checkpoints per leaf kept in LDS
+constexpr int kMaxCk = 32;  // chunk boundaries per leaf kept in LDS
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -433,7 +433,6 @@ __device__ inline LeafCtx leaf_ctx(const FmmArgs &a, int li, int lane) {
 
 // Work units of one near step over cnt sources: wave steps plus a fixed
 // cost for the step's loads and synchronisation.
-constexpr int kStepUnits = 8;
 __device__ inline int64_t near_units(int cnt, int nrep) {
   return (int64_t)((cnt + nrep - 1) / nrep) + kStepUnits;
 }
@@ -802,7 +801,6 @@ __device__ inline void smooth_eval(float zx, float zy, const float4 *row, float 
 // cost of one evaluation in pair steps) shared by the cells of a batch.
 constexpr int kSmoothBatch = 5;
 constexpr int kSmoothRow = kSmoothF4 + 1;  // coefficients + the cell's offset
-constexpr int kSmoothUnits = 32;
 static_assert(kSmoothBatch * kSmoothRow <= kStageSlots, "smooth batch exceeds the stage");
 __device__ inline int smooth_batch_size(const LeafCtx &L) { return min(L.nrep, kSmoothBatch); }
 __device__ inline int64_t smooth_units(const LeafCtx &L) {
@@ -817,22 +815,6 @@ __device__ inline NodeRec srun_node(const FmmArgs &a, int c, int t, int lane, in
   NodeRec C = {};
   if (idx < rfl(r.y)) C = a.nodes[idx];
   return C;
-}
-
-__device__ inline int64_t item_units(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
-                                     int kind, int t, int lane) {
-  if (kind == kItemSmooth) return smooth_units(L);
-  if (kind == kItemSmoothRun) {
-    // a smooth cell, or a small leaf summed directly, per lane; wave sum
-    int idx;
-    const NodeRec N = srun_node(a, c, t, lane, idx);
-    const int cnt = node_count(N.cnt);
-    int u = cnt >= a.smooth_min ? (int)smooth_units(L) : (cnt > 0 ? (int)near_units(cnt, L.nrep) : 0);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) u += __shfl_xor(u, o, 64);
-    return (int64_t)rfl(u);
-  }
-  return near_units(tile_count(C, kind == kItemRange, t), L.nrep);
 }
 
 // The nb cells held by lanes 0..nb-1 in `cell`: their coefficients and offsets
@@ -898,7 +880,7 @@ __device__ inline void item_steps(const FmmArgs &a, const LeafCtx &L, int c, con
             NodeRec S;
             S.cx = rfl64d(__shfl(N.cx, k, 64));
             S.cy = rfl64d(__shfl(N.cy, k, 64));
-            S.region = 0.0;
+            S.region = rfl64d(__shfl(N.region, k, 64));
             S.first = rfl(__shfl(N.first, k, 64));
             S.cnt = rfl(__shfl(N.cnt, k, 64));
             near_leaf<MIXED, UNI>(a, L, S, lane, st, ax, ay);
@@ -935,78 +917,117 @@ __device__ inline bool far_cell(double inv_theta, double RA, double rcut, const 
   return dx * dx + dy * dy >= lim * lim;
 }
 
-// State of the walk: a 4-bit mask of still-open children per level, the
-// current level, the first child of the current group and the current node.
-struct WalkState {
-  Slots2 sl;  // open children per level
-  Slots2 rl;  // of those, whole near subtrees (summed as one run)
-  int d, cur, node;
-  bool expand;
-};
-
-// Wave-uniform walk of the tree for target leaf L: far(c) for every far
-// cell, near(c, C, kind) for every near item -- a leaf, a whole near subtree
-// or (SMOOTH, with smooth cells built) a smooth cell (false stops the walk).  Climbing uses the group parent
-// array, so the state is a few words and a walk can be saved and resumed.
-// Node indices are uniform across the wave (readfirstlane), so node records
-// come through scalar loads.  An expansion tests its four children on lanes
-// 0-3 at once (far, open, whole near), combined with ballots.
-template <bool SMOOTH, typename FarFn, typename NearFn>
-__device__ inline void walk(const FmmArgs &a, const LeafCtx &L, WalkState &ws, int lane,
-                            FarFn &&far, NearFn &&near) {
+// The walk of one target leaf, level-parallel: `stack` (this wave's LDS,
+// kStack entries) holds the first-child index of every open node still to be
+// expanded.  A step takes up to 16 of them (the latest pushed) and classifies
+// their 64 children at once, lane = child: far cells go to far(c) and near
+// items to item(c, C, kind), both in lane order; open children are pushed.
+// item returning false ends the walk.  The order depends on the tree and the
+// target leaf only, so every pass over a leaf (fused, continuation, inline)
+// meets the same items in the same order.  Pops shrink to one node per step
+// near the top of the stack; a depth-first descent then adds at most 3
+// entries per level, which kStackMargin covers.
+constexpr int kStack = 512;
+constexpr int kStackMargin = 96;
+template <bool SMOOTH, typename FarFn, typename ItemFn>
+__device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, int *stack,
+                                 FarFn &&far, ItemFn &&item) {
   const bool smooth_on = SMOOTH && a.smooth_region > 0.0;
-  for (;;) {
-    if (ws.expand) {
-      const int base = rfl(ldro(a.nodes + ws.node).first);
-      bool is_far = false, is_open = false, is_run = false;
-      if (lane < 4) {
-        const NodeRec C = a.nodes[base + lane];
-        if (node_count(C.cnt) > 0) {
-          is_far = far_cell(L.inv_theta, L.RA, L.rcut, L.NA, C);
-          is_open = !is_far;
-          // whole near subtrees: summed as one run of sources or, if all
-          // their blobs lie in smooth-level nodes, as one run of those;
-          // with smooth cells among other blobs they are opened
-          is_run = is_open && node_internal(C.cnt) &&
-                   (!smooth_on || C.region <= a.smooth_region ||
-                    a.srange[base + lane].x != -2) &&
-                   whole_near(L.inv_theta, L.RA, L.rcut, L.NA, C);
-        }
+  const int root = rfl(root_of(a, L.NA.first));
+  int ns = 1;
+  if (lane == 0) stack[0] = ldro(a.nodes + root).first;
+  wave_sync();
+  while (ns > 0) {
+    const int k = min(min(16, ns), max(1, (kStack - kStackMargin - ns) / 3));
+    int c = -1;
+    if ((lane >> 2) < k) c = stack[ns - k + (lane >> 2)] + (lane & 3);
+    wave_sync();
+    ns -= k;
+    NodeRec C = {};
+    if (c >= 0) C = a.nodes[c];
+    bool f = false, it = false, op = false;
+    int kind = kItemLeaf;
+    if (node_count(C.cnt) > 0) {
+      if (far_cell(L.inv_theta, L.RA, L.rcut, L.NA, C)) {
+        f = true;
+      } else if (smooth_on && smooth_cell(a, C)) {
+        it = true;
+        kind = kItemSmooth;
+      } else if (!node_internal(C.cnt)) {
+        it = true;
+      } else if (whole_near(L.inv_theta, L.RA, L.rcut, L.NA, C) &&
+                 (!smooth_on || a.srange[c].x != -2)) {
+        // whole near subtree: one run of sources, or of smooth-level nodes
+        // if all its blobs lie in them; with smooth cells among other blobs
+        // it is opened
+        it = true;
+        kind = (smooth_on && a.srange[c].x >= 0) ? kItemSmoothRun : kItemRange;
+      } else {
+        op = true;
       }
-      uint32_t fm = (uint32_t)__builtin_amdgcn_ballot_w64(is_far);
-      const uint32_t om = (uint32_t)__builtin_amdgcn_ballot_w64(is_open);
-      const uint32_t rm = (uint32_t)__builtin_amdgcn_ballot_w64(is_run);
-      while (fm) {
-        const int q = __builtin_ctz(fm);
-        fm &= fm - 1;
-        far(base + q);
-      }
-      ws.d++;
-      ws.sl.set(ws.d, om);
-      ws.rl.set(ws.d, rm);
-      ws.cur = base;
     }
-    uint32_t m;
-    for (;;) {
-      m = ws.sl.get(ws.d);
-      if (m) break;
-      if (ws.d == 1) return;
-      ws.cur = rfl(ldro(a.gparent + (ws.cur >> 2)));
-      ws.d--;
+    const uint64_t fm = __builtin_amdgcn_ballot_w64(f);
+    const uint64_t im = __builtin_amdgcn_ballot_w64(it);
+    const uint64_t om = __builtin_amdgcn_ballot_w64(op);
+    if (op) {
+      const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(om >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)om, 0));
+      if (ns + below < kStack) stack[ns + below] = C.first;
     }
-    const int q = 31 - __clz(m);
-    ws.sl.set(ws.d, m & ~(1u << q));
-    ws.node = rfl((ws.cur & ~3) + q);
-    const NodeRec C = ldro(a.nodes + ws.node);
-    const bool range = (ws.rl.get(ws.d) >> q) & 1u;
-    const bool sm = smooth_on && smooth_cell(a, C);
-    ws.expand = node_internal(C.cnt) && !range && !sm;
-    const int kind = sm      ? kItemSmooth
-                     : !range ? kItemLeaf
-                     : (smooth_on && rfl(ldro(a.srange + ws.node).x) >= 0) ? kItemSmoothRun
-                                                                           : kItemRange;
-    if (!ws.expand && !near(ws.node, C, kind)) return;
+    ns += __builtin_popcountll(om);
+    wave_sync();
+    uint64_t m = fm;
+    while (m) {
+      const int b = __builtin_ctzll(m);
+      m &= m - 1;
+      far(rfl(__shfl(c, b, 64)));
+    }
+    m = im;
+    while (m) {
+      const int b = __builtin_ctzll(m);
+      m &= m - 1;
+      NodeRec S;
+      S.cx = rfl64d(__shfl(C.cx, b, 64));
+      S.cy = rfl64d(__shfl(C.cy, b, 64));
+      S.region = rfl64d(__shfl(C.region, b, 64));
+      S.first = rfl(__shfl(C.first, b, 64));
+      S.cnt = rfl(__shfl(C.cnt, b, 64));
+      if (!item(rfl(__shfl(c, b, 64)), S, rfl(__shfl(kind, b, 64)))) return;
+    }
   }
+}
+
+// Work units of tiles [0, t) of item c (nt tiles): near_units per tile; a
+// smooth run sums fixed per-node units (a smooth cell kSmoothUnits, a small
+// leaf its blobs + kStepUnits) kept as prefix sums over the nodes (sunits).
+__device__ inline int64_t item_prefix(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
+                                      int kind, int nt, int t) {
+  if (t <= 0) return 0;
+  if (kind == kItemSmooth) return smooth_units(L);
+  if (kind == kItemLeaf) return near_units(node_count(C.cnt), L.nrep);
+  if (kind == kItemRange) {
+    int64_t u = (int64_t)min(t, nt - 1) * near_units(64, L.nrep);
+    if (t >= nt) u += near_units(node_count(C.cnt) - 64 * (nt - 1), L.nrep);
+    return u;
+  }
+  const int2 r = ldro(a.srange + c);
+  const int lo = rfl(r.x), hi = rfl(r.y);
+  return (int64_t)rfl((int)(ldro(a.sunits + min(lo + 64 * t, hi)) - ldro(a.sunits + lo)));
+}
+
+// First tile t in [t0, nt) whose work before it, base + prefix(t), reaches v
+// (nt if none); the prefix does not decrease in t.
+__device__ inline int item_find(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
+                                int kind, int nt, int64_t base, int64_t v, int t0) {
+  int lo = t0, hi = nt;  // answer in [lo, hi]
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (base + item_prefix(a, L, c, C, kind, nt, mid) >= v)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
 }
 
 // Sum over the replicas of each target; the result is valid in the lanes
@@ -1028,29 +1049,18 @@ __device__ inline double2 reduce_replicas(const LeafCtx &L, int lane, double2 *r
   return make_double2(sx, sy);
 }
 
-// Near work of one chunk: resume the walk at checkpoint r (its item, from
-// tile r.off) and sum near steps until the work count reaches `end`; far
-// cells are skipped (the fused pass took them).
+// Near work of one chunk: the tiles whose work before them lies in
+// [start, end), met by walking the leaf again from the root (far cells are
+// the fused pass's); smooth cells in batches, the last batch summed at the
+// end.
 template <bool MIXED, bool UNI>
-__device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const SplitCk &r, int64_t end,
-                                 int lane, typename StageOf<MIXED>::T *st, double &ax,
+__device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t start, int64_t end,
+                                 int lane, typename StageOf<MIXED>::T *st, int *stack, double &ax,
                                  double &ay) {
-  WalkState ws;
-  ws.sl.lo = rfl64(r.lo);
-  ws.sl.hi = rfl64(r.hi);
-  ws.rl.lo = rfl64(r.rlo);
-  ws.rl.hi = rfl64(r.rhi);
-  ws.d = rfl(r.d);
-  ws.cur = rfl(r.cur);
-  ws.node = rfl(r.node);
-  ws.expand = false;
-  int64_t cum = (int64_t)rfl64((uint64_t)r.cum);
-  const int64_t stop = (int64_t)rfl64((uint64_t)end);
   constexpr bool SM = MIXED && UNI;
-  // pending smooth cells (lane k holds cell k of the batch)
+  int64_t cum = 0;
   int scell = 0, snb = 0;
   const int sbsz = smooth_batch_size(L);
-  // steps of an item from tile t0 on that start below `stop`
   auto push = [&](int cell) {
     if constexpr (SM) {
       if (lane == snb) scell = cell;
@@ -1060,28 +1070,17 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, const Split
       }
     }
   };
-  auto item = [&](int c, const NodeRec &C, int kind, int t0) -> bool {
+  walk_wide<SM>(a, L, lane, stack, [](int) {}, [&](int c, const NodeRec &C, int kind) -> bool {
+    const int64_t c0 = cum;
+    if (c0 >= end) return false;
     const int nt = item_tiles(a, c, C, kind);
-    int te = t0;
-    int64_t cu = cum;
-    while (te < nt && cu < stop) cu += item_units(a, L, c, C, kind, te++, lane);
-    if (te > t0) item_steps<MIXED, UNI>(a, L, c, C, kind, t0, te, lane, st, ax, ay, push);
-    cum = cu;
-    return te == nt;
-  };
-  // the checkpoint's item is a leaf, a whole near subtree (internal: a run
-  // of sources, or with smooth cells a run of smooth-level nodes) or a
-  // smooth cell
-  const NodeRec C0 = ldro(a.nodes + ws.node);
-  const bool smooth_on = SM && a.smooth_region > 0.0;
-  const int kind0 = (smooth_on && smooth_cell(a, C0)) ? kItemSmooth
-                    : !node_internal(C0.cnt) ? kItemLeaf
-                    : (smooth_on && rfl(ldro(a.srange + ws.node).x) >= 0) ? kItemSmoothRun
-                                                                          : kItemRange;
-  if (item(ws.node, C0, kind0, rfl(r.off)))
-    walk<SM>(a, L, ws, lane, [](int) {}, [&](int c, const NodeRec &C, int kind) -> bool {
-      return item(c, C, kind, 0);
-    });
+    cum = c0 + item_prefix(a, L, c, C, kind, nt, nt);
+    if (cum <= start) return true;
+    const int ta = item_find(a, L, c, C, kind, nt, c0, start, 0);
+    const int tb = item_find(a, L, c, C, kind, nt, c0, end, ta);
+    if (ta < tb) item_steps<MIXED, UNI>(a, L, c, C, kind, ta, tb, lane, st, ax, ay, push);
+    return tb == nt;
+  });
   if constexpr (SM) {
     if (snb > 0) smooth_batch(a, L, scell, snb, lane, st, ax, ay);
   }
@@ -1092,7 +1091,7 @@ template <bool MIXED, bool UNI>
 __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
                                  double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
                                  typename StageOf<MIXED>::T (*s_stage)[kStageSlots],
-                                 SplitCk (*s_ck)[kMaxCk]) {
+                                 int64_t (*s_ck)[kMaxCk], int (*s_stack)[kStack]) {
   li = rfl(li);
   const LeafCtx L = leaf_ctx<MIXED, UNI>(a, li, lane);
   const NodeRec &NA = L.NA;
@@ -1144,13 +1143,8 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   };
 
   // near work: summed here up to the first threshold, then only counted,
-  // with a checkpoint every `step` units (kept in LDS; when the slots run
-  // out, every second checkpoint is dropped and the step doubles)
-  WalkState ws;
-  ws.d = 0;
-  ws.cur = 0;
-  ws.node = rfl(root_of(a, NA.first));
-  ws.expand = true;
+  // with a chunk boundary every `step` units (kept in LDS; when the slots run
+  // out, every second boundary is dropped and the step doubles)
   const bool split = a.split_budget > 0;
   int64_t cum = 0, step = split ? a.split_budget : 0;
   int64_t thr = split ? step : INT64_MAX;
@@ -1169,78 +1163,41 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
       }
     }
   };
-  auto near = [&](int c, const NodeRec &C, int kind) -> bool {
+  auto item = [&](int c, const NodeRec &C, int kind) -> bool {
     const int nt = item_tiles(a, c, C, kind);
+    const int64_t c0 = cum;
+    const int64_t total = item_prefix(a, L, c, C, kind, nt, nt);
     int t = 0;
     if (nck == 0) {
-      // the steps that start below the first threshold are summed here
-      int64_t cu = cum;
-      while (t < nt && cu < thr) cu += item_units(a, L, c, C, kind, t++, lane);
+      // the tiles that start below the first threshold are summed here
+      t = c0 + total <= thr ? nt : item_find(a, L, c, C, kind, nt, c0, thr, 0);
       if (t > 0) item_steps<MIXED, UNI>(a, L, c, C, kind, 0, t, lane, s_stage[w], ax, ay, push);
-      cum = cu;
     }
-    for (; t < nt; t++) {
-      if (cum >= thr) {
-        if (nck == kMaxCk) {
-          uint64_t lo = 0, hi = 0, rlo = 0, rhi = 0;
-          int64_t cu = 0;
-          int rd = 0, rc = 0, rn = 0, ro = 0;
-          if (lane < kMaxCk / 2) {
-            const SplitCk &r = s_ck[w][2 * lane];
-            lo = r.lo;
-            hi = r.hi;
-            rlo = r.rlo;
-            rhi = r.rhi;
-            cu = r.cum;
-            rd = r.d;
-            rc = r.cur;
-            rn = r.node;
-            ro = r.off;
-          }
-          wave_sync();
-          if (lane < kMaxCk / 2) {
-            SplitCk &r = s_ck[w][lane];
-            r.lo = lo;
-            r.hi = hi;
-            r.rlo = rlo;
-            r.rhi = rhi;
-            r.cum = cu;
-            r.end = 0;
-            r.li = li;
-            r.d = rd;
-            r.cur = rc;
-            r.node = rn;
-            r.off = ro;
-          }
-          wave_sync();
-          nck = kMaxCk / 2;
-          step *= 2;
-          thr = (int64_t)rfl64((uint64_t)s_ck[w][nck - 1].cum) + step;
-        }
-        if (cum >= thr) {
-          if (lane == 0) {
-            SplitCk &r = s_ck[w][nck];
-            r.lo = ws.sl.lo;
-            r.hi = ws.sl.hi;
-            r.rlo = ws.rl.lo;
-            r.rhi = ws.rl.hi;
-            r.cum = cum;
-            r.end = 0;
-            r.li = li;
-            r.d = ws.d;
-            r.cur = ws.cur;
-            r.node = ws.node;
-            r.off = t;
-          }
-          nck++;
-          thr = cum + step;
-        }
+    // a chunk boundary at every tile whose work before it reaches thr
+    while (t < nt && c0 + total > thr) {
+      t = item_find(a, L, c, C, kind, nt, c0, thr, t);
+      if (t >= nt) break;
+      if (nck == kMaxCk) {
+        int64_t keep = 0;
+        if (lane < kMaxCk / 2) keep = s_ck[w][2 * lane];
+        wave_sync();
+        if (lane < kMaxCk / 2) s_ck[w][lane] = keep;
+        wave_sync();
+        nck = kMaxCk / 2;
+        step *= 2;
+        thr = (int64_t)rfl64((uint64_t)s_ck[w][nck - 1]) + step;
+        continue;
       }
-      cum += item_units(a, L, c, C, kind, t, lane);
+      const int64_t at = c0 + item_prefix(a, L, c, C, kind, nt, t);
+      if (lane == 0) s_ck[w][nck] = at;
+      nck++;
+      thr = at + step;
     }
+    cum = c0 + total;
     return true;
   };
-  walk<SM>(a, L, ws, lane, far, near);
+  int *stack = s_stack[w];
+  walk_wide<SM>(a, L, lane, stack, far, item);
   if constexpr (SM) {
     if (snb > 0) smooth_batch(a, L, scell, snb, lane, s_stage[w], ax, ay);
   }
@@ -1281,8 +1238,8 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   }
   const double2 S = reduce_replicas(L, lane, s_red[w], ax, ay);
 
-  // hand the checkpoints to the continuation pass; without room for them,
-  // run the chunks here with the arithmetic of the summing pass
+  // hand the chunks (work from s_ck[k] to s_ck[k + 1]) to the continuation
+  // pass; without room for them, run them here with the same arithmetic
   bool inline_chunks = false;
   double2 extra = make_double2(0.0, 0.0);
   if (split) {
@@ -1293,9 +1250,11 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
       base = (uint32_t)rfl((int)__shfl((int)base, 0, 64));
       const bool fits = (int64_t)base + nck <= (int64_t)a.ck_cap;
       if (lane < nck && (int64_t)base + lane < (int64_t)a.ck_cap) {
-        SplitCk r = s_ck[w][lane];
-        r.end = lane + 1 < nck ? s_ck[w][lane + 1].cum : INT64_MAX;
-        if (!fits) r.li = -1;  // the continuation pass skips these
+        SplitCk r;
+        r.cum = s_ck[w][lane];
+        r.end = lane + 1 < nck ? s_ck[w][lane + 1] : INT64_MAX;
+        r.li = fits ? li : -1;  // the continuation pass skips li < 0
+        r.pad = 0;
         a.ck[base + lane] = r;
       }
       if (fits) {
@@ -1303,10 +1262,10 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
       } else {
         inline_chunks = true;
         for (int k = 0; k < nck; k++) {
-          const SplitCk r = s_ck[w][k];
-          const int64_t end = k + 1 < nck ? s_ck[w][k + 1].cum : INT64_MAX;
+          const int64_t start = (int64_t)rfl64((uint64_t)s_ck[w][k]);
+          const int64_t end = k + 1 < nck ? (int64_t)rfl64((uint64_t)s_ck[w][k + 1]) : INT64_MAX;
           double cx = 0.0, cy = 0.0;
-          run_chunk<MIXED, UNI>(a, L, r, end, lane, s_stage[w], cx, cy);
+          run_chunk<MIXED, UNI>(a, L, start, end, lane, s_stage[w], stack, cx, cy);
           const double2 P = reduce_replicas(L, lane, s_red[w], cx, cy);
           extra.x += P.x;
           extra.y += P.y;
@@ -1350,10 +1309,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fm
   __shared__ double2 s_loc[1][kFmmMaxP];
   __shared__ double2 s_red[1][64];
   __shared__ typename StageOf<MIXED>::T s_stage[1][kStageSlots];
-  __shared__ SplitCk s_ck[1][kMaxCk];
+  __shared__ int64_t s_ck[1][kMaxCk];
+  __shared__ int s_stack[1][kStack];
   const int li = xcd_leaf(blockIdx.x, a.nleaves);
   if (li >= a.nleaves) return;
-  fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck);
+  fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck, s_stack);
 }
 
 // Persistent form: a fixed grid whose waves take target leaves from a work
@@ -1368,7 +1328,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
   __shared__ double2 s_loc[4][kFmmMaxP];
   __shared__ double2 s_red[4][64];
   __shared__ typename StageOf<MIXED>::T s_stage[4][kStageSlots];
-  __shared__ SplitCk s_ck[4][kMaxCk];
+  __shared__ int64_t s_ck[4][kMaxCk];
+  __shared__ int s_stack[4][kStack];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = *nleaves;
   for (;;) {
@@ -1376,17 +1337,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
     if (lane == 0) li = atomicAdd(counter, 1u);
     li = __shfl(li, 0, 64);
     if (li >= total) break;
-    fmm_process_leaf<MIXED, UNI>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck);
+    fmm_process_leaf<MIXED, UNI>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck, s_stack);
   }
 }
 
-// Continuation pass: waves take checkpoint records from a work counter; the
+// Continuation pass: waves take chunk records from a work counter; the
 // partial sums of record j go to part[64 j + target].  Exits when the counter
 // passes the number of records written by the fused pass.
 template <bool MIXED, bool UNI>
 __global__ __launch_bounds__(256) void fmm_chunk_kernel(FmmArgs a, uint32_t *__restrict__ counter) {
   __shared__ double2 s_red[4][64];
   __shared__ typename StageOf<MIXED>::T s_stage[4][kStageSlots];
+  __shared__ int s_stack[4][kStack];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = min(*a.ck_count, (uint32_t)a.ck_cap);
   for (;;) {
@@ -1399,7 +1361,8 @@ __global__ __launch_bounds__(256) void fmm_chunk_kernel(FmmArgs a, uint32_t *__r
     if (li < 0) continue;
     const LeafCtx L = leaf_ctx<MIXED, UNI>(a, li, lane);
     double ax = 0.0, ay = 0.0;
-    run_chunk<MIXED, UNI>(a, L, r, r.end, lane, s_stage[w], ax, ay);
+    run_chunk<MIXED, UNI>(a, L, (int64_t)rfl64((uint64_t)r.cum), (int64_t)rfl64((uint64_t)r.end),
+                          lane, s_stage[w], s_stack[w], ax, ay);
     const double2 P = reduce_replicas(L, lane, s_red[w], ax, ay);
     if (L.tact && L.rep == 0) a.part[(size_t)j * 64 + L.ti] = P;
   }
