@@ -187,8 +187,10 @@ int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int npare
 // m = q .. 0, then c_ij lam^(i+j) / sigma in Horner order.
 __device__ inline int smooth_tri(int m) { return m * (kSmoothQ + 1) - m * (m - 1) / 2; }
 
-// One wave per node; lane t < (q+1)(q+2)/2 sums term (m, n) over the node's
-// blobs (FP64), then lane k < kSmoothC forms output coefficient k.
+// One wave per node.  The node's blobs go through in chunks of 64: lane j
+// forms cir_j d_j^m and conj(d_j)^n (m, n <= q) of blob j in LDS, then lane
+// t < (q+1)(q+2)/2 adds the products of term t = (m, n) over the chunk (FP64).
+// Lane k < kSmoothC then forms output coefficient k.
 __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restrict__ nodes,
                                                          int nnodes,
                                                          const int32_t *__restrict__ nfirst,
@@ -200,6 +202,8 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
   constexpr int Q = kSmoothQ;
   constexpr int NT = (Q + 1) * (Q + 2) / 2;
   __shared__ double2 mu[NT];
+  __shared__ double2 pm[Q + 1][64];  // cir d^m
+  __shared__ double2 pn[Q + 1][64];  // conj(d)^n
   const int node = blockIdx.x;
   if (node >= nnodes) return;
   const NodeRec C = nodes[node];
@@ -209,28 +213,41 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
   double fact[2 * Q + 2];
   fact[0] = 1.0;
   for (int k = 1; k < 2 * Q + 2; k++) fact[k] = fact[k - 1] * (double)k;
+  int m = 0, n = 0;
   if (lane < NT) {
-    int m = 0;
     while (lane >= smooth_tri(m + 1)) m++;
-    const int n = lane - smooth_tri(m);
-    const int f = nfirst[node];
-    double ar = 0.0, ai = 0.0;
-    for (int j = f; j < f + cnt; j++) {
+    n = lane - smooth_tri(m);
+  }
+  const int f = nfirst[node];
+  double ar = 0.0, ai = 0.0;
+  for (int j0 = 0; j0 < cnt; j0 += 64) {
+    const int nb = min(64, cnt - j0);
+    if (lane < nb) {
+      const int j = f + j0 + lane;
       const double dr = (C.cx - sx[j]) * inv_sigma, di = (C.cy - sy[j]) * inv_sigma;
-      double pr = sg[j], pi = 0.0;
-      for (int k = 0; k < m; k++) {
-        const double t = pr * dr - pi * di;
+      double pr = sg[j], pi = 0.0, qr = 1.0, qi = 0.0;
+      for (int k = 0; k <= Q; k++) {
+        pm[k][lane] = make_double2(pr, pi);
+        pn[k][lane] = make_double2(qr, qi);
+        double t = pr * dr - pi * di;  // times d
         pi = pr * di + pi * dr;
         pr = t;
+        t = qr * dr + qi * di;  // times conj(d)
+        qi = qi * dr - qr * di;
+        qr = t;
       }
-      for (int k = 0; k < n; k++) {  // times conj(d)
-        const double t = pr * dr + pi * di;
-        pi = pi * dr - pr * di;
-        pr = t;
-      }
-      ar += pr;
-      ai += pi;
     }
+    __syncthreads();
+    if (lane < NT) {
+      for (int j = 0; j < nb; j++) {
+        const double2 u = pm[m][j], v = pn[n][j];
+        ar += u.x * v.x - u.y * v.y;
+        ai += u.x * v.y + u.y * v.x;
+      }
+    }
+    __syncthreads();
+  }
+  if (lane < NT) {
     const double s = 1.0 / (fact[m] * fact[n]);
     mu[lane] = make_double2(ar * s, ai * s);
   }
@@ -239,9 +256,9 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
     const double lam = 0.83255461115769775635;  // sqrt(ln 2)
     double vr = 0.0, vi = 0.0;
     if (lane <= Q) {
-      const int m = Q - lane;
-      const double2 v = mu[smooth_tri(m)];
-      const double sc = pow(lam, (double)(m + 1));
+      const int mm = Q - lane;
+      const double2 v = mu[smooth_tri(mm)];
+      const double sc = pow(lam, (double)(mm + 1));
       vr = v.x * sc;
       vi = v.y * sc;
     } else if (lane < kSmoothC) {
@@ -252,8 +269,7 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
       }
       const int i = Q - 1 - j - k;
       for (int l = 0; i + j + 2 * l + 1 <= Q; l++) {
-        const int m = j + l, n = i + l + 1;
-        const double2 v = mu[smooth_tri(m) + n];
+        const double2 v = mu[smooth_tri(j + l) + i + l + 1];
         double c = fact[j + l] / (fact[l] * fact[j]) * fact[i + l] / fact[i];
         if ((i + j + l) & 1) c = -c;
         vr += v.x * c;

@@ -339,7 +339,8 @@ __device__ inline int root_of(const FmmArgs &a, int first) {
 // the checkpoints on other waves, and a last pass adds their partial sums in
 // chunk order (deterministic).
 // ---------------------------------------------------------------------------
-constexpr int kMaxCk = 32;  // chunk boundaries per leaf kept in LDS
+constexpr int kMaxCk = 32;     // chunk boundaries per leaf kept in LDS
+constexpr int kFarList = 256;  // far cells listed per burst
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -920,8 +921,9 @@ __device__ inline bool far_cell(double inv_theta, double RA, double rcut, const 
 // The walk of one target leaf, level-parallel: `stack` (this wave's LDS,
 // kStack entries) holds the first-child index of every open node still to be
 // expanded.  A step takes up to 16 of them (the latest pushed) and classifies
-// their 64 children at once, lane = child: far cells go to far(c) and near
-// items to item(c, C, kind), both in lane order; open children are pushed.
+// their 64 children at once, lane = child: far cells go to far(mask, c) (the
+// lanes of `mask` hold one each) and near items to item(c, C, kind) in lane
+// order; open children are pushed.
 // item returning false ends the walk.  The order depends on the tree and the
 // target leaf only, so every pass over a leaf (fused, continuation, inline)
 // meets the same items in the same order.  Pops shrink to one node per step
@@ -976,13 +978,8 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
     }
     ns += __builtin_popcountll(om);
     wave_sync();
-    uint64_t m = fm;
-    while (m) {
-      const int b = __builtin_ctzll(m);
-      m &= m - 1;
-      far(rfl(__shfl(c, b, 64)));
-    }
-    m = im;
+    if (fm) far(fm, c);
+    uint64_t m = im;
     while (m) {
       const int b = __builtin_ctzll(m);
       m &= m - 1;
@@ -1070,7 +1067,7 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t sta
       }
     }
   };
-  walk_wide<SM>(a, L, lane, stack, [](int) {}, [&](int c, const NodeRec &C, int kind) -> bool {
+  walk_wide<SM>(a, L, lane, stack, [](uint64_t, int) {}, [&](int c, const NodeRec &C, int kind) -> bool {
     const int64_t c0 = cum;
     if (c0 >= end) return false;
     const int nt = item_tiles(a, c, C, kind);
@@ -1091,7 +1088,8 @@ template <bool MIXED, bool UNI>
 __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
                                  double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
                                  typename StageOf<MIXED>::T (*s_stage)[kStageSlots],
-                                 int64_t (*s_ck)[kMaxCk], int (*s_stack)[kStack]) {
+                                 int64_t (*s_ck)[kMaxCk], int (*s_stack)[kStack],
+                                 int (*s_far)[kFarList]) {
   li = rfl(li);
   const LeafCtx L = leaf_ctx<MIXED, UNI>(a, li, lane);
   const NodeRec &NA = L.NA;
@@ -1102,44 +1100,96 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   const int mi = lane & 15, mk = lane >> 4;
   const bool row_ok = mi < pA;
   const double2 sApow = cpow_u(make_double2(sA, 0.0), mi + 1);  // s_A^(i+1), real
-  FarAcc<MIXED> acc;
 
-  // Four far cells (c[k] < 0: none) -> up to two MFMA steps.  Lane group mk
-  // (16 lanes, row i = mi) forms (s_A w)^(i+1) and (-1)^(i+1) a_i w^i of
-  // cell c[mk] once; each MFMA step then takes the real and imaginary parts
-  // of two cells (K slots cell re, cell im, cell' re, cell' im) from the
-  // groups that formed them (lane permutes).
-  int pend[4] = {-1, -1, -1, -1};
-  int npend = 0;
-  auto far_quad = [&]() {
-    const int cell = pend[mk];
-    double2 pw = make_double2(0.0, 0.0), q = make_double2(0.0, 0.0);
-    if (cell >= 0 && row_ok) {
-      const NodeRec C = a.nodes[cell];
-      const double dr = C.cx - NA.cx, di = C.cy - NA.cy;
-      const double inv = 1.0 / (dr * dr + di * di);
-      const double2 wv = make_double2(dr * inv, -di * inv);
-      const double2 wi = cpow_u(wv, mi);                          // w^i
-      pw = cmul(cmul(wi, wv), make_double2(sApow.x, 0.0));        // (s_A w)^(i+1)
-      const double2 ak = a.coef[(size_t)cell * a.p + mi];
-      q = cmul(ak, wi);                                           // a_i w^i
-      if ((mi & 1) == 0) q = make_double2(-q.x, -q.y);            // (-1)^(i+1)
+  // Far cells are listed in LDS (s_far) by the walk and summed in bursts, at
+  // the end or when the list is full, so the MFMA accumulators live only
+  // there.  Per quad of cells, lane group mk (16 lanes, row i = mi) forms
+  // (s_A w)^(i+1) and (-1)^(i+1) a_i w^i of cell 4q + mk once; each MFMA step
+  // then takes the real and imaginary parts of two cells (K slots cell re,
+  // cell im, cell' re, cell' im) from the groups that formed them (lane
+  // permutes).  The next quad's cell data is loaded before this quad's
+  // arithmetic.  Each burst adds its part of the local expansion,
+  // Lt_l = (1/s_A) sum_k C(l+k,k) G[l][k], to s_loc.
+  if (lane < kFmmMaxP) s_loc[w][lane] = make_double2(0.0, 0.0);
+  int nfar = 0;
+  auto far_flush = [&]() {
+    FarAcc<MIXED> acc;
+    const int nq = (nfar + 3) >> 2;
+    double dr = 0.0, di = 0.0;
+    double2 ak = make_double2(0.0, 0.0);
+    bool ok = false;
+    auto fetch = [&](int q, double &r, double &i, double2 &k, bool &v) {
+      const int idx = 4 * q + mk;
+      const int cell = idx < nfar ? s_far[w][idx] : -1;
+      v = cell >= 0 && row_ok;
+      if (v) {
+        const NodeRec C = a.nodes[cell];
+        r = C.cx - NA.cx;
+        i = C.cy - NA.cy;
+        k = a.coef[(size_t)cell * a.p + mi];
+      }
+    };
+    fetch(0, dr, di, ak, ok);
+    for (int q = 0; q < nq; q++) {
+      double ndr = 0.0, ndi = 0.0;
+      double2 nak = make_double2(0.0, 0.0);
+      bool nok = false;
+      if (q + 1 < nq) fetch(q + 1, ndr, ndi, nak, nok);
+      double2 pw = make_double2(0.0, 0.0), qv = make_double2(0.0, 0.0);
+      if (ok) {
+        const double inv = 1.0 / (dr * dr + di * di);
+        const double2 wv = make_double2(dr * inv, -di * inv);
+        const double2 wi = cpow_u(wv, mi);                          // w^i
+        pw = cmul(cmul(wi, wv), make_double2(sApow.x, 0.0));        // (s_A w)^(i+1)
+        qv = cmul(ak, wi);                                          // a_i w^i
+        if ((mi & 1) == 0) qv = make_double2(-qv.x, -qv.y);         // (-1)^(i+1)
+      }
+      const bool im = (mk & 1) != 0;
+      for (int st = 0; st < 2; st++) {
+        if (4 * q + 2 * st >= nfar) continue;  // wave-uniform
+        const int src = mi + 16 * (2 * st + (mk >> 1));
+        const double px = __shfl(pw.x, src, 64), py = __shfl(pw.y, src, 64);
+        const double qx = __shfl(qv.x, src, 64), qy = __shfl(qv.y, src, 64);
+        acc.add(im ? py : px, im ? -qy : qx, im ? qx : qy);
+        acc.tick();
+      }
+      dr = ndr;
+      di = ndi;
+      ak = nak;
+      ok = nok;
     }
-    const bool im = (mk & 1) != 0;
-    for (int st = 0; st < 2; st++) {
-      if (pend[2 * st] < 0 && pend[2 * st + 1] < 0) continue;  // wave-uniform
-      const int src = mi + 16 * (2 * st + (mk >> 1));
-      const double px = __shfl(pw.x, src, 64), py = __shfl(pw.y, src, 64);
-      const double qx = __shfl(q.x, src, 64), qy = __shfl(q.y, src, 64);
-      acc.add(im ? py : px, im ? -qy : qx, im ? qx : qy);
-      acc.tick();
+    acc.fold();
+    // lane holds rows acc.row(mk, r), column mi of G (real part in r0,
+    // imaginary in r1)
+    for (int r = 0; r < 4; r++) {
+      const int row = acc.row(mk, r);
+      double bc = 1.0;  // C(row + mi, mi)
+      for (int t = 1; t <= mi; t++) bc = bc * (double)(row + t) / (double)t;
+      double gr = (mi < pA) ? acc.r0(r) * bc : 0.0;
+      double gi = (mi < pA) ? acc.r1(r) * bc : 0.0;
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) {
+        gr += __shfl_xor(gr, o, 64);
+        gi += __shfl_xor(gi, o, 64);
+      }
+      if (mi == 0 && row < kFmmMaxP) {
+        double2 v = s_loc[w][row];
+        v.x += gr / sA;
+        v.y += gi / sA;
+        s_loc[w][row] = v;
+      }
     }
-    pend[0] = pend[1] = pend[2] = pend[3] = -1;
-    npend = 0;
+    wave_sync();
+    nfar = 0;
   };
-  auto far = [&](int c) {
-    pend[npend++] = c;
-    if (npend == 4) far_quad();
+  auto far = [&](uint64_t fm, int c) {
+    const int nf = __builtin_popcountll(fm);
+    if (nfar + nf > kFarList) far_flush();
+    if ((fm >> lane) & 1)
+      s_far[w][nfar + __builtin_amdgcn_mbcnt_hi((uint32_t)(fm >> 32),
+                                                __builtin_amdgcn_mbcnt_lo((uint32_t)fm, 0))] = c;
+    nfar += nf;
+    wave_sync();
   };
 
   // near work: summed here up to the first threshold, then only counted,
@@ -1201,25 +1251,8 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   if constexpr (SM) {
     if (snb > 0) smooth_batch(a, L, scell, snb, lane, s_stage[w], ax, ay);
   }
-  if (npend > 0) far_quad();
-  acc.fold();
+  if (nfar > 0) far_flush();
 
-  // local expansion: Lt_l = (1/s_A) sum_k C(l+k,k) G[l][k]; lane holds rows
-  // acc.row(mk, r), column mi of G (real part in r0, imaginary in r1)
-  for (int r = 0; r < 4; r++) {
-    const int row = acc.row(mk, r);
-    // C(row + mi, mi)
-    double bc = 1.0;
-    for (int t = 1; t <= mi; t++) bc = bc * (double)(row + t) / (double)t;
-    double gr = (mi < pA) ? acc.r0(r) * bc : 0.0;
-    double gi = (mi < pA) ? acc.r1(r) * bc : 0.0;
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-      gr += __shfl_xor(gr, o, 64);
-      gi += __shfl_xor(gi, o, 64);
-    }
-    if (mi == 0 && row < kFmmMaxP) s_loc[w][row] = make_double2(gr / sA, gi / sA);
-  }
   // s_loc[w], s_red[w], s_stage[w] and s_ck[w] belong to this wave alone: a
   // wave-level fence orders the LDS writes of some lanes before the reads of
   // others
@@ -1311,9 +1344,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fm
   __shared__ typename StageOf<MIXED>::T s_stage[1][kStageSlots];
   __shared__ int64_t s_ck[1][kMaxCk];
   __shared__ int s_stack[1][kStack];
+  __shared__ int s_far[1][kFarList];
   const int li = xcd_leaf(blockIdx.x, a.nleaves);
   if (li >= a.nleaves) return;
-  fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck, s_stack);
+  fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck, s_stack,
+                               s_far);
 }
 
 // Persistent form: a fixed grid whose waves take target leaves from a work
@@ -1330,6 +1365,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
   __shared__ typename StageOf<MIXED>::T s_stage[4][kStageSlots];
   __shared__ int64_t s_ck[4][kMaxCk];
   __shared__ int s_stack[4][kStack];
+  __shared__ int s_far[4][kFarList];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = *nleaves;
   for (;;) {
@@ -1337,7 +1373,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
     if (lane == 0) li = atomicAdd(counter, 1u);
     li = __shfl(li, 0, 64);
     if (li >= total) break;
-    fmm_process_leaf<MIXED, UNI>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck, s_stack);
+    fmm_process_leaf<MIXED, UNI>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck, s_stack,
+                                 s_far);
   }
 }
 
