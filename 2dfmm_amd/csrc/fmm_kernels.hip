@@ -797,15 +797,20 @@ __device__ inline void smooth_eval(float zx, float zy, const float4 *row, float 
   gy += fmaf(E, pr, s0r);
 }
 
-// Smooth cells are summed kSmoothBatch at a time: replica r of every target
-// takes cell r of the batch.  Work units of one cell: kSmoothUnits (about the
-// cost of one evaluation in pair steps) shared by the cells of a batch.
+// Smooth cells are summed up to kSmoothBatch at a time (the stage holds
+// their coefficients).  Work units of one evaluation: kSmoothUnits (about its
+// cost in pair steps).
 constexpr int kSmoothBatch = 5;
 constexpr int kSmoothRow = kSmoothF4 + 1;  // coefficients + the cell's offset
 static_assert(kSmoothBatch * kSmoothRow <= kStageSlots, "smooth batch exceeds the stage");
-__device__ inline int smooth_batch_size(const LeafCtx &L) { return min(L.nrep, kSmoothBatch); }
+// Batch: as many cells as the stage holds that split evenly over the
+// replicas (replica r takes cells r, r + nrep, ...).
+__device__ inline int smooth_batch_size(const LeafCtx &L) {
+  return L.nrep >= kSmoothBatch ? kSmoothBatch : L.nrep * (kSmoothBatch / L.nrep);
+}
+// a cell's share of the batch's time: one evaluation per replica in turn
 __device__ inline int64_t smooth_units(const LeafCtx &L) {
-  const int b = smooth_batch_size(L);
+  const int b = min(L.nrep, kSmoothBatch);
   return (kSmoothUnits + b - 1) / b;
 }
 
@@ -820,8 +825,8 @@ __device__ inline NodeRec srun_node(const FmmArgs &a, int c, int t, int lane, in
 
 // The nb cells held by lanes 0..nb-1 in `cell`: their coefficients and offsets
 // from the target leaf (FP64 difference, then FP32 scaled) are staged in the
-// wave's LDS slice st, kSmoothRow float4 per cell; then every (target,
-// replica r < nb) lane sums cell r.
+// wave's LDS slice st, kSmoothRow float4 per cell; then replica r of every
+// target sums cells r, r + nrep, ... of the batch.
 __device__ inline void smooth_batch(const FmmArgs &a, const LeafCtx &L, int cell, int nb, int lane,
                                     float4 *st, double &ax, double &ay) {
   const int k0 = lane / kSmoothRow, k1 = (lane + 64) / kSmoothRow;
@@ -841,15 +846,19 @@ __device__ inline void smooth_batch(const FmmArgs &a, const LeafCtx &L, int cell
   stage(lane, k0, c0);
   stage(lane + 64, k1, c1);
   wave_sync();
-  float gx = 0.f, gy = 0.f;
-  if (L.tact && L.rep < nb) {
-    const float4 *row = st + L.rep * kSmoothRow;
-    const float4 o = row[kSmoothF4];
-    smooth_eval(L.lx - o.x, L.ly - o.y, row, gx, gy);
+  // each cell's value goes to the FP64 sums alone, so a cell adds the same
+  // whatever batch it falls in
+  if (L.tact) {
+    for (int k = L.rep; k < nb; k += L.nrep) {
+      const float4 *row = st + k * kSmoothRow;
+      const float4 o = row[kSmoothF4];
+      float gx = 0.f, gy = 0.f;
+      smooth_eval(L.lx - o.x, L.ly - o.y, row, gx, gy);
+      ax += (double)gx;
+      ay += (double)gy;
+    }
   }
   wave_sync();
-  ax += (double)gx;
-  ay += (double)gy;
 }
 
 // Steps ta..tb-1 of near item c of any kind; smooth cells go to push(cell)
