@@ -146,9 +146,10 @@ int launch_direct(int ns, const double *d_sxy, const double *d_cir, const double
                   hipStream_t s);
 
 // ---- accurate fast mode (fmm_kernels.hip) ------------------------------
-// One chunk of the near work of a target leaf with much of it: the tiles of
-// its near items whose work before them (in walk order) lies in [cum, end);
-// the continuation pass walks the leaf again and sums them.
+// One chunk of the near work of a target leaf with much of it: the tiles
+// from position cum up to end, a position being (ordinal of the near item in
+// walk order) << 32 | tile; the continuation pass walks the leaf again and
+// sums them.
 struct SplitCk {
   int64_t cum, end;
   int32_t li, pad;  // target leaf (-1: skip)

@@ -1055,16 +1055,21 @@ __device__ inline double2 reduce_replicas(const LeafCtx &L, int lane, double2 *r
   return make_double2(sx, sy);
 }
 
-// Near work of one chunk: the tiles whose work before them lies in
-// [start, end), met by walking the leaf again from the root (far cells are
-// the fused pass's); smooth cells in batches, the last batch summed at the
-// end.
+// Positions in a leaf's near work: (item ordinal in walk order) << 32 | tile.
+__device__ inline int64_t work_pos(int k, int t) { return ((int64_t)k << 32) | (uint32_t)t; }
+
+// Near work of one chunk: the tiles from position start up to (not
+// including) end, met by walking the leaf again from the root (far cells are
+// the fused pass's; items before the chunk are only counted); smooth cells in
+// batches, the last batch summed at the end.
 template <bool MIXED, bool UNI>
 __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t start, int64_t end,
                                  int lane, typename StageOf<MIXED>::T *st, int *stack, double &ax,
                                  double &ay) {
   constexpr bool SM = MIXED && UNI;
-  int64_t cum = 0;
+  const int k0 = (int)(start >> 32), t0 = (int)(uint32_t)start;
+  const int k1 = (int)(end >> 32), t1 = (int)(uint32_t)end;
+  int k = 0;
   int scell = 0, snb = 0;
   const int sbsz = smooth_batch_size(L);
   auto push = [&](int cell) {
@@ -1077,15 +1082,14 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t sta
     }
   };
   walk_wide<SM>(a, L, lane, stack, [](uint64_t, int) {}, [&](int c, const NodeRec &C, int kind) -> bool {
-    const int64_t c0 = cum;
-    if (c0 >= end) return false;
+    const int ki = k++;
+    if (ki < k0) return true;
+    if (ki > k1 || (ki == k1 && t1 == 0)) return false;
     const int nt = item_tiles(a, c, C, kind);
-    cum = c0 + item_prefix(a, L, c, C, kind, nt, nt);
-    if (cum <= start) return true;
-    const int ta = item_find(a, L, c, C, kind, nt, c0, start, 0);
-    const int tb = item_find(a, L, c, C, kind, nt, c0, end, ta);
+    const int ta = ki == k0 ? t0 : 0;
+    const int tb = ki == k1 ? min(t1, nt) : nt;
     if (ta < tb) item_steps<MIXED, UNI>(a, L, c, C, kind, ta, tb, lane, st, ax, ay, push);
-    return tb == nt;
+    return ki < k1;
   });
   if constexpr (SM) {
     if (snb > 0) smooth_batch(a, L, scell, snb, lane, st, ax, ay);
@@ -1097,8 +1101,8 @@ template <bool MIXED, bool UNI>
 __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
                                  double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
                                  typename StageOf<MIXED>::T (*s_stage)[kStageSlots],
-                                 int64_t (*s_ck)[kMaxCk], int (*s_stack)[kStack],
-                                 int (*s_far)[kFarList]) {
+                                 int64_t (*s_ck)[kMaxCk], int64_t (*s_cku)[kMaxCk],
+                                 int (*s_stack)[kStack], int (*s_far)[kFarList]) {
   li = rfl(li);
   const LeafCtx L = leaf_ctx<MIXED, UNI>(a, li, lane);
   const NodeRec &NA = L.NA;
@@ -1222,7 +1226,9 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
       }
     }
   };
+  int nitem = 0;
   auto item = [&](int c, const NodeRec &C, int kind) -> bool {
+    const int ki = nitem++;
     const int nt = item_tiles(a, c, C, kind);
     const int64_t c0 = cum;
     const int64_t total = item_prefix(a, L, c, C, kind, nt, nt);
@@ -1237,18 +1243,28 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
       t = item_find(a, L, c, C, kind, nt, c0, thr, t);
       if (t >= nt) break;
       if (nck == kMaxCk) {
-        int64_t keep = 0;
-        if (lane < kMaxCk / 2) keep = s_ck[w][2 * lane];
+        int64_t keep = 0, keepu = 0;
+        if (lane < kMaxCk / 2) {
+          keep = s_ck[w][2 * lane];
+          keepu = s_cku[w][2 * lane];
+        }
         wave_sync();
-        if (lane < kMaxCk / 2) s_ck[w][lane] = keep;
+        if (lane < kMaxCk / 2) {
+          s_ck[w][lane] = keep;
+          s_cku[w][lane] = keepu;
+        }
         wave_sync();
         nck = kMaxCk / 2;
         step *= 2;
-        thr = (int64_t)rfl64((uint64_t)s_ck[w][nck - 1]) + step;
+        thr = (int64_t)rfl64((uint64_t)s_cku[w][nck - 1]) + step;
         continue;
       }
+      // a boundary: its position (where the chunk starts) and its work count
       const int64_t at = c0 + item_prefix(a, L, c, C, kind, nt, t);
-      if (lane == 0) s_ck[w][nck] = at;
+      if (lane == 0) {
+        s_ck[w][nck] = work_pos(ki, t);
+        s_cku[w][nck] = at;
+      }
       nck++;
       thr = at + step;
     }
@@ -1280,7 +1296,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   }
   const double2 S = reduce_replicas(L, lane, s_red[w], ax, ay);
 
-  // hand the chunks (work from s_ck[k] to s_ck[k + 1]) to the continuation
+  // hand the chunks (positions s_ck[k] to s_ck[k + 1]) to the continuation
   // pass; without room for them, run them here with the same arithmetic
   bool inline_chunks = false;
   double2 extra = make_double2(0.0, 0.0);
@@ -1354,9 +1370,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fm
   __shared__ int64_t s_ck[1][kMaxCk];
   __shared__ int s_stack[1][kStack];
   __shared__ int s_far[1][kFarList];
+  __shared__ int64_t s_cku[1][kMaxCk];
   const int li = xcd_leaf(blockIdx.x, a.nleaves);
   if (li >= a.nleaves) return;
-  fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck, s_stack,
+  fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck, s_cku, s_stack,
                                s_far);
 }
 
@@ -1375,6 +1392,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
   __shared__ int64_t s_ck[4][kMaxCk];
   __shared__ int s_stack[4][kStack];
   __shared__ int s_far[4][kFarList];
+  __shared__ int64_t s_cku[4][kMaxCk];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = *nleaves;
   for (;;) {
@@ -1382,8 +1400,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
     if (lane == 0) li = atomicAdd(counter, 1u);
     li = __shfl(li, 0, 64);
     if (li >= total) break;
-    fmm_process_leaf<MIXED, UNI>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck, s_stack,
-                                 s_far);
+    fmm_process_leaf<MIXED, UNI>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck, s_cku,
+                                 s_stack, s_far);
   }
 }
 
