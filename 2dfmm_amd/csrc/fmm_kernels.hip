@@ -1209,8 +1209,10 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   // with a chunk boundary every `step` units (kept in LDS; when the slots run
   // out, every second boundary is dropped and the step doubles)
   const bool split = a.split_budget > 0;
-  int64_t cum = 0, step = split ? a.split_budget : 0;
-  int64_t thr = split ? step : INT64_MAX;
+  // this wave sums the first split_budget units; the continuation chunks
+  // are a quarter of that, so the pass after this one ends sooner
+  int64_t cum = 0, step = split ? max(a.split_budget / 4, 1) : 0;
+  int64_t thr = split ? (int64_t)a.split_budget : INT64_MAX;
   int nck = 0;
   constexpr bool SM = MIXED && UNI;
   // pending smooth cells (lane k holds cell k of the batch); only the summing
