@@ -385,6 +385,12 @@ struct StageOf<true> {
   typedef float4 T;
 };
 constexpr int kStageSlots = 128;
+// The mixed form's stage has room past kStageSlots for the values of one
+// smooth batch (5 cells x 64 targets, float2 each) and the 64 targets'
+// coordinates.
+constexpr int kSmoothResSlots = 192;
+template <bool MIXED>
+constexpr int stage_slots() { return MIXED ? kStageSlots + kSmoothResSlots : kStageSlots; }
 
 // Per-target-leaf values; lanes are (target ti, replica rep) pairs.
 // Source j of a staged step goes to LDS slot (j mod nrep) stride + j / nrep
@@ -803,15 +809,27 @@ __device__ inline void smooth_eval(float zx, float zy, const float4 *row, float 
 constexpr int kSmoothBatch = 5;
 constexpr int kSmoothRow = kSmoothF4 + 1;  // coefficients + the cell's offset
 static_assert(kSmoothBatch * kSmoothRow <= kStageSlots, "smooth batch exceeds the stage");
-// Batch: as many cells as the stage holds that split evenly over the
-// replicas (replica r takes cells r, r + nrep, ...).
+static_assert((kSmoothBatch + 1) * 64 * 8 <= kSmoothResSlots * 16, "smooth values exceed the stage");
+// A batch of nb cells is nb x cA evaluations, (target, cell) pairs dealt to
+// the 64 lanes in rounds; nb (at most the kSmoothBatch the stage holds) is
+// chosen to fill the rounds best, the larger on a tie.
 __device__ inline int smooth_batch_size(const LeafCtx &L) {
-  return L.nrep >= kSmoothBatch ? kSmoothBatch : L.nrep * (kSmoothBatch / L.nrep);
+  int best = 1, bu = 0;  // bu: evaluations per 64 lane slots, x 1024
+  for (int nb = 1; nb <= kSmoothBatch; nb++) {
+    const int ne = nb * L.cA;
+    const int u = (ne * 1024) / (64 * ((ne + 63) >> 6));
+    if (u >= bu) {
+      bu = u;
+      best = nb;
+    }
+  }
+  return best;
 }
-// a cell's share of the batch's time: one evaluation per replica in turn
+// a cell's share of the batch's time: the batch's rounds over its cells
 __device__ inline int64_t smooth_units(const LeafCtx &L) {
-  const int b = min(L.nrep, kSmoothBatch);
-  return (kSmoothUnits + b - 1) / b;
+  const int nb = smooth_batch_size(L);
+  const int rounds = (nb * L.cA + 63) >> 6;
+  return (kSmoothUnits * rounds + nb - 1) / nb;
 }
 
 // Node k (lane k) of tile t of smooth run c: its record (cnt 0 past the end).
@@ -825,8 +843,11 @@ __device__ inline NodeRec srun_node(const FmmArgs &a, int c, int t, int lane, in
 
 // The nb cells held by lanes 0..nb-1 in `cell`: their coefficients and offsets
 // from the target leaf (FP64 difference, then FP32 scaled) are staged in the
-// wave's LDS slice st, kSmoothRow float4 per cell; then replica r of every
-// target sums cells r, r + nrep, ... of the batch.
+// wave's LDS slice st, kSmoothRow float4 per cell.  The nb x cA evaluations
+// e = (target e mod cA, cell e / cA) are dealt to the lanes in rounds (lane e
+// starts at its own (ti, rep)); each value goes to st's result slots, and
+// the lane of target t (replica 0) adds its nb values in cell order to its
+// FP64 sums -- a cell adds the same whatever batch it falls in.
 __device__ inline void smooth_batch(const FmmArgs &a, const LeafCtx &L, int cell, int nb, int lane,
                                     float4 *st, double &ax, double &ay) {
   const int k0 = lane / kSmoothRow, k1 = (lane + 64) / kSmoothRow;
@@ -843,19 +864,35 @@ __device__ inline void smooth_batch(const FmmArgs &a, const LeafCtx &L, int cell
       }
     }
   };
+  float2 *res = reinterpret_cast<float2 *>(st + kStageSlots);
+  float2 *tl = res + kSmoothBatch * 64;  // target coordinates
   stage(lane, k0, c0);
   stage(lane + 64, k1, c1);
+  if (L.rep == 0) tl[L.ti] = make_float2(L.lx, L.ly);
   wave_sync();
-  // each cell's value goes to the FP64 sums alone, so a cell adds the same
-  // whatever batch it falls in
-  if (L.tact) {
-    for (int k = L.rep; k < nb; k += L.nrep) {
-      const float4 *row = st + k * kSmoothRow;
-      const float4 o = row[kSmoothF4];
-      float gx = 0.f, gy = 0.f;
-      smooth_eval(L.lx - o.x, L.ly - o.y, row, gx, gy);
-      ax += (double)gx;
-      ay += (double)gy;
+  const int cA = L.cA, ne = nb * cA;
+  const int r64 = 64 % cA, q64 = 64 / cA;
+  int t = L.ti, k = L.rep;
+  for (int e = lane; e < ne; e += 64) {
+    const float4 *row = st + k * kSmoothRow;
+    const float4 o = row[kSmoothF4];
+    const float2 p = tl[t];
+    float gx = 0.f, gy = 0.f;
+    smooth_eval(p.x - o.x, p.y - o.y, row, gx, gy);
+    res[e] = make_float2(gx, gy);
+    t += r64;
+    k += q64;
+    if (t >= cA) {
+      t -= cA;
+      k++;
+    }
+  }
+  wave_sync();
+  if (L.tact && L.rep == 0) {
+    for (int kk = 0; kk < nb; kk++) {
+      const float2 g = res[kk * cA + L.ti];
+      ax += (double)g.x;
+      ay += (double)g.y;
     }
   }
   wave_sync();
@@ -1100,7 +1137,7 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t sta
 template <bool MIXED, bool UNI>
 __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
                                  double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
-                                 typename StageOf<MIXED>::T (*s_stage)[kStageSlots],
+                                 typename StageOf<MIXED>::T (*s_stage)[stage_slots<MIXED>()],
                                  int64_t (*s_ck)[kMaxCk], int64_t (*s_cku)[kMaxCk],
                                  int (*s_stack)[kStack], int (*s_far)[kFarList]) {
   li = rfl(li);
@@ -1209,10 +1246,8 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   // with a chunk boundary every `step` units (kept in LDS; when the slots run
   // out, every second boundary is dropped and the step doubles)
   const bool split = a.split_budget > 0;
-  // this wave sums the first split_budget units; the continuation chunks
-  // are a quarter of that, so the pass after this one ends sooner
-  int64_t cum = 0, step = split ? max(a.split_budget / 4, 1) : 0;
-  int64_t thr = split ? (int64_t)a.split_budget : INT64_MAX;
+  int64_t cum = 0, step = split ? a.split_budget : 0;
+  int64_t thr = split ? step : INT64_MAX;
   int nck = 0;
   constexpr bool SM = MIXED && UNI;
   // pending smooth cells (lane k holds cell k of the batch); only the summing
@@ -1368,7 +1403,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fm
     FmmArgs a) {
   __shared__ double2 s_loc[1][kFmmMaxP];
   __shared__ double2 s_red[1][64];
-  __shared__ typename StageOf<MIXED>::T s_stage[1][kStageSlots];
+  __shared__ typename StageOf<MIXED>::T s_stage[1][stage_slots<MIXED>()];
   __shared__ int64_t s_ck[1][kMaxCk];
   __shared__ int s_stack[1][kStack];
   __shared__ int s_far[1][kFarList];
@@ -1390,7 +1425,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
                                                                   uint32_t *__restrict__ counter) {
   __shared__ double2 s_loc[4][kFmmMaxP];
   __shared__ double2 s_red[4][64];
-  __shared__ typename StageOf<MIXED>::T s_stage[4][kStageSlots];
+  __shared__ typename StageOf<MIXED>::T s_stage[4][stage_slots<MIXED>()];
   __shared__ int64_t s_ck[4][kMaxCk];
   __shared__ int s_stack[4][kStack];
   __shared__ int s_far[4][kFarList];
@@ -1413,7 +1448,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
 template <bool MIXED, bool UNI>
 __global__ __launch_bounds__(256) void fmm_chunk_kernel(FmmArgs a, uint32_t *__restrict__ counter) {
   __shared__ double2 s_red[4][64];
-  __shared__ typename StageOf<MIXED>::T s_stage[4][kStageSlots];
+  __shared__ typename StageOf<MIXED>::T s_stage[4][stage_slots<MIXED>()];
   __shared__ int s_stack[4][kStack];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = min(*a.ck_count, (uint32_t)a.ck_cap);
