@@ -148,7 +148,8 @@ struct Context {
   DevBuf<RootBox> root;
   std::vector<int> split_off;
   // host side
-  uint32_t *h_total = nullptr;
+  uint32_t *h_total = nullptr;  // pinned; [8]: chunk records taken by the last evaluation
+  size_t ck_need = 0;
   double *h_uv = nullptr;
   size_t h_uv_cap = 0;
   hipEvent_t ev_uv = nullptr;
@@ -176,6 +177,7 @@ static int ctx_init() {
   }
   FMM_HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   FMM_HIP_TRY(hipHostMalloc((void **)&g.h_total, 64, hipHostMallocDefault));
+  memset(g.h_total, 0, 64);
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_uv, hipEventDisableTiming));
   for (auto &e : g.tev) FMM_HIP_TRY(hipEventCreate(&e));
   g.init = true;
@@ -475,12 +477,19 @@ static FmmArgs fmm_args(double *d_uv) {
 }
 
 // Buffers of the split passes for the current tree: room for two chunks per
-// target leaf on average (a fused wave that finds no room runs its chunks
-// itself), and the grid of the continuation passes.
+// target leaf on average, or 1.25 x the most chunks an evaluation has asked
+// for (a fused wave that finds no room runs its chunks itself, serially, so
+// the buffer grows after an evaluation overflowed it), and the grid of the
+// continuation passes.
 static int ensure_split() {
   if (g.split_budget <= 0) return 0;
   hipStream_t s = g.stream;
-  const size_t cap = 2 * (size_t)g.ntleaves + 4096;
+  // chunks taken by the last evaluation (pinned copy, read without waiting:
+  // a hint only)
+  const size_t taken = g.h_total[8];
+  g.ck_need = std::max(g.ck_need, taken + taken / 4);
+  const size_t cap = std::min(std::max(2 * (size_t)g.ntleaves + 4096, g.ck_need),
+                              16 * (size_t)g.ntleaves + 4096);
   if (g.ck.ensure(cap, s) || g.leaf_ck.ensure((size_t)g.ntleaves + 1, s) ||
       g.part.ensure(64 * g.ck.cap, s) || g.split_ctr.ensure(2, s))
     return -1;
@@ -579,6 +588,9 @@ static int eval_sources_dev(double *d_uv) {
     rc = launch_fmm_eval(fmm_args(d_uv), mixed, g.ntl_dev.p, g.split_ctr.p, split_blocks(),
                          g.stream);
   mark(6);
+  if (!rc && g.t_mode != FMM2D_MODE_TREECODE && g.split_budget > 0)
+    FMM_HIP_TRY(hipMemcpyAsync(g.h_total + 8, g.split_ctr.p, sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, g.stream));
   return rc;
 }
 
