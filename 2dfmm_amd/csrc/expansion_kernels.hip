@@ -158,9 +158,57 @@ __global__ __launch_bounds__(64) void m2m_kernel(const NodeRec *__restrict__ nod
   if (lane < nterms) coef[(size_t)par * nterms + lane] = make_double2(br, bi);
 }
 
+// Up to 16 terms: the four children at once, lane = 16 child + n; their
+// terms are then added in the order q = 3 .. 0 of m2m_kernel.
+__global__ __launch_bounds__(64) void m2m4_kernel(const NodeRec *__restrict__ nodes,
+                                                  const int32_t *__restrict__ parents,
+                                                  int nparents, int nterms,
+                                                  double2 *__restrict__ coef) {
+  __shared__ double2 dpow[4][16];
+  __shared__ double2 ach[4][16];
+  const int pi_ = blockIdx.x;
+  if (pi_ >= nparents) return;
+  const int lane = threadIdx.x, q = lane >> 4, n = lane & 15;
+  const int par = parents[pi_];
+  const NodeRec P = nodes[par];
+  const int ch = P.first + q;
+  const NodeRec C = nodes[ch];
+  const bool has = node_count(C.cnt) >= 1;
+  if (n < nterms) {
+    // cpow(-(p_pos - c_pos), n - k) of fmm.c:50
+    dpow[q][n] = cpow_int(-(P.cx - C.cx), -(P.cy - C.cy), n);
+    ach[q][n] = has ? coef[(size_t)ch * nterms + n] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double cr = 0.0, ci = 0.0;
+  if (has && n < nterms) {
+    double binom = 1.0;
+    for (int k = 0; k <= n; k++) {
+      if (k > 0) binom = binom * (double)(n - k + 1) / (double)k;
+      const double2 a = ach[q][k];
+      const double xr = a.x * binom, xi = a.y * binom;
+      const double2 d = dpow[q][n - k];
+      cr += xr * d.x - xi * d.y;
+      ci += xr * d.y + xi * d.x;
+    }
+  }
+  // children 3, 2, 1, 0 in turn
+  double br = 0.0, bi = 0.0;
+  for (int qq = 3; qq >= 0; qq--) {
+    br += __shfl(cr, 16 * qq + n, 64);
+    bi += __shfl(ci, 16 * qq + n, 64);
+  }
+  if (q == 0 && n < nterms) coef[(size_t)par * nterms + n] = make_double2(br, bi);
+}
+
 int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int nparents, int nterms,
                      double2 *d_coef, hipStream_t s) {
   if (nparents <= 0) return 0;
+  if (nterms <= 16) {
+    m2m4_kernel<<<nparents, 64, 0, s>>>(d_nodes, d_parents, nparents, nterms, d_coef);
+    FMM_HIP_TRY(hipGetLastError());
+    return 0;
+  }
   m2m_kernel<<<nparents, 64, 0, s>>>(d_nodes, d_parents, nparents, nterms, d_coef);
   FMM_HIP_TRY(hipGetLastError());
   return 0;

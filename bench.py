@@ -8,8 +8,9 @@ Gaussian mixture (centres U[0,1]^2, std log-uniform in [1e-3, 5e-2], core
 radius 5e-4, circulation N(0, 1/N)), seeded.
 
 The headline `value` is the mixed-precision mode with learned per-leaf
-order and near-field radius (FP32 pair sums accumulated in FP64, FP32-MFMA
-far-field contraction, FP64 expansions) when it meets rel. L2 <= 1e-6, else
+order and near-field radius (FP32 pair sums accumulated in FP64, smooth
+cells summed through their Taylor coefficients, FP32-MFMA far-field
+contraction, FP64 expansions) when it meets rel. L2 <= 1e-6, else
 the mixed mode with fixed parameters; the plain FP64 mode is timed on the
 same input, and each states its error against the FP64 direct sum on
 sampled targets.  The reference-rule treecode is timed beside them.  Inputs
@@ -270,7 +271,8 @@ def main():
             fmm.set_mode(fmm.MODE_MIXED if mixed else fmm.MODE_FMM64)
             fmm.set_fmm_params(args.p, args.leaf, args.theta, args.cut)
             prec = "f32" if mixed else "f64"
-            t = "true" if mixed else "false"
+            # <mixed, one core radius>: the workload has one core radius
+            t = "true, true" if mixed else "false, false"
             kernel = f"fmm_leaf_kernel<{t}> + fmm_chunk_kernel<{t}> + fmm_split_sum_kernel"
             if mode == "learned":
                 with np.load(wpath) as z:
