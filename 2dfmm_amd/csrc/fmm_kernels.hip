@@ -339,8 +339,8 @@ __device__ inline int root_of(const FmmArgs &a, int first) {
 // the checkpoints on other waves, and a last pass adds their partial sums in
 // chunk order (deterministic).
 // ---------------------------------------------------------------------------
-constexpr int kMaxCk = 32;     // chunk boundaries per leaf kept in LDS
-constexpr int kFarList = 256;  // far cells listed per burst
+constexpr int kMaxCk = 64;     // chunk boundaries per leaf kept in LDS
+constexpr int kFarList = 128;  // far cells listed per burst
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -975,7 +975,7 @@ __device__ inline bool far_cell(double inv_theta, double RA, double rcut, const 
 // meets the same items in the same order.  Pops shrink to one node per step
 // near the top of the stack; a depth-first descent then adds at most 3
 // entries per level, which kStackMargin covers.
-constexpr int kStack = 512;
+constexpr int kStack = 384;
 constexpr int kStackMargin = 96;
 template <bool SMOOTH, typename FarFn, typename ItemFn>
 __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, int *stack,
@@ -1246,8 +1246,11 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   // with a chunk boundary every `step` units (kept in LDS; when the slots run
   // out, every second boundary is dropped and the step doubles)
   const bool split = a.split_budget > 0;
-  int64_t cum = 0, step = split ? a.split_budget : 0;
-  int64_t thr = split ? step : INT64_MAX;
+  // the mixed form's chunks are an eighth of the first budget: few leaves
+  // are split there, and the continuation pass then ends with its longest
+  // chunk (walking the leaf again up to it, then summing it, on one wave)
+  int64_t cum = 0, step = split ? (MIXED ? max(a.split_budget / 8, 1) : a.split_budget) : 0;
+  int64_t thr = split ? (int64_t)a.split_budget : INT64_MAX;
   int nck = 0;
   constexpr bool SM = MIXED && UNI;
   // pending smooth cells (lane k holds cell k of the batch); only the summing
