@@ -38,8 +38,42 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kFmmMaxP = 16;
 
+// C(r + i, i) for r, i < kFmmMaxP (the M2L weights of a far-cell burst):
+// integers below 2^28, so every entry is exact in FP64.
+struct M2lBinom {
+  double v[kFmmMaxP][kFmmMaxP];
+};
+constexpr M2lBinom make_m2l_binom() {
+  M2lBinom t{};
+  for (int r = 0; r < kFmmMaxP; r++) {
+    double b = 1.0;
+    for (int i = 0; i < kFmmMaxP; i++) {
+      if (i > 0) b = b * (double)(r + i) / (double)i;
+      t.v[r][i] = b;
+    }
+  }
+  return t;
+}
+__constant__ M2lBinom kM2lBinom = make_m2l_binom();
+
 __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// a b with fused multiply-adds
+__device__ inline double2 cmul_fma(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
+
+// w^i for 0 <= i < 16 in the same steps in every lane (a loop over the bits
+// of a lane's own exponent runs the longest trip in every lane): the product
+// of w^(2^b) over the set bits b of i.
+__device__ inline double2 cpow16(double2 w, int i) {
+  const double2 one = make_double2(1.0, 0.0);
+  const double2 w2 = cmul_fma(w, w), w4 = cmul_fma(w2, w2), w8 = cmul_fma(w4, w4);
+  const double2 lo = cmul_fma((i & 1) ? w : one, (i & 2) ? w2 : one);
+  const double2 hi = cmul_fma((i & 4) ? w4 : one, (i & 8) ? w8 : one);
+  return cmul_fma(lo, hi);
 }
 
 __device__ inline double2 cpow_u(double2 b, int e) {
@@ -1187,11 +1221,17 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
       if (q + 1 < nq) fetch(q + 1, ndr, ndi, nak, nok);
       double2 pw = make_double2(0.0, 0.0), qv = make_double2(0.0, 0.0);
       if (ok) {
-        const double inv = 1.0 / (dr * dr + di * di);
+        // w = 1 / (dr + i di); 1 / |d|^2 from the hardware reciprocal and
+        // two Newton steps
+        const double r2 = fma(dr, dr, di * di);
+        double inv = __builtin_amdgcn_rcp(r2);
+        inv = inv * fma(-r2, inv, 2.0);
+        inv = inv * fma(-r2, inv, 2.0);
         const double2 wv = make_double2(dr * inv, -di * inv);
-        const double2 wi = cpow_u(wv, mi);                          // w^i
-        pw = cmul(cmul(wi, wv), make_double2(sApow.x, 0.0));        // (s_A w)^(i+1)
-        qv = cmul(ak, wi);                                          // a_i w^i
+        const double2 wi = cpow16(wv, mi);                          // w^i
+        const double2 wi1 = cmul_fma(wi, wv);
+        pw = make_double2(wi1.x * sApow.x, wi1.y * sApow.x);        // (s_A w)^(i+1)
+        qv = cmul_fma(ak, wi);                                      // a_i w^i
         if ((mi & 1) == 0) qv = make_double2(-qv.x, -qv.y);         // (-1)^(i+1)
       }
       const bool im = (mk & 1) != 0;
@@ -1213,8 +1253,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
     // imaginary in r1)
     for (int r = 0; r < 4; r++) {
       const int row = acc.row(mk, r);
-      double bc = 1.0;  // C(row + mi, mi)
-      for (int t = 1; t <= mi; t++) bc = bc * (double)(row + t) / (double)t;
+      const double bc = kM2lBinom.v[row][mi];  // C(row + mi, mi)
       double gr = (mi < pA) ? acc.r0(r) * bc : 0.0;
       double gi = (mi < pA) ? acc.r1(r) * bc : 0.0;
 #pragma unroll
