@@ -319,6 +319,7 @@ struct FarAcc;
 
 template <>
 struct FarAcc<false> {
+  typedef double T;  // type of the MFMA operands
   d4_t g0 = {0, 0, 0, 0}, g1 = {0, 0, 0, 0};
   __device__ inline void add(double a, double b0, double b1) {
     g0 = mfma16x16x4(a, b0, g0);
@@ -336,9 +337,10 @@ struct FarAcc<true> {
   f4_t c0 = {0, 0, 0, 0}, c1 = {0, 0, 0, 0};
   d4_t g0 = {0, 0, 0, 0}, g1 = {0, 0, 0, 0};
   int n = 0;
-  __device__ inline void add(double a, double b0, double b1) {
-    c0 = mfma16x16x4((float)a, (float)b0, c0);
-    c1 = mfma16x16x4((float)a, (float)b1, c1);
+  typedef float T;
+  __device__ inline void add(float a, float b0, float b1) {
+    c0 = mfma16x16x4(a, b0, c0);
+    c1 = mfma16x16x4(a, b1, c1);
   }
   __device__ inline void tick() {
     if (++n == 4) fold();
@@ -405,6 +407,26 @@ __device__ inline uint64_t rfl64(uint64_t v) {
 }
 __device__ inline double rfl64d(double v) {
   return __builtin_bit_cast(double, rfl64(__builtin_bit_cast(uint64_t, v)));
+}
+
+// The value of lane b (wave-uniform) in every lane: v_readlane, no LDS
+// round trip.
+__device__ inline int lane_of(int v, int b) { return __builtin_amdgcn_readlane(v, b); }
+__device__ inline double lane_of(double v, int b) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, b);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), b);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+// the record of node held by lane b
+__device__ inline NodeRec node_of_lane(const NodeRec &N, int b) {
+  NodeRec S;
+  S.cx = lane_of(N.cx, b);
+  S.cy = lane_of(N.cy, b);
+  S.region = lane_of(N.region, b);
+  S.first = lane_of(N.first, b);
+  S.cnt = lane_of(N.cnt, b);
+  return S;
 }
 
 // LDS stage of one wave: up to 64 sources of a near step, replica-major
@@ -954,17 +976,11 @@ __device__ inline void item_steps(const FmmArgs &a, const LeafCtx &L, int c, con
         while (m) {
           const int k = __builtin_ctzll(m);
           m &= m - 1;
-          const int node = rfl(__shfl(idx, k, 64));
+          const int node = lane_of(idx, k);
           if ((sm >> k) & 1) {
             push(node);
           } else {
-            NodeRec S;
-            S.cx = rfl64d(__shfl(N.cx, k, 64));
-            S.cy = rfl64d(__shfl(N.cy, k, 64));
-            S.region = rfl64d(__shfl(N.region, k, 64));
-            S.first = rfl(__shfl(N.first, k, 64));
-            S.cnt = rfl(__shfl(N.cnt, k, 64));
-            near_leaf<MIXED, UNI>(a, L, S, lane, st, ax, ay);
+            near_leaf<MIXED, UNI>(a, L, node_of_lane(N, k), lane, st, ax, ay);
           }
         }
       }
@@ -1063,13 +1079,7 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
     while (m) {
       const int b = __builtin_ctzll(m);
       m &= m - 1;
-      NodeRec S;
-      S.cx = rfl64d(__shfl(C.cx, b, 64));
-      S.cy = rfl64d(__shfl(C.cy, b, 64));
-      S.region = rfl64d(__shfl(C.region, b, 64));
-      S.first = rfl(__shfl(C.first, b, 64));
-      S.cnt = rfl(__shfl(C.cnt, b, 64));
-      if (!item(rfl(__shfl(c, b, 64)), S, rfl(__shfl(kind, b, 64)))) return;
+      if (!item(lane_of(c, b), node_of_lane(C, b), lane_of(kind, b))) return;
     }
   }
 }
@@ -1234,12 +1244,15 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
         qv = cmul_fma(ak, wi);                                      // a_i w^i
         if ((mi & 1) == 0) qv = make_double2(-qv.x, -qv.y);         // (-1)^(i+1)
       }
+      // operands rounded to the MFMA type before they cross lanes
+      typedef typename FarAcc<MIXED>::T FT;
+      const FT pwx = (FT)pw.x, pwy = (FT)pw.y, qvx = (FT)qv.x, qvy = (FT)qv.y;
       const bool im = (mk & 1) != 0;
       for (int st = 0; st < 2; st++) {
         if (4 * q + 2 * st >= nfar) continue;  // wave-uniform
         const int src = mi + 16 * (2 * st + (mk >> 1));
-        const double px = __shfl(pw.x, src, 64), py = __shfl(pw.y, src, 64);
-        const double qx = __shfl(qv.x, src, 64), qy = __shfl(qv.y, src, 64);
+        const FT px = __shfl(pwx, src, 64), py = __shfl(pwy, src, 64);
+        const FT qx = __shfl(qvx, src, 64), qy = __shfl(qvy, src, 64);
         acc.add(im ? py : px, im ? -qy : qx, im ? qx : qy);
         acc.tick();
       }
