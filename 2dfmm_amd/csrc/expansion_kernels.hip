@@ -232,8 +232,18 @@ int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int npare
 // The coefficients are stored for the scaled coordinates of the evaluation,
 // Z = lam Z' with lam = sqrt(ln 2) (Z' in units of sigma / sqrt(log2 e), where
 // e^{-|Z|^2} = 2^{-|Z'|^2}), and divided by sigma: nu_m lam^(m+1) / sigma for
-// m = q .. 0, then c_ij lam^(i+j) / sigma in Horner order.
+// m = q .. 0, then P in real monomials, P = sum_{a+b<=q-1} d_ab x^a y^b
+// (z = x + i y), so that Horner steps multiply by real x and y (half the
+// work of complex steps):
+//   d_ab = i^b sum_{i+j=a+b} c_ij sum_{k+l=b} (-1)^l C(i,k) C(j,l)
+// from (x + iy)^i (x - iy)^j, stored as d_ab lam^(a+b) / sigma in Horner
+// order (b = q-1 .. 0, a = q-1-b .. 0).
 __device__ inline int smooth_tri(int m) { return m * (kSmoothQ + 1) - m * (m - 1) / 2; }
+// storage slot of term (i, j) of the n >= 1 part
+__device__ inline int smooth_slot(int i, int j) {
+  constexpr int Q = kSmoothQ;
+  return Q + 1 + (Q - 1 - j) * (Q - j) / 2 + (Q - 1 - j - i);
+}
 
 // One wave per node.  The node's blobs go through in chunks of 64: lane j
 // forms cir_j d_j^m and conj(d_j)^n (m, n <= q) of blob j in LDS, then lane
@@ -250,6 +260,7 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
   constexpr int Q = kSmoothQ;
   constexpr int NT = (Q + 1) * (Q + 2) / 2;
   __shared__ double2 mu[NT];
+  __shared__ double2 cz[kSmoothC];   // c_ij lam^(i+j) in storage order
   __shared__ double2 pm[Q + 1][64];  // cir d^m
   __shared__ double2 pn[Q + 1][64];  // conj(d)^n
   const int node = blockIdx.x;
@@ -300,9 +311,10 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
     mu[lane] = make_double2(ar * s, ai * s);
   }
   __syncthreads();
-  if (lane <= 2 * (kSmoothF4 - 1) + 1) {
+  double vr = 0.0, vi = 0.0;
+  int ti = 0, tj = 0;  // (i, j) of the n >= 1 term of this lane
+  {
     const double lam = 0.83255461115769775635;  // sqrt(ln 2)
-    double vr = 0.0, vi = 0.0;
     if (lane <= Q) {
       const int mm = Q - lane;
       const double2 v = mu[smooth_tri(mm)];
@@ -326,10 +338,37 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
       const double sc = pow(lam, (double)(i + j));
       vr *= sc;
       vi *= sc;
+      ti = i;
+      tj = j;
     }
+  }
+  if (lane < kSmoothC) cz[lane] = make_double2(vr, vi);
+  __syncthreads();
+  if (lane > Q && lane < kSmoothC) {
+    // d_ab (a = ti, b = tj): the terms c_ij with i + j = a + b
+    const int a = ti, b = tj, n = a + b;
+    double sr = 0.0, si = 0.0;
+    for (int i = 0; i <= n; i++) {
+      const int j = n - i;
+      double w = 0.0;
+      for (int l = max(0, b - i); l <= min(j, b); l++) {
+        const double t = fact[i] / (fact[b - l] * fact[i - b + l]) * (fact[j] / (fact[l] * fact[j - l]));
+        w += (l & 1) ? -t : t;
+      }
+      const double2 v = cz[smooth_slot(i, j)];
+      sr += w * v.x;
+      si += w * v.y;
+    }
+    switch (b & 3) {  // times i^b
+      case 0: vr = sr; vi = si; break;
+      case 1: vr = -si; vi = sr; break;
+      case 2: vr = -sr; vi = -si; break;
+      default: vr = si; vi = -sr; break;
+    }
+  }
+  if (lane <= 2 * (kSmoothF4 - 1) + 1)
     reinterpret_cast<float2 *>(out + (size_t)node * kSmoothF4)[lane] =
         make_float2((float)(vr * inv_sigma), (float)(vi * inv_sigma));
-  }
 }
 
 int launch_smooth_coef(const NodeRec *d_nodes, int nnodes, const int32_t *d_nfirst,

@@ -93,8 +93,8 @@ int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int npare
 // expansion of the regularised kernel f(z) = (1 - e^{-|z|^2}) / z (z in core
 // radii) about its centre, of total degree kSmoothQ in (d, conj d).  Per
 // cell kSmoothC complex FP32 coefficients: nu_q .. nu_0 (the n = 0 column),
-// then c_ij of the n >= 1 part in Horner order (j = q-1 .. 0, i = q-1-j .. 0);
-// see smooth_coef_kernel.  Coordinates of the evaluation are the FP32
+// then d_ab of the n >= 1 part in real monomials x^a y^b, in Horner order
+// (b = q-1 .. 0, a = q-1-b .. 0); see smooth_coef_kernel.  Coordinates of the evaluation are the FP32
 // leaf-relative ones scaled by sw = sqrt(log2 e) / sigma.
 constexpr int kSmoothQ = 8;
 constexpr int kSmoothC = (kSmoothQ + 1) + kSmoothQ * (kSmoothQ + 1) / 2;

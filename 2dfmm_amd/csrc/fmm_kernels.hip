@@ -789,7 +789,7 @@ constexpr SmoothSeries kSmoothSer = make_smooth_series();
 // One smooth cell at one target, FP32: Z' = (zx, zy) is the target relative
 // to the cell centre in the scaled coordinates (sigma / sqrt(log2 e) units),
 // row[] the cell's coefficients.  With w = ln2 |Z'|^2 and E = e^-w = 2^-|Z'|^2,
-//   F = zb sum_m phi^(m)(w) nu_m zb^m + E sum_j zb^j sum_i c_ij z^i    (Horner),
+//   F = zb sum_m phi^(m)(w) nu_m zb^m + E sum_b y^b sum_a d_ab x^a    (Horner),
 // phi^(m)(w) = (-1)^m int_0^1 t^m e^{-wt} dt from w phi^(m) + m phi^(m-1) =
 // (-1)^(m+1) E: upward from phi = (1 - E)/w for w >= kSmoothW0, downward from
 // the series of phi^(q) below it (each direction only where it is stable).
@@ -831,25 +831,22 @@ __device__ inline void smooth_eval(float zx, float zy, const float4 *row, float 
   }
   float s0r = fmaf(ar, zx, ai * zy), s0i = fmaf(ai, zx, -ar * zy);
   asm volatile("" : "+v"(s0r), "+v"(s0i) : : "memory");
-  // n >= 1: P = sum_j zb^j Q_j(z), Q_j(z) = sum_i c_ij z^i
+  // n >= 1: P = sum_b y^b R_b(x), R_b(x) = sum_a d_ab x^a (real monomials:
+  // each Horner step multiplies a complex value by a real coordinate)
   float pr = 0.f, pi = 0.f;
   int k = Q + 1;
 #pragma unroll
-  for (int j = Q - 1; j >= 0; j--) {
+  for (int b = Q - 1; b >= 0; b--) {
     c = co(k++);
     float qr = c.x, qi = c.y;
 #pragma unroll
-    for (int i = Q - 2 - j; i >= 0; i--) {
+    for (int i = Q - 2 - b; i >= 0; i--) {
       c = co(k++);
-      const float tr = fmaf(qr, zx, fmaf(-qi, zy, c.x));
-      const float ti = fmaf(qr, zy, fmaf(qi, zx, c.y));
-      qr = tr;
-      qi = ti;
+      qr = fmaf(qr, zx, c.x);
+      qi = fmaf(qi, zx, c.y);
     }
-    const float tr = fmaf(pr, zx, fmaf(pi, zy, qr));
-    const float ti = fmaf(pi, zx, fmaf(-pr, zy, qi));
-    pr = tr;
-    pi = ti;
+    pr = fmaf(pr, zy, qr);
+    pi = fmaf(pi, zy, qi);
     // the next row's coefficient reads stay after this row's arithmetic (all
     // rows' reads at once would hold ~90 registers): an empty asm statement
     // that takes the running sum and orders memory

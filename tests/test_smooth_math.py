@@ -4,8 +4,9 @@ The library sums a cell that is small against the core radius through a
 Taylor expansion of f(z) = (1 - exp(-|z|^2)) / z in (z, conj z) about the cell
 centre (core-radius units).  Its closed forms -- T_m0 = zb^(m+1) phi^(m)(w),
 T_mn = E H_{m,n-1} with the Hermite-type polynomials H, the regrouping of the
-n >= 1 part into one polynomial P(z, zb), and phi^(m) by the upward recurrence
-or the series -- are restated here and compared with brute-force sums of the
+n >= 1 part into one polynomial P(z, zb), its coefficients in real monomials
+x^a y^b (what the kernel stores), and phi^(m) by the upward recurrence or the
+series -- are restated here and compared with brute-force sums of the
 exact kernel (not with themselves), so a wrong sign, index or factorial in any
 of them fails."""
 import math
@@ -150,3 +151,37 @@ def test_scaled_coordinates():
         P = sum(v * Zp ** i * zb ** j for (i, j), v in cs.items())
         F = s0 + 2.0 ** (-wp) * P
         assert abs(F - evaluate(Z, nu, c)) <= 1e-12 * np.sum(np.abs(g))
+
+
+def real_basis(c, q=Q):
+    """d_ab = i^b sum_{i+j=a+b} c_ij sum_{k+l=b} (-1)^l C(i,k) C(j,l): P(z, zb)
+    rewritten as sum d_ab x^a y^b with z = x + iy."""
+    d = {}
+    for b in range(q):
+        for a in range(q - b):
+            n = a + b
+            v = 0j
+            for i in range(n + 1):
+                j = n - i
+                w = sum((-1) ** l * math.comb(i, b - l) * math.comb(j, l)
+                        for l in range(max(0, b - i), min(j, b) + 1))
+                v += w * c[(i, j)]
+            d[(a, b)] = v * 1j ** b
+    return d
+
+
+def test_real_monomial_basis():
+    """The stored d_ab give the same P as the c_ij at any point (a wrong sign,
+    power of i or binomial changes P)."""
+    rng = np.random.default_rng(5)
+    c = {(i, j): complex(*rng.normal(size=2)) for j in range(Q) for i in range(Q - j)}
+    d = real_basis(c)
+    for _ in range(20):
+        Z = complex(*rng.uniform(-2.5, 2.5, size=2))
+        p_z = sum(v * Z ** i * Z.conjugate() ** j for (i, j), v in c.items())
+        p_xy = sum(v * Z.real ** a * Z.imag ** b for (a, b), v in d.items())
+        assert abs(p_z - p_xy) <= 1e-12 * max(1.0, abs(p_z)) * 10 ** 3, (Z, p_z, p_xy)
+    # one monomial alone: z^2 zb = (x^2 + y^2)(x + iy) -> x^3, x y^2, i x^2 y, i y^3
+    one = {k: (1.0 if k == (2, 1) else 0.0) for k in c}
+    d1 = {k: v for k, v in real_basis(one).items() if abs(v) > 0}
+    assert d1 == {(3, 0): 1, (1, 2): 1, (2, 1): 1j, (0, 3): 1j}, d1
