@@ -1206,40 +1206,40 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   auto far_flush = [&]() {
     FarAcc<MIXED> acc;
     const int nq = (nfar + 3) >> 2;
-    double dr = 0.0, di = 0.0;
-    double2 ak = make_double2(0.0, 0.0);
-    bool ok = false;
-    auto fetch = [&](int q, double &r, double &i, double2 &k, bool &v) {
+    // centre and coefficient of the quad's cell, loaded a quad ahead; a lane
+    // without a cell (past the list, or row i >= p) loads node 0 and its
+    // result is dropped, so the loads and the arithmetic take no branches
+    double2 cc, ak;
+    bool ok;
+    auto fetch = [&](int q, double2 &c2, double2 &k, bool &v) {
       const int idx = 4 * q + mk;
       const int cell = idx < nfar ? s_far[w][idx] : -1;
       v = cell >= 0 && row_ok;
-      if (v) {
-        const NodeRec C = a.nodes[cell];
-        r = C.cx - NA.cx;
-        i = C.cy - NA.cy;
-        k = a.coef[(size_t)cell * a.p + mi];
-      }
+      const int cs = v ? cell : 0;
+      c2 = make_double2(a.nodes[cs].cx, a.nodes[cs].cy);
+      k = a.coef[(size_t)cs * a.p + (v ? mi : 0)];
     };
-    fetch(0, dr, di, ak, ok);
+    fetch(0, cc, ak, ok);
     for (int q = 0; q < nq; q++) {
-      double ndr = 0.0, ndi = 0.0;
-      double2 nak = make_double2(0.0, 0.0);
+      double2 ncc = cc, nak = ak;
       bool nok = false;
-      if (q + 1 < nq) fetch(q + 1, ndr, ndi, nak, nok);
-      double2 pw = make_double2(0.0, 0.0), qv = make_double2(0.0, 0.0);
-      if (ok) {
-        // w = 1 / (dr + i di); 1 / |d|^2 from the hardware reciprocal and
-        // two Newton steps
-        const double r2 = fma(dr, dr, di * di);
-        double inv = __builtin_amdgcn_rcp(r2);
-        inv = inv * fma(-r2, inv, 2.0);
-        inv = inv * fma(-r2, inv, 2.0);
-        const double2 wv = make_double2(dr * inv, -di * inv);
-        const double2 wi = cpow16(wv, mi);                          // w^i
-        const double2 wi1 = cmul_fma(wi, wv);
-        pw = make_double2(wi1.x * sApow.x, wi1.y * sApow.x);        // (s_A w)^(i+1)
-        qv = cmul_fma(ak, wi);                                      // a_i w^i
-        if ((mi & 1) == 0) qv = make_double2(-qv.x, -qv.y);         // (-1)^(i+1)
+      if (q + 1 < nq) fetch(q + 1, ncc, nak, nok);
+      // w = 1 / (dr + i di); 1 / |d|^2 from the hardware reciprocal and two
+      // Newton steps
+      const double dr = cc.x - NA.cx, di = cc.y - NA.cy;
+      const double r2 = fma(dr, dr, di * di);
+      double inv = __builtin_amdgcn_rcp(r2);
+      inv = inv * fma(-r2, inv, 2.0);
+      inv = inv * fma(-r2, inv, 2.0);
+      const double2 wv = make_double2(dr * inv, -di * inv);
+      const double2 wi = cpow16(wv, mi);                          // w^i
+      const double2 wi1 = cmul_fma(wi, wv);
+      double2 pw = make_double2(wi1.x * sApow.x, wi1.y * sApow.x);  // (s_A w)^(i+1)
+      double2 qv = cmul_fma(ak, wi);                                // a_i w^i
+      if ((mi & 1) == 0) qv = make_double2(-qv.x, -qv.y);           // (-1)^(i+1)
+      if (!ok) {
+        pw = make_double2(0.0, 0.0);
+        qv = make_double2(0.0, 0.0);
       }
       // operands rounded to the MFMA type before they cross lanes
       typedef typename FarAcc<MIXED>::T FT;
@@ -1253,8 +1253,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
         acc.add(im ? py : px, im ? -qy : qx, im ? qx : qy);
         acc.tick();
       }
-      dr = ndr;
-      di = ndi;
+      cc = ncc;
       ak = nak;
       ok = nok;
     }
