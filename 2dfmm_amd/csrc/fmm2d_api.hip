@@ -234,6 +234,11 @@ static int resolve_params() {
 
 static int apply_learned();
 
+// Tree levels the first key sort orders (two key bits per level, below the
+// system bits): most trees are shallower, and a deeper one is sorted again on
+// every bit.
+constexpr int kSortLevels = 16;
+
 // The tree build of make_tree on device inputs (caller order).  With nsys > 1
 // the inputs are nsys independent systems, system s owning blobs
 // [h_off[s], h_off[s+1]); each gets its own tree (a forest) in the same pass.
@@ -279,70 +284,84 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   if (g.nodes.ensure_keep(est, 0, s) || g.gparent.ensure_keep(est / 4 + 1, 0, s) ||
       g.split.ensure_keep(est / 4 + nsys + 1, 0, s))
     return -1;
-  mark(0);
-  int nroots = 1;
-  if (nsys == 1) {
-    if (launch_bbox(dxy, n, g.root.p, g.bbox_part.p, s)) return -1;
-    if (launch_keygen(dxy, n, g.root.p, D, g.key.p, g.idx.p, s)) return -1;
-    if (launch_init_root(g.nodes.p, g.gparent.p, g.root.p, n, g.split.p, s)) return -1;
-  } else {
-    g.root_shift = 2 * D;
-    if (g.doff.ensure(nsys + 1, s)) return -1;
-    FMM_HIP_TRY(hipMemcpyAsync(g.doff.p, h_off, sizeof(int32_t) * (nsys + 1),
-                               hipMemcpyHostToDevice, s));
-    if (launch_forest_roots(dxy, n, g.doff.p, nsys, D, g.root.p, g.key.p, g.idx.p, g.nodes.p,
-                            g.gparent.p, nroot_slots, s))
-      return -1;
-    // level 0 split list: the non-empty systems
-    g.h_roots.clear();
-    for (int k = 0; k < nsys; k++)
-      if (h_off[k + 1] > h_off[k]) g.h_roots.push_back(k);
-    nroots = (int)g.h_roots.size();
-    if (nroots > 0)
-      FMM_HIP_TRY(hipMemcpyAsync(g.split.p, g.h_roots.data(), sizeof(int32_t) * nroots,
+  int nroots = 1, child_base = 0;
+  for (int full = 0;; full = 1) {
+    mark(0);
+    if (nsys == 1) {
+      if (launch_bbox(dxy, n, g.root.p, g.bbox_part.p, s)) return -1;
+      if (launch_keygen(dxy, n, g.root.p, D, g.key.p, g.idx.p, s)) return -1;
+      if (launch_init_root(g.nodes.p, g.gparent.p, g.root.p, n, g.split.p, s)) return -1;
+    } else {
+      g.root_shift = 2 * D;
+      if (g.doff.ensure(nsys + 1, s)) return -1;
+      FMM_HIP_TRY(hipMemcpyAsync(g.doff.p, h_off, sizeof(int32_t) * (nsys + 1),
                                  hipMemcpyHostToDevice, s));
-  }
-  mark(1);
-  if (radix_sort_pairs(&g.key.p, &g.idx.p, &g.key_alt.p, &g.idx_alt.p, n, 2 * D + sysbits,
-                       g.hist.p, g.hist_scan.p, s))
-    return -1;
-  mark(2);
-  // (the sort may swap key/key_alt and idx/idx_alt; each pair is always sized
-  // together, so their capacities are equal and need no fix-up)
-  // level-by-level build
-  g.split_off.assign({0, nroots});
-  int child_base = nroot_slots, nsplit_prev = nroots, level = 1;
-  while (nsplit_prev > 0) {
-    if (level > D) {
-      set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D, g.t_num_p);
-      return -1;
+      if (launch_forest_roots(dxy, n, g.doff.p, nsys, D, g.root.p, g.key.p, g.idx.p, g.nodes.p,
+                              g.gparent.p, nroot_slots, s))
+        return -1;
+      // level 0 split list: the non-empty systems
+      g.h_roots.clear();
+      for (int k = 0; k < nsys; k++)
+        if (h_off[k + 1] > h_off[k]) g.h_roots.push_back(k);
+      nroots = (int)g.h_roots.size();
+      if (nroots > 0)
+        FMM_HIP_TRY(hipMemcpyAsync(g.split.p, g.h_roots.data(), sizeof(int32_t) * nroots,
+                                   hipMemcpyHostToDevice, s));
     }
-    const int nchild = 4 * nsplit_prev;
-    if (g.nodes.ensure_keep((size_t)child_base + nchild, child_base, s) ||
-        g.gparent.ensure_keep(((size_t)child_base + nchild) / 4, child_base / 4, s) ||
-        g.flag.ensure(nchild, s) || g.pos.ensure(nchild, s))
+    mark(1);
+    // the top sysbits + 2 kSortLevels key bits first: enough for a tree of
+    // up to kSortLevels levels; a deeper tree is sorted again on every bit
+    const int kbits = 2 * D + sysbits;
+    const int lo_bit = full ? 0 : std::max(0, kbits - (sysbits + 2 * kSortLevels));
+    const int lsort = lo_bit == 0 ? D : (kbits - lo_bit - sysbits) / 2;
+    if (radix_sort_pairs(&g.key.p, &g.idx.p, &g.key_alt.p, &g.idx_alt.p, n, lo_bit, kbits,
+                         g.hist.p, g.hist_scan.p, s))
       return -1;
-    if (launch_build_level(g.key.p, 2 * D, level, g.split.p + g.split_off[level - 1], nsplit_prev,
-                           child_base, g.t_num_p, g.nodes.p, g.gparent.p, g.flag.p, s))
-      return -1;
-    if (scan_u32(g.flag.p, g.pos.p, nchild, g.total.p, s)) return -1;
-    FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    FMM_HIP_TRY(hipStreamSynchronize(s));
-    const int nsplit = (int)g.h_total[0];
-    if (nsplit > 0 && level == D) {
-      set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D, g.t_num_p);
-      return -1;
+    mark(2);
+    // (the sort may swap key/key_alt and idx/idx_alt; each pair is always
+    // sized together, so their capacities are equal and need no fix-up)
+    // level-by-level build
+    g.split_off.assign({0, nroots});
+    child_base = nroot_slots;
+    int nsplit_prev = nroots, level = 1;
+    while (nsplit_prev > 0) {
+      if (level > D) {
+        set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D,
+                  g.t_num_p);
+        return -1;
+      }
+      if (level > lsort) break;  // deeper than the sorted bits reach
+      const int nchild = 4 * nsplit_prev;
+      if (g.nodes.ensure_keep((size_t)child_base + nchild, child_base, s) ||
+          g.gparent.ensure_keep(((size_t)child_base + nchild) / 4, child_base / 4, s) ||
+          g.flag.ensure(nchild, s) || g.pos.ensure(nchild, s))
+        return -1;
+      if (launch_build_level(g.key.p, 2 * D, level, g.split.p + g.split_off[level - 1],
+                             nsplit_prev, child_base, g.t_num_p, g.nodes.p, g.gparent.p, g.flag.p,
+                             s))
+        return -1;
+      if (scan_u32(g.flag.p, g.pos.p, nchild, g.total.p, s)) return -1;
+      FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 s));
+      FMM_HIP_TRY(hipStreamSynchronize(s));
+      const int nsplit = (int)g.h_total[0];
+      if (nsplit > 0 && level == D) {
+        set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D,
+                  g.t_num_p);
+        return -1;
+      }
+      const int off = g.split_off[level];
+      if (g.split.ensure_keep((size_t)off + nsplit + 1, off, s)) return -1;
+      if (nsplit > 0 &&
+          launch_compact_split(g.flag.p, g.pos.p, nchild, child_base, g.split.p + off, s))
+        return -1;
+      g.split_off.push_back(off + nsplit);
+      child_base += nchild;
+      g.depth = level;
+      level++;
+      nsplit_prev = nsplit;
     }
-    const int off = g.split_off[level];
-    if (g.split.ensure_keep((size_t)off + nsplit + 1, off, s)) return -1;
-    if (nsplit > 0 &&
-        launch_compact_split(g.flag.p, g.pos.p, nchild, child_base, g.split.p + off, s))
-      return -1;
-    g.split_off.push_back(off + nsplit);
-    child_base += nchild;
-    g.depth = level;
-    level++;
-    nsplit_prev = nsplit;
+    if (nsplit_prev == 0) break;
   }
   g.nnodes = child_base;
   g.nleaves = (child_base - nroot_slots + nroots) - g.split_off.back();

@@ -59,10 +59,13 @@ int scan_u32(const uint32_t *d_in, uint32_t *d_out, int n, uint32_t *d_total, hi
 int launch_bbox(const double *d_xy, int n, RootBox *d_root, double *d_partial, hipStream_t s);
 int launch_keygen(const double *d_xy, int n, const RootBox *d_root, int depth, uint64_t *d_key,
                   uint32_t *d_idx, hipStream_t s);
-// Stable LSD radix sort of (key, idx) pairs on the low `bits` bits.  Uses the
-// alternate buffers; the sorted result ends in (*key, *idx) (pointers swapped).
+// Stable LSD radix sort of (key, idx) pairs on bits [lo_bit, bits) (8-bit
+// digits from lo_bit up; pairs equal on those bits keep their input order).
+// Uses the alternate buffers; the sorted result ends in (*key, *idx)
+// (pointers swapped).
 int radix_sort_pairs(uint64_t **key, uint32_t **idx, uint64_t **key_alt, uint32_t **idx_alt,
-                     int n, int bits, uint32_t *d_hist, uint32_t *d_hist_scan, hipStream_t s);
+                     int n, int lo_bit, int bits, uint32_t *d_hist, uint32_t *d_hist_scan,
+                     hipStream_t s);
 size_t radix_sort_hist_words(int n);
 // Forest of nsys systems (blobs [off[s], off[s+1]) on the device): per-system
 // root squares, keys (s << 2*depth) | path, root nodes 0..nsys-1 and padding

@@ -476,10 +476,11 @@ size_t radix_sort_hist_words(int n) {
 }
 
 int radix_sort_pairs(uint64_t **key, uint32_t **idx, uint64_t **key_alt, uint32_t **idx_alt, int n,
-                     int bits, uint32_t *d_hist, uint32_t *d_hist_scan, hipStream_t s) {
+                     int lo_bit, int bits, uint32_t *d_hist, uint32_t *d_hist_scan,
+                     hipStream_t s) {
   if (n <= 1) return 0;
   const int ntiles = (n + kRsTile - 1) / kRsTile;
-  for (int shift = 0; shift < bits; shift += 8) {
+  for (int shift = lo_bit; shift < bits; shift += 8) {
     rs_hist_kernel<<<ntiles, 256, 0, s>>>(*key, n, shift, d_hist, ntiles);
     FMM_HIP_TRY(hipGetLastError());
     if (scan_u32(d_hist, d_hist_scan, 256 * ntiles, nullptr, s) != 0) return -1;

@@ -89,6 +89,7 @@ CASES = [
     ("mixture", 20000, 16, 32, 16, 0.3),
     ("uniform", 3000, 17, 1, 5, 0.5),
     ("sheet", 8000, 18, 20, 200, 0.6),
+    ("deep", 3200, 31, 8, 5, 0.5),
 ]
 
 
@@ -98,6 +99,13 @@ def _inputs(kind, n, seed):
         return wl.uniform_square(n, seed=seed)
     if kind == "sheet":
         return wl.sheet(n, seed=seed)
+    if kind == "deep":
+        # a knot of 200 blobs of side 2e-6 in a uniform square: 23 levels,
+        # deeper than the first key sort orders
+        xy, cir, cor = wl.uniform_square(n - 200, seed=seed)
+        knot = np.array([0.3141, 0.2718]) + np.random.default_rng(seed).uniform(-1e-6, 1e-6, (200, 2))
+        return (np.concatenate([xy, knot]), np.concatenate([cir, cir[:200]]),
+                np.concatenate([cor, cor[:200]]))
     return wl.gaussian_mixture(n, seed=seed)
 
 
