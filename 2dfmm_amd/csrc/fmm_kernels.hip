@@ -573,14 +573,16 @@ __device__ inline void lds_sum64(double tx, double ty, const double4 *row, int c
 }
 
 // All sources of a near leaf for one target, read at wave-uniform indices
-// (scalar loads), groups of four pairs summed in FP32, accumulated in FP64.
+// (scalar loads, four at a time), groups of eight pairs summed in FP32,
+// accumulated in FP64.
 template <bool SERIES, bool UNI>
 __device__ inline void scalar_sum32(float px, float py, const float4 *src, int cnt, double &ax,
                                     double &ay) {
   int j = 0;
   if (cnt >= 4) {
-    // the next group's loads are issued before this group's arithmetic
+    // the next four loads are issued before the arithmetic of these four
     float4 n0 = ldro(src), n1 = ldro(src + 1), n2 = ldro(src + 2), n3 = ldro(src + 3);
+    float gx = 0.f, gy = 0.f;
     for (; j + 3 < cnt; j += 4) {
       const float4 s0 = n0, s1 = n1, s2 = n2, s3 = n3;
       if (j + 7 < cnt) {
@@ -589,14 +591,18 @@ __device__ inline void scalar_sum32(float px, float py, const float4 *src, int c
         n2 = ldro(src + j + 6);
         n3 = ldro(src + j + 7);
       }
-      float gx = 0.f, gy = 0.f;
       pair32<SERIES, UNI>(px, py, s0, gx, gy);
       pair32<SERIES, UNI>(px, py, s1, gx, gy);
       pair32<SERIES, UNI>(px, py, s2, gx, gy);
       pair32<SERIES, UNI>(px, py, s3, gx, gy);
-      ax += (double)gx;
-      ay += (double)gy;
+      if (j & 4) {
+        ax += (double)gx;
+        ay += (double)gy;
+        gx = gy = 0.f;
+      }
     }
+    ax += (double)gx;
+    ay += (double)gy;
   }
   if (j < cnt) {
     float gx = 0.f, gy = 0.f;

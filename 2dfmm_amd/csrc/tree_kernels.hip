@@ -610,6 +610,15 @@ __global__ __launch_bounds__(256) void leaf_order_kernel(const NodeRec *__restri
   const NodeRec r = nodes[node];
   if (node_internal(r.cnt)) return;
   const int c = node_count(r.cnt), lo = r.first;
+  if (c <= 64) {
+    // one id per lane; ranks from the other lanes' ids read with v_readlane
+    const uint32_t my = lane < c ? idx[lo + lane] : 0xffffffffu;
+    int rank = 0;
+    for (int f = 0; f < c; f++)
+      rank += (uint32_t)__builtin_amdgcn_readlane((int)my, f) < my ? 1 : 0;
+    if (lane < c) perm[lo + rank] = my;
+    return;
+  }
   for (int e = lane; e < c; e += 64) {
     const uint32_t my = idx[lo + e];
     int rank = 0;
