@@ -451,9 +451,30 @@ constexpr int stage_slots() { return MIXED ? kStageSlots + kSmoothResSlots : kSt
 // Per-target-leaf values; lanes are (target ti, replica rep) pairs.
 // Source j of a staged step goes to LDS slot (j mod nrep) stride + j / nrep
 // (spos for j = lane), stride = ceil(64 / nrep).
+// Smooth cells are summed up to kSmoothBatch at a time (the stage holds
+// their coefficients).  A batch of nb cells is nb x cA evaluations, (target,
+// cell) pairs dealt to the 64 lanes in rounds; nb is chosen to fill the
+// rounds best, the larger on a tie.
+constexpr int kSmoothBatch = 5;
+__device__ inline int smooth_batch_size(int cA) {
+  int best = 1, bu = 0;  // bu: evaluations per 64 lane slots, x 1024
+  for (int nb = 1; nb <= kSmoothBatch; nb++) {
+    const int ne = nb * cA;
+    const int u = (ne * 1024) / (64 * ((ne + 63) >> 6));
+    if (u >= bu) {
+      bu = u;
+      best = nb;
+    }
+  }
+  return best;
+}
+
 struct LeafCtx {
   NodeRec NA;
   int cA, pA, nrep, ti, rep, tj, stride, spos;
+  int sbsz;       // smooth batch size
+  uint32_t rinv;  // ceil(2^20 / nrep): divisions by nrep as a multiply
+  int64_t su;     // work units of one smooth cell
   bool tact;
   double RA, sA, rcut, inv_theta, tx, ty;
   float lx, ly;
@@ -476,6 +497,10 @@ __device__ inline LeafCtx leaf_ctx(const FmmArgs &a, int li, int lane) {
     L.rcut = (double)lp.y * a.smax;
   }
   L.nrep = max(1, 64 / L.cA);
+  L.rinv = ((1u << 20) + (uint32_t)L.nrep - 1u) / (uint32_t)L.nrep;
+  L.sbsz = smooth_batch_size(L.cA);
+  // a cell's share of its batch's time: the batch's rounds over its cells
+  L.su = (kSmoothUnits * ((L.sbsz * L.cA + 63) >> 6) + L.sbsz - 1) / L.sbsz;
   L.ti = lane % L.cA;
   L.rep = lane / L.cA;
   L.stride = (64 + L.nrep - 1) / L.nrep;
@@ -496,8 +521,9 @@ __device__ inline LeafCtx leaf_ctx(const FmmArgs &a, int li, int lane) {
 
 // Work units of one near step over cnt sources: wave steps plus a fixed
 // cost for the step's loads and synchronisation.
-__device__ inline int64_t near_units(int cnt, int nrep) {
-  return (int64_t)((cnt + nrep - 1) / nrep) + kStepUnits;
+__device__ inline int64_t near_units(int cnt, const LeafCtx &L) {
+  // ceil(cnt / nrep), exact for cnt + nrep < 2^12
+  return (int64_t)(((uint64_t)(cnt + L.nrep - 1) * L.rinv) >> 20) + kStepUnits;
 }
 
 // One pair (target, source s) of the mixed form, target at (px, py) and
@@ -628,8 +654,9 @@ __device__ inline bool needs_series(const FmmArgs &a, const LeafCtx &L, const No
 // leaf's sources are staged once in this wave's LDS slice `st` and read from
 // there, so each source crosses the vector memory path once per wave.
 template <bool MIXED, bool UNI>
-__device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeRec &C, int lane,
-                                 typename StageOf<MIXED>::T *st, double &ax, double &ay) {
+__device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeRec &C, bool ser,
+                                 int lane, typename StageOf<MIXED>::T *st, double &ax,
+                                 double &ay) {
   const int cnt = node_count(C.cnt);
   const int f = rfl(C.first);
   const int nrep = L.nrep;
@@ -639,7 +666,6 @@ __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeR
     const float dx = UNI ? (float)((C.cx - L.NA.cx) * a.sw) : (float)(C.cx - L.NA.cx);
     const float dy = UNI ? (float)((C.cy - L.NA.cy) * a.sw) : (float)(C.cy - L.NA.cy);
     const float px = L.lx - dx, py = L.ly - dy;
-    const bool ser = needs_series(a, L, C);
     if (nrep == 1) {
       if (ser)
         scalar_sum32<true, UNI>(px, py, a.loc + f, cnt, ax, ay);
@@ -687,11 +713,10 @@ __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeR
 // next tile's loads are issued before the current tile's sums.
 template <bool MIXED, bool UNI>
 __device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
-                                  int ta, int tb, int lane, typename StageOf<MIXED>::T *st,
-                                  double &ax, double &ay) {
+                                  bool ser, int ta, int tb, int lane,
+                                  typename StageOf<MIXED>::T *st, double &ax, double &ay) {
   const int s0 = rfl(ldro(a.nfirst + c));
   const int cnt = node_count(C.cnt);
-  const bool ser = MIXED && needs_series(a, L, C);
   double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
   float2 zw = make_float2(0.f, 0.f);
   auto load = [&](int t) {
@@ -862,35 +887,11 @@ __device__ inline void smooth_eval(float zx, float zy, const float4 *row, float 
   gy += fmaf(E, pr, s0r);
 }
 
-// Smooth cells are summed up to kSmoothBatch at a time (the stage holds
-// their coefficients).  Work units of one evaluation: kSmoothUnits (about its
-// cost in pair steps).
-constexpr int kSmoothBatch = 5;
+// Work units of one smooth evaluation: kSmoothUnits (about its cost in pair
+// steps).
 constexpr int kSmoothRow = kSmoothF4 + 1;  // coefficients + the cell's offset
 static_assert(kSmoothBatch * kSmoothRow <= kStageSlots, "smooth batch exceeds the stage");
 static_assert((kSmoothBatch + 1) * 64 * 8 <= kSmoothResSlots * 16, "smooth values exceed the stage");
-// A batch of nb cells is nb x cA evaluations, (target, cell) pairs dealt to
-// the 64 lanes in rounds; nb (at most the kSmoothBatch the stage holds) is
-// chosen to fill the rounds best, the larger on a tie.
-__device__ inline int smooth_batch_size(const LeafCtx &L) {
-  int best = 1, bu = 0;  // bu: evaluations per 64 lane slots, x 1024
-  for (int nb = 1; nb <= kSmoothBatch; nb++) {
-    const int ne = nb * L.cA;
-    const int u = (ne * 1024) / (64 * ((ne + 63) >> 6));
-    if (u >= bu) {
-      bu = u;
-      best = nb;
-    }
-  }
-  return best;
-}
-// a cell's share of the batch's time: the batch's rounds over its cells
-__device__ inline int64_t smooth_units(const LeafCtx &L) {
-  const int nb = smooth_batch_size(L);
-  const int rounds = (nb * L.cA + 63) >> 6;
-  return (kSmoothUnits * rounds + nb - 1) / nb;
-}
-
 // Node k (lane k) of tile t of smooth run c: its record (cnt 0 past the end).
 __device__ inline NodeRec srun_node(const FmmArgs &a, int c, int t, int lane, int &idx) {
   const int2 r = ldro(a.srange + c);
@@ -930,7 +931,7 @@ __device__ inline void smooth_batch(const FmmArgs &a, const LeafCtx &L, int cell
   if (L.rep == 0) tl[L.ti] = make_float2(L.lx, L.ly);
   wave_sync();
   const int cA = L.cA, ne = nb * cA;
-  const int r64 = 64 % cA, q64 = 64 / cA;
+  const int q64 = L.nrep, r64 = 64 - q64 * cA;  // 64 / cA, 64 % cA (cA <= 64)
   int t = L.ti, k = L.rep;
   for (int e = lane; e < ne; e += 64) {
     const float4 *row = st + k * kSmoothRow;
@@ -961,7 +962,7 @@ __device__ inline void smooth_batch(const FmmArgs &a, const LeafCtx &L, int cell
 // (the caller's batch), the small leaves of a smooth run are summed here.
 template <bool MIXED, bool UNI, typename Push>
 __device__ inline void item_steps(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
-                                  int kind, int ta, int tb, int lane,
+                                  int kind, bool ser, int ta, int tb, int lane,
                                   typename StageOf<MIXED>::T *st, double &ax, double &ay,
                                   Push &&push) {
   if constexpr (MIXED && UNI) {
@@ -976,6 +977,9 @@ __device__ inline void item_steps(const FmmArgs &a, const LeafCtx &L, int c, con
         const int cnt = node_count(N.cnt);
         uint64_t m = __builtin_amdgcn_ballot_w64(cnt > 0);
         const uint64_t sm = __builtin_amdgcn_ballot_w64(cnt >= a.smooth_min);
+        // the series test of each small leaf, one per lane
+        const uint64_t serm =
+            __builtin_amdgcn_ballot_w64(cnt > 0 && cnt < a.smooth_min && needs_series(a, L, N));
         while (m) {
           const int k = __builtin_ctzll(m);
           m &= m - 1;
@@ -983,26 +987,26 @@ __device__ inline void item_steps(const FmmArgs &a, const LeafCtx &L, int c, con
           if ((sm >> k) & 1) {
             push(node);
           } else {
-            near_leaf<MIXED, UNI>(a, L, node_of_lane(N, k), lane, st, ax, ay);
+            near_leaf<MIXED, UNI>(a, L, node_of_lane(N, k), (serm >> k) & 1, lane, st, ax, ay);
           }
         }
       }
       return;
     }
   }
-  near_steps<MIXED, UNI>(a, L, c, C, kind == kItemRange, ta, tb, lane, st, ax, ay);
+  near_steps<MIXED, UNI>(a, L, c, C, kind == kItemRange, ser, ta, tb, lane, st, ax, ay);
 }
 
 // Steps ta..tb-1 of near item c.
 template <bool MIXED, bool UNI>
 __device__ inline void near_steps(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
-                                  bool range, int ta, int tb, int lane,
+                                  bool range, bool ser, int ta, int tb, int lane,
                                   typename StageOf<MIXED>::T *st, double &ax,
                                   double &ay) {
   if (range)
-    near_range<MIXED, UNI>(a, L, c, C, ta, tb, lane, st, ax, ay);
+    near_range<MIXED, UNI>(a, L, c, C, ser, ta, tb, lane, st, ax, ay);
   else
-    near_leaf<MIXED, UNI>(a, L, C, lane, st, ax, ay);
+    near_leaf<MIXED, UNI>(a, L, C, ser, lane, st, ax, ay);
 }
 
 // The far test of a cell C for target leaf A: R_A + R_C <= theta d and
@@ -1070,6 +1074,9 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
     const uint64_t fm = __builtin_amdgcn_ballot_w64(f);
     const uint64_t im = __builtin_amdgcn_ballot_w64(it);
     const uint64_t om = __builtin_amdgcn_ballot_w64(op);
+    // the series test of each near leaf or range (mixed form), one per lane
+    const uint64_t sm = __builtin_amdgcn_ballot_w64(
+        it && (kind == kItemLeaf || kind == kItemRange) && needs_series(a, L, C));
     if (op) {
       const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(om >> 32),
                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)om, 0));
@@ -1082,7 +1089,7 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
     while (m) {
       const int b = __builtin_ctzll(m);
       m &= m - 1;
-      if (!item(lane_of(c, b), node_of_lane(C, b), lane_of(kind, b))) return;
+      if (!item(lane_of(c, b), node_of_lane(C, b), lane_of(kind, b), (sm >> b) & 1)) return;
     }
   }
 }
@@ -1093,11 +1100,11 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
 __device__ inline int64_t item_prefix(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
                                       int kind, int nt, int t) {
   if (t <= 0) return 0;
-  if (kind == kItemSmooth) return smooth_units(L);
-  if (kind == kItemLeaf) return near_units(node_count(C.cnt), L.nrep);
+  if (kind == kItemSmooth) return L.su;
+  if (kind == kItemLeaf) return near_units(node_count(C.cnt), L);
   if (kind == kItemRange) {
-    int64_t u = (int64_t)min(t, nt - 1) * near_units(64, L.nrep);
-    if (t >= nt) u += near_units(node_count(C.cnt) - 64 * (nt - 1), L.nrep);
+    int64_t u = (int64_t)min(t, nt - 1) * near_units(64, L);
+    if (t >= nt) u += near_units(node_count(C.cnt) - 64 * (nt - 1), L);
     return u;
   }
   const int2 r = ldro(a.srange + c);
@@ -1155,7 +1162,7 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t sta
   const int k1 = (int)(end >> 32), t1 = (int)(uint32_t)end;
   int k = 0;
   int scell = 0, snb = 0;
-  const int sbsz = smooth_batch_size(L);
+  const int sbsz = L.sbsz;
   auto push = [&](int cell) {
     if constexpr (SM) {
       if (lane == snb) scell = cell;
@@ -1165,14 +1172,15 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t sta
       }
     }
   };
-  walk_wide<SM>(a, L, lane, stack, [](uint64_t, int) {}, [&](int c, const NodeRec &C, int kind) -> bool {
+  walk_wide<SM>(a, L, lane, stack, [](uint64_t, int) {},
+                [&](int c, const NodeRec &C, int kind, bool ser) -> bool {
     const int ki = k++;
     if (ki < k0) return true;
     if (ki > k1 || (ki == k1 && t1 == 0)) return false;
     const int nt = item_tiles(a, c, C, kind);
     const int ta = ki == k0 ? t0 : 0;
     const int tb = ki == k1 ? min(t1, nt) : nt;
-    if (ta < tb) item_steps<MIXED, UNI>(a, L, c, C, kind, ta, tb, lane, st, ax, ay, push);
+    if (ta < tb) item_steps<MIXED, UNI>(a, L, c, C, kind, ser, ta, tb, lane, st, ax, ay, push);
     return ki < k1;
   });
   if constexpr (SM) {
@@ -1310,7 +1318,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   // pending smooth cells (lane k holds cell k of the batch); only the summing
   // part of the walk adds cells, so the last batch is summed after the walk
   int scell = 0, snb = 0;
-  const int sbsz = smooth_batch_size(L);
+  const int sbsz = L.sbsz;
   auto push = [&](int cell) {
     if constexpr (SM) {
       if (lane == snb) scell = cell;
@@ -1321,7 +1329,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
     }
   };
   int nitem = 0;
-  auto item = [&](int c, const NodeRec &C, int kind) -> bool {
+  auto item = [&](int c, const NodeRec &C, int kind, bool ser) -> bool {
     const int ki = nitem++;
     const int nt = item_tiles(a, c, C, kind);
     const int64_t c0 = cum;
@@ -1330,7 +1338,8 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
     if (nck == 0) {
       // the tiles that start below the first threshold are summed here
       t = c0 + total <= thr ? nt : item_find(a, L, c, C, kind, nt, c0, thr, 0);
-      if (t > 0) item_steps<MIXED, UNI>(a, L, c, C, kind, 0, t, lane, s_stage[w], ax, ay, push);
+      if (t > 0)
+        item_steps<MIXED, UNI>(a, L, c, C, kind, ser, 0, t, lane, s_stage[w], ax, ay, push);
     }
     // a chunk boundary at every tile whose work before it reaches thr
     while (t < nt && c0 + total > thr) {
