@@ -524,60 +524,79 @@ int launch_init_root(NodeRec *d_nodes, int32_t *d_gparent, const RootBox *d_root
   return 0;
 }
 
-__device__ inline int lower_bound_digit(const uint64_t *__restrict__ key, int lo, int hi, int shift,
-                                        uint32_t q) {
-  // first index in [lo, hi) whose digit >= q (digits are non-decreasing there)
-  while (lo < hi) {
-    const int mid = lo + ((hi - lo) >> 1);
-    if ((uint32_t)((key[mid] >> shift) & 3u) < q) lo = mid + 1; else hi = mid;
+// First index in [lo, hi) whose digit at `shift` is >= q (digits do not
+// decrease there), found by one wave: each step probes 64 evenly spaced keys
+// and keeps the gap where the digit first reaches q (about four dependent
+// loads for a million keys instead of twenty).
+__device__ inline int wave_lower_bound(const uint64_t *__restrict__ key, int lo, int hi, int shift,
+                                       uint32_t q, int lane) {
+  while (hi - lo > 64) {
+    const int pos = lo + (int)(((int64_t)(hi - lo) * (lane + 1)) / 65);
+    const bool ge = (uint32_t)((key[pos] >> shift) & 3u) >= q;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(ge);
+    if (m == 0) {
+      lo = __builtin_amdgcn_readlane(pos, 63) + 1;
+    } else {
+      const int f = __builtin_ctzll(m);
+      const int pf = __builtin_amdgcn_readlane(pos, f);
+      if (f > 0) lo = __builtin_amdgcn_readlane(pos, f - 1) + 1;
+      hi = pf;
+    }
   }
-  return lo;
+  const int i = lo + lane;
+  const uint64_t m =
+      __builtin_amdgcn_ballot_w64(i < hi && (uint32_t)((key[i] >> shift) & 3u) >= q);
+  return m ? lo + __builtin_ctzll(m) : hi;
 }
 
 // The range start of an internal node is kept in `first` until its own
-// children are built (then `first` becomes the child index).
+// children are built (then `first` becomes the child index).  One wave per
+// split node: its three digit boundaries by wave searches, lanes 0-3 write
+// the four children.
 __global__ __launch_bounds__(256) void build_level_kernel(const uint64_t *__restrict__ key,
                                                           int shift, const int32_t *__restrict__ split_prev,
                                                           int nsplit, int child_base, int num_p,
                                                           NodeRec *__restrict__ nodes,
                                                           int32_t *__restrict__ gparent,
                                                           uint32_t *__restrict__ flag) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (j >= nsplit) return;
   const int par = split_prev[j];
-  NodeRec P = nodes[par];
+  const NodeRec P = nodes[par];
   const int lo = P.first, hi = lo + node_count(P.cnt);
-  int b[5];
-  b[0] = lo;
-  b[4] = hi;
-  b[2] = lower_bound_digit(key, lo, hi, shift, 2u);
-  b[1] = lower_bound_digit(key, lo, b[2], shift, 1u);
-  b[3] = lower_bound_digit(key, b[2], hi, shift, 3u);
+  const int b2 = wave_lower_bound(key, lo, hi, shift, 2u, lane);
+  const int b1 = wave_lower_bound(key, lo, b2, shift, 1u, lane);
+  const int b3 = wave_lower_bound(key, b2, hi, shift, 3u, lane);
   const int cb = child_base + 4 * j;
-  const double h = 0.25 * P.region;
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
+  if (lane < 4) {
+    const int q = lane;
+    const int bq = q == 0 ? lo : q == 1 ? b1 : q == 2 ? b2 : b3;
+    const int be = q == 0 ? b1 : q == 1 ? b2 : q == 2 ? b3 : hi;
+    const double h = 0.25 * P.region;
     NodeRec c;
     // quadtree.c:61-79
     c.cx = (q & 1) ? P.cx + h : P.cx - h;
     c.cy = (q < 2) ? P.cy + h : P.cy - h;
     c.region = 0.5 * P.region;
-    const int cnt = b[q + 1] - b[q];
-    c.first = b[q];
+    const int cnt = be - bq;
+    c.first = bq;
     const bool split = cnt >= num_p;
     c.cnt = split ? (cnt | kInternalBit) : cnt;
     nodes[cb + q] = c;
     flag[4 * j + q] = split ? 1u : 0u;
   }
-  gparent[cb >> 2] = par;
-  nodes[par].first = cb;
+  if (lane == 0) {
+    gparent[cb >> 2] = par;
+    nodes[par].first = cb;
+  }
 }
 
 int launch_build_level(const uint64_t *d_key, int depth_bits, int level, const int32_t *d_split_prev,
                        int nsplit_prev, int child_base, int num_p, NodeRec *d_nodes,
                        int32_t *d_gparent, uint32_t *d_flag, hipStream_t s) {
   const int shift = depth_bits - 2 * level;
-  build_level_kernel<<<(nsplit_prev + 255) / 256, 256, 0, s>>>(
+  build_level_kernel<<<(nsplit_prev + 3) / 4, 256, 0, s>>>(
       d_key, shift, d_split_prev, nsplit_prev, child_base, num_p, d_nodes, d_gparent, d_flag);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
