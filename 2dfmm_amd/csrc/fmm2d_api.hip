@@ -366,7 +366,9 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   g.nnodes = child_base;
   g.nleaves = (child_base - nroot_slots + nroots) - g.split_off.back();
   if (launch_leaf_order(g.nodes.p, g.nnodes, g.idx.p, g.perm.p, g.t_num_p, s)) return -1;
-  if (launch_gather(g.perm.p, n, dxy, dcir, dcor, g.sx.p, g.sy.p, g.sg.p, g.ss.p, s)) return -1;
+  if (g.loc64.ensure(n, s) ||
+      launch_gather(g.perm.p, n, dxy, dcir, dcor, g.loc64.p, g.sx.p, g.sy.p, g.sg.p, g.ss.p, s))
+    return -1;
   mark(3);
   const size_t ncoef = (size_t)g.nnodes * g.t_nterms;
   if (g.coef.ensure(ncoef, s)) return -1;

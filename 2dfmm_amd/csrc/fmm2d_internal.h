@@ -82,8 +82,10 @@ int launch_compact_split(const uint32_t *d_flag, const uint32_t *d_pos, int nchi
                          int32_t *d_split_next, hipStream_t s);
 int launch_leaf_order(const NodeRec *d_nodes, int nnodes, const uint32_t *d_idx_sorted,
                       uint32_t *d_perm, int num_p, hipStream_t s);
+// Blobs into leaf order (d_rec: n records of scratch).
 int launch_gather(const uint32_t *d_perm, int n, const double *d_xy, const double *d_cir,
-                  const double *d_cor, double *sx, double *sy, double *sg, double *ss, hipStream_t s);
+                  const double *d_cor, double4 *d_rec, double *sx, double *sy, double *sg,
+                  double *ss, hipStream_t s);
 
 // ---- expansions (expansion_kernels.hip) ----------------------------------
 int launch_s2m(const NodeRec *d_nodes, int nnodes, const double *sx, const double *sy,

@@ -654,28 +654,40 @@ int launch_leaf_order(const NodeRec *d_nodes, int nnodes, const uint32_t *d_idx_
   return 0;
 }
 
+// (x, y, cir, cor) of every blob in one 32-byte record (caller order), so
+// the gather into leaf order reads one record per blob instead of three
+// scattered words.
+__global__ __launch_bounds__(256) void pack_kernel(const double2 *__restrict__ xy,
+                                                   const double *__restrict__ cir,
+                                                   const double *__restrict__ cor, int n,
+                                                   double4 *__restrict__ rec) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double2 q = xy[i];
+  rec[i] = make_double4(q.x, q.y, cir[i], cor[i]);
+}
+
 __global__ __launch_bounds__(256) void gather_kernel(const uint32_t *__restrict__ perm, int n,
-                                                     const double2 *__restrict__ xy,
-                                                     const double *__restrict__ cir,
-                                                     const double *__restrict__ cor,
+                                                     const double4 *__restrict__ rec,
                                                      double *__restrict__ sx, double *__restrict__ sy,
                                                      double *__restrict__ sg,
                                                      double *__restrict__ ss) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const uint32_t p = perm[i];
-  const double2 q = xy[p];
-  sx[i] = q.x;
-  sy[i] = q.y;
-  sg[i] = cir[p];
-  ss[i] = cor[p];
+  const double4 r = rec[perm[i]];
+  sx[i] = r.x;
+  sy[i] = r.y;
+  sg[i] = r.z;
+  ss[i] = r.w;
 }
 
 int launch_gather(const uint32_t *d_perm, int n, const double *d_xy, const double *d_cir,
-                  const double *d_cor, double *sx, double *sy, double *sg, double *ss,
-                  hipStream_t s) {
-  gather_kernel<<<(n + 255) / 256, 256, 0, s>>>(d_perm, n, reinterpret_cast<const double2 *>(d_xy),
-                                                d_cir, d_cor, sx, sy, sg, ss);
+                  const double *d_cor, double4 *d_rec, double *sx, double *sy, double *sg,
+                  double *ss, hipStream_t s) {
+  pack_kernel<<<(n + 255) / 256, 256, 0, s>>>(reinterpret_cast<const double2 *>(d_xy), d_cir,
+                                              d_cor, n, d_rec);
+  FMM_HIP_TRY(hipGetLastError());
+  gather_kernel<<<(n + 255) / 256, 256, 0, s>>>(d_perm, n, d_rec, sx, sy, sg, ss);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
