@@ -375,7 +375,7 @@ __device__ inline int root_of(const FmmArgs &a, int first) {
 // the checkpoints on other waves, and a last pass adds their partial sums in
 // chunk order (deterministic).
 // ---------------------------------------------------------------------------
-constexpr int kMaxCk = 64;     // chunk boundaries per leaf kept in LDS
+constexpr int kMaxCk = 128;    // chunk boundaries per leaf kept in LDS
 constexpr int kFarList = 128;  // far cells listed per burst
 
 __device__ inline void wave_sync() {
@@ -1193,7 +1193,7 @@ template <bool MIXED, bool UNI>
 __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a, int li, int w, int lane,
                                  double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
                                  typename StageOf<MIXED>::T (*s_stage)[stage_slots<MIXED>()],
-                                 int64_t (*s_ck)[kMaxCk], int64_t (*s_cku)[kMaxCk],
+                                 int64_t (*s_ck)[kMaxCk], int32_t (*s_cku)[kMaxCk],
                                  int (*s_stack)[kStack], int (*s_far)[kFarList]) {
   li = rfl(li);
   const LeafCtx L = leaf_ctx<MIXED, UNI>(a, li, lane);
@@ -1346,7 +1346,8 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
       t = item_find(a, L, c, C, kind, nt, c0, thr, t);
       if (t >= nt) break;
       if (nck == kMaxCk) {
-        int64_t keep = 0, keepu = 0;
+        int64_t keep = 0;
+        int32_t keepu = 0;
         if (lane < kMaxCk / 2) {
           keep = s_ck[w][2 * lane];
           keepu = s_cku[w][2 * lane];
@@ -1366,7 +1367,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
       const int64_t at = c0 + item_prefix(a, L, c, C, kind, nt, t);
       if (lane == 0) {
         s_ck[w][nck] = work_pos(ki, t);
-        s_cku[w][nck] = at;
+        s_cku[w][nck] = (int32_t)min(at, (int64_t)INT32_MAX);  // saturated: only a step base
       }
       nck++;
       thr = at + step;
@@ -1410,13 +1411,15 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
       if (lane == 0) base = atomicAdd(a.ck_count, (uint32_t)nck);
       base = (uint32_t)rfl((int)__shfl((int)base, 0, 64));
       const bool fits = (int64_t)base + nck <= (int64_t)a.ck_cap;
-      if (lane < nck && (int64_t)base + lane < (int64_t)a.ck_cap) {
-        SplitCk r;
-        r.cum = s_ck[w][lane];
-        r.end = lane + 1 < nck ? s_ck[w][lane + 1] : INT64_MAX;
-        r.li = fits ? li : -1;  // the continuation pass skips li < 0
-        r.pad = 0;
-        a.ck[base + lane] = r;
+      for (int k = lane; k < nck; k += 64) {
+        if ((int64_t)base + k < (int64_t)a.ck_cap) {
+          SplitCk r;
+          r.cum = s_ck[w][k];
+          r.end = k + 1 < nck ? s_ck[w][k + 1] : INT64_MAX;
+          r.li = fits ? li : -1;  // the continuation pass skips li < 0
+          r.pad = 0;
+          a.ck[base + k] = r;
+        }
       }
       if (fits) {
         rec = make_int2((int)base, nck);
@@ -1473,7 +1476,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fm
   __shared__ int64_t s_ck[1][kMaxCk];
   __shared__ int s_stack[1][kStack];
   __shared__ int s_far[1][kFarList];
-  __shared__ int64_t s_cku[1][kMaxCk];
+  __shared__ int32_t s_cku[1][kMaxCk];
   const int li = xcd_leaf(blockIdx.x, a.nleaves);
   if (li >= a.nleaves) return;
   fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck, s_cku, s_stack,
@@ -1495,7 +1498,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
   __shared__ int64_t s_ck[4][kMaxCk];
   __shared__ int s_stack[4][kStack];
   __shared__ int s_far[4][kFarList];
-  __shared__ int64_t s_cku[4][kMaxCk];
+  __shared__ int32_t s_cku[4][kMaxCk];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = *nleaves;
   for (;;) {
