@@ -63,6 +63,7 @@ lib.fmm_dump_tree.argtypes = [_up, _ip, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _dp]
 lib.fmm_dump_lists.argtypes = [C.c_int, _dp, C.c_int, _up, _ip, _ip]
 lib.fmm_direct.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_int]
 lib.fmm_direct_device.argtypes = [C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int]
+lib.fmm_direct_device_noalloc.argtypes = [C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp]
 lib.fmm_make_forest.argtypes = [C.c_int, _ip, _dp, _dp, _dp]
 lib.fmm_make_forest_device.argtypes = [C.c_int, _ip, _vp, _vp, _vp]
 lib.fmm_set_persistent.argtypes = [C.c_int, C.c_int]
@@ -421,6 +422,19 @@ def direct(xy, cir, cor, txy, style: int = 1) -> np.ndarray:
 def direct_device(ns, xy_ptr, cir_ptr, cor_ptr, nt, txy_ptr, uv_ptr, style: int = 1) -> None:
     _check(lib.fmm_direct_device(int(ns), xy_ptr, cir_ptr, cor_ptr, int(nt), txy_ptr, uv_ptr,
                                  int(style)), "direct_device")
+
+
+NOALLOC_MAX_TARGETS = 1 << 20
+
+
+def direct_device_noalloc(ns, xy_ptr, cir_ptr, cor_ptr, nt, txy_ptr, uv_ptr, style: int = 1,
+                          stream: int = 0) -> None:
+    """direct_device with its partial sums in the library's fixed device array
+    (no allocation; nt <= 2^20); runs on `stream` (0: the null stream) and
+    returns without waiting."""
+    _check(lib.fmm_direct_device_noalloc(int(ns), xy_ptr, cir_ptr, cor_ptr, int(nt), txy_ptr,
+                                         uv_ptr, int(style), stream or None),
+           "direct_device_noalloc")
 
 
 PHASES = ("bbox_keys", "radix_sort", "tree_levels", "upward", "evaluate")

@@ -212,6 +212,18 @@ int fmm_direct(int ns, const double *xy, const double *cir, const double *cor, i
 int fmm_direct_device(int ns, const double *d_xy, const double *d_cir, const double *d_cor, int nt,
                       const double *d_txy, double *d_uv, int style);
 
+/* fmm_direct_device without any allocation (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums live in a fixed 16 MiB device array that each GPU holds until
+ * the process ends, so nt is limited to 2^20 targets (larger nt: -1 with a
+ * message, nothing launched).  Runs on `stream` (a hipStream_t; NULL = the
+ * null stream) and returns without waiting.  Results equal
+ * fmm_direct_device's bit for bit.  Every call on one GPU shares its array:
+ * calls that overlap on one GPU (other streams or threads) race. */
+int fmm_direct_device_noalloc(int ns, const double *d_xy, const double *d_cir, const double *d_cor,
+                              int nt, const double *d_txy, double *d_uv, int style, void *stream);
+
 /* Measurement (no reference counterpart: the reference has no timers).
  * With timing on, HIP events on the library's stream bracket the phases of
  * the last build and evaluation; fmm_get_timing fills up to 5 values in ms:
