@@ -530,7 +530,7 @@ static int apply_learned() {
   const int nl = g.ntleaves;
   if (nl <= 0) return 0;
   if (g.feat.ensure((size_t)nl * 8, s) || g.mlp_out.ensure((size_t)nl * 2, s) ||
-      g.leaf_par.ensure(nl, s) || g.mean_abs.ensure(1, s))
+      g.leaf_par.ensure(nl, s) || g.mean_abs.ensure(kMeanAbsWords, s))
     return -1;
   FmmArgs a = fmm_args(nullptr);
   if (launch_leaf_features(a, g.t_num_p, g.sg.p, g.n, g.mean_abs.p, g.feat.p, s)) return -1;
@@ -626,7 +626,62 @@ static int eval_batch_dev(int nt, const double *d_txy, double *d_uv) {
     FMM_HIP_TRY(hipMemsetAsync(d_uv, 0, sizeof(double) * 2 * (size_t)nt, g.stream));
     return 0;
   }
+  if (g.t_mode != FMM2D_MODE_TREECODE) return launch_point_eval(fmm_args(nullptr), nt, d_txy, d_uv, g.stream);
   return launch_parity_eval(view(), nt, d_txy, nullptr, d_uv, g.stream);
+}
+
+// Pinned host buffer for the velocities at all blobs.
+static int ensure_host_uv_buffer(size_t doubles) {
+  if (doubles <= g.h_uv_cap) return 0;
+  if (g.h_uv) (void)hipHostFree(g.h_uv);
+  g.h_uv = nullptr;
+  g.h_uv_cap = 0;
+  if (hipHostMalloc((void **)&g.h_uv, sizeof(double) * doubles, hipHostMallocDefault) != hipSuccess) {
+    set_error("pinned host allocation failed");
+    return -1;
+  }
+  g.h_uv_cap = doubles;
+  return 0;
+}
+
+// The velocities at all blobs (caller order) and the blob positions in host
+// memory, for biot_tree's lookups: waits for make_tree's prefetch, or, when
+// there was none (prefetch off, a tree built from device arrays), evaluates
+// all blobs now -- in the accurate modes a blob's velocity is the all-blob
+// evaluation's, whichever way the tree was built.  Positions of a tree built
+// from device arrays come back from the leaf-ordered copies.
+static int ensure_host_uv() {
+  if (g.uv_pending) {
+    FMM_HIP_TRY(hipEventSynchronize(g.ev_uv));
+    g.uv_pending = false;
+    g.uv_ready = true;
+  }
+  if (g.uv_ready) return 0;
+  hipStream_t s = g.stream;
+  const size_t nb = g.n_real >= 0 ? (size_t)g.n_real : (size_t)g.n;
+  if (g.uv_dev.ensure(2 * (size_t)g.n, s) || ensure_host_uv_buffer(2 * nb)) return -1;
+  if (eval_sources_dev(g.uv_dev.p)) return -1;
+  FMM_HIP_TRY(hipMemcpyAsync(g.h_uv, g.uv_dev.p, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, s));
+  if (g.h_src.size() != 2 * nb) {
+    std::vector<double> hx(g.n), hy(g.n);
+    std::vector<uint32_t> hp(g.n);
+    FMM_HIP_TRY(hipMemcpyAsync(hx.data(), g.sx.p, sizeof(double) * g.n, hipMemcpyDeviceToHost, s));
+    FMM_HIP_TRY(hipMemcpyAsync(hy.data(), g.sy.p, sizeof(double) * g.n, hipMemcpyDeviceToHost, s));
+    FMM_HIP_TRY(hipMemcpyAsync(hp.data(), g.perm.p, sizeof(uint32_t) * g.n, hipMemcpyDeviceToHost, s));
+    FMM_HIP_TRY(hipStreamSynchronize(s));
+    g.h_src.assign(2 * nb, 0);
+    for (int j = 0; j < g.n; j++) {
+      const uint32_t b = hp[j];
+      if (b >= nb) continue;  // a mirror image, not a blob of the caller
+      memcpy(&g.h_src[2 * (size_t)b], &hx[j], 8);
+      memcpy(&g.h_src[2 * (size_t)b + 1], &hy[j], 8);
+    }
+    g.hash.clear();
+    g.cursor = 0;
+  }
+  FMM_HIP_TRY(hipStreamSynchronize(s));
+  g.uv_ready = true;
+  return 0;
 }
 
 static int host_eval_batch(int nt, const double *xy, double *uv) {
@@ -648,10 +703,11 @@ static inline uint64_t mix64(uint64_t a, uint64_t b) {
 }
 
 static void build_hash() {
+  const int nb = (int)(g.h_src.size() / 2);
   size_t cap = 16;
-  while (cap < 2 * (size_t)g.n) cap <<= 1;
+  while (cap < 2 * (size_t)nb) cap <<= 1;
   g.hash.assign(cap, -1);
-  for (int i = 0; i < g.n; i++) {
+  for (int i = 0; i < nb; i++) {
     size_t h = mix64(g.h_src[2 * i], g.h_src[2 * i + 1]) & (cap - 1);
     while (g.hash[h] >= 0) {
       const int j = g.hash[h];
@@ -663,7 +719,7 @@ static void build_hash() {
 }
 
 static int lookup(uint64_t bx, uint64_t by) {
-  if (g.cursor < (size_t)g.n && g.h_src[2 * g.cursor] == bx && g.h_src[2 * g.cursor + 1] == by)
+  if (g.cursor < g.h_src.size() / 2 && g.h_src[2 * g.cursor] == bx && g.h_src[2 * g.cursor + 1] == by)
     return (int)g.cursor;
   if (g.hash.empty()) build_hash();
   const size_t cap = g.hash.size();
@@ -749,19 +805,7 @@ void make_tree(int nvor_b, const double *vor_b, const double *cir_b, const doubl
       g.h_src.resize(2 * (size_t)n);
       if (n > 0) memcpy(g.h_src.data(), vor_b, sizeof(double) * 2 * n);
       if (g.prefetch && n > 0) {
-        if (g.uv_dev.ensure(2 * (size_t)n, s)) rc = -1;
-        if (!rc && 2 * (size_t)n > g.h_uv_cap) {
-          if (g.h_uv) (void)hipHostFree(g.h_uv);
-          g.h_uv = nullptr;
-          g.h_uv_cap = 0;
-          if (hipHostMalloc((void **)&g.h_uv, sizeof(double) * 2 * (size_t)n,
-                            hipHostMallocDefault) != hipSuccess) {
-            set_error("pinned host allocation failed");
-            rc = -1;
-          } else {
-            g.h_uv_cap = 2 * (size_t)n;
-          }
-        }
+        if (g.uv_dev.ensure(2 * (size_t)n, s) || ensure_host_uv_buffer(2 * (size_t)n)) rc = -1;
         if (!rc) rc = eval_sources_dev(g.uv_dev.p);
         if (!rc && (hipMemcpyAsync(g.h_uv, g.uv_dev.p, sizeof(double) * 2 * n,
                                    hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -788,7 +832,17 @@ void biot_tree(double x, double y, double *u, double *v) {
     return;
   }
   if (g.n == 0) return;
-  if (g.uv_pending) {
+  if (g.nsys > 1) {
+    fprintf(stderr, "biot_tree: a forest of systems is evaluated by fmm_eval_sources\n");
+    return;
+  }
+  if (g.t_mode != FMM2D_MODE_TREECODE) {
+    // accurate modes: a blob's velocity is always the all-blob evaluation's
+    if (ensure_host_uv()) {
+      fprintf(stderr, "biot_tree: %s\n", g_err);
+      return;
+    }
+  } else if (g.uv_pending) {
     if (hipEventSynchronize(g.ev_uv) != hipSuccess) {
       fprintf(stderr, "biot_tree: velocity prefetch failed\n");
       return;
@@ -1106,7 +1160,7 @@ int fmm_leaf_features(float *feat) {
   const int nl = g.ntleaves;
   if (nl <= 0) return 0;
   hipStream_t s = g.stream;
-  if (g.feat.ensure((size_t)nl * 8, s) || g.mean_abs.ensure(1, s)) return -1;
+  if (g.feat.ensure((size_t)nl * 8, s) || g.mean_abs.ensure(kMeanAbsWords, s)) return -1;
   if (launch_leaf_features(fmm_args(nullptr), g.t_num_p, g.sg.p, g.n, g.mean_abs.p, g.feat.p, s))
     return -1;
   FMM_HIP_TRY(hipMemcpyAsync(feat, g.feat.p, sizeof(float) * nl * 8, hipMemcpyDeviceToHost, s));

@@ -248,6 +248,9 @@ int launch_mirror_images(int n, const double *d_xy, const double *d_cir, double 
 
 // ---- learned per-leaf parameters (learned_kernels.hip) --------------------
 constexpr int kMlpWeights = 4866;  // 8 -> 64 -> 64 -> 2
+// d_mean_abs: kMeanAbsWords doubles; [0] gets mean |cir| (computed in a fixed
+// order), the rest is scratch for block partials.
+constexpr int kMeanAbsWords = 1 + 1024;
 int launch_leaf_features(const FmmArgs &a, int leaf_cap, const double *d_cir, int n,
                          double *d_mean_abs, float *d_feat, hipStream_t s);
 int launch_mlp_forward(const float *d_w, int n, const float *d_x, float *d_out, hipStream_t s);
@@ -265,5 +268,9 @@ int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *
                                uint32_t *d_counter, int nblocks, uint32_t *d_split_ctr,
                                int split_blocks, hipStream_t s);
 int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s);
+// Accurate modes at nt arbitrary points (interleaved xy): lane per point,
+// far test with the point as a target leaf of size 0, FP64 (single tree).
+int launch_point_eval(const FmmArgs &args, int nt, const double *d_txy, double *d_uv,
+                      hipStream_t s);
 
 }  // namespace fmm2d

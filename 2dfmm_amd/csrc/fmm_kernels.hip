@@ -1662,6 +1662,105 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
   out[3 * li + 2] = smooth;
 }
 
+// ---------------------------------------------------------------------------
+// Accurate modes at arbitrary points (not blobs of the tree: e.g. the sheet
+// centres of e_velo_mpi.f90:59-70): one lane per point, depth first through
+// the tree.  The far test is the target leaf's with the leaf shrunk to the
+// point (R_A = 0): a cell is far iff R_C <= theta d and d - R_C >= r_cut, so
+// the neglected core term and the truncation obey the same bounds as for the
+// blobs.  A far cell's multipole is evaluated at the point,
+//   phi += sum_{k<p} a_k w^(k+1),  w = 1 / (z - c_C)   (Horner),
+// and near leaves are summed with the regularised kernel, both in FP64 in
+// either accurate mode (few points: precision is cheaper than a second path).
+// Fixed p and r_cut (learned per-leaf values belong to target leaves).
+// ---------------------------------------------------------------------------
+__device__ inline void m2p_horner(const double2 *__restrict__ a, int p, double dx, double dy,
+                                  double &phr, double &phi) {
+  const double r2 = fma(dx, dx, dy * dy);
+  double ri = __builtin_amdgcn_rcp(r2);
+  ri = ri * fma(-r2, ri, 2.0);
+  ri = ri * fma(-r2, ri, 2.0);
+  const double wr = dx * ri, wi = -dy * ri;
+  double sr = 0.0, si = 0.0;
+  for (int k = p - 1; k >= 0; k--) {
+    const double2 ak = a[k];
+    const double nr = fma(sr, wr, fma(-si, wi, ak.x));
+    const double ni = fma(sr, wi, fma(si, wr, ak.y));
+    sr = nr;
+    si = ni;
+  }
+  phr += sr * wr - si * wi;
+  phi += sr * wi + si * wr;
+}
+
+__global__ __launch_bounds__(256) void point_eval_kernel(FmmArgs a, int nt,
+                                                         const double2 *__restrict__ txy,
+                                                         double2 *__restrict__ uv) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nt) return;
+  const double2 z = txy[t];
+  NodeRec P = {};
+  P.cx = z.x;
+  P.cy = z.y;
+  const double inv_theta = 1.0 / a.theta;
+  double phr = 0.0, phi = 0.0, ax = 0.0, ay = 0.0;
+  Slots2 sl;
+  int d = 0, cur = 0, node = 0;  // a single tree: root 0, always split
+  bool expand = true;
+  for (;;) {
+    if (expand) {
+      const int base = a.nodes[node].first;
+      uint32_t m = 0;
+      for (int q = 0; q < 4; q++) {
+        const NodeRec C = a.nodes[base + q];
+        if (node_count(C.cnt) == 0) continue;
+        if (far_cell(inv_theta, 0.0, a.rcut, P, C))
+          m2p_horner(a.coef + (size_t)(base + q) * a.p, a.p, z.x - C.cx, z.y - C.cy, phr, phi);
+        else
+          m |= 1u << q;
+      }
+      d++;
+      sl.set(d, m);
+      cur = base;
+    }
+    uint32_t m;
+    bool done = false;
+    for (;;) {
+      m = sl.get(d);
+      if (m) break;
+      if (d == 1) {
+        done = true;
+        break;
+      }
+      cur = a.gparent[cur >> 2];
+      d--;
+    }
+    if (done) break;
+    const int q = 31 - __clz(m);
+    sl.set(d, m & ~(1u << q));
+    node = (cur & ~3) + q;
+    const NodeRec N = a.nodes[node];
+    if (node_internal(N.cnt)) {
+      expand = true;
+    } else {
+      const int f = N.first, c = node_count(N.cnt);
+      for (int j = f; j < f + c; j++) pair64(z.x, z.y, a.loc64[j], ax, ay);
+      expand = false;
+    }
+  }
+  uv[t] = make_double2(0.5 * phi / M_PI + 0.5 * ax / M_PI, 0.5 * phr / M_PI + 0.5 * ay / M_PI);
+}
+
+int launch_point_eval(const FmmArgs &args, int nt, const double *d_txy, double *d_uv,
+                      hipStream_t s) {
+  if (nt <= 0) return 0;
+  point_eval_kernel<<<(nt + 255) / 256, 256, 0, s>>>(args, nt,
+                                                     reinterpret_cast<const double2 *>(d_txy),
+                                                     reinterpret_cast<double2 *>(d_uv));
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s) {
   if (args.nleaves <= 0) return 0;
   fmm_count_kernel<<<(args.nleaves + 255) / 256, 256, 0, s>>>(args, d_out);

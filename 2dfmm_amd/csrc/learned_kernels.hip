@@ -55,8 +55,13 @@ __global__ __launch_bounds__(256) void leaf_features_kernel(const FmmArgs a, int
   for (int k = 0; k < kIn; k++) feat[(size_t)li * kIn + k] = f[k];
 }
 
+// mean |cir| in a fixed order: block partials (grid-stride sums, then a
+// fixed butterfly and the four waves in order), then one block adds the
+// partials in block order -- the same bits on every run.
+constexpr int kAbsBlocks = 1024;
+static_assert(kMeanAbsWords >= 1 + kAbsBlocks, "mean |cir| scratch");
 __global__ __launch_bounds__(256) void abs_sum_kernel(const double *__restrict__ v, int n,
-                                                      double *__restrict__ out) {
+                                                      double *__restrict__ part) {
   double s = 0.0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += fabs(v[i]);
 #pragma unroll
@@ -64,19 +69,24 @@ __global__ __launch_bounds__(256) void abs_sum_kernel(const double *__restrict__
   __shared__ double sm[4];
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(out, (sm[0] + sm[1]) + (sm[2] + sm[3]));
+  if (threadIdx.x == 0) part[blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
 }
 
-__global__ void scale_kernel(double *v, double f) { *v *= f; }
+// out[0] = (sum of the nb partials at out[1..nb], in order) / n
+__global__ __launch_bounds__(64) void abs_mean_kernel(double *__restrict__ out, int nb, int n) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; b++) s += out[1 + b];
+  out[0] = s / (double)(n > 0 ? n : 1);
+}
 
 int launch_leaf_features(const FmmArgs &a, int leaf_cap, const double *d_cir, int n,
                          double *d_mean_abs, float *d_feat, hipStream_t s) {
   if (a.nleaves <= 0) return 0;
-  FMM_HIP_TRY(hipMemsetAsync(d_mean_abs, 0, sizeof(double), s));
   int nb = (n + 255) / 256;
-  nb = nb > 1024 ? 1024 : (nb < 1 ? 1 : nb);
-  abs_sum_kernel<<<nb, 256, 0, s>>>(d_cir, n, d_mean_abs);
-  scale_kernel<<<1, 1, 0, s>>>(d_mean_abs, 1.0 / (n > 0 ? n : 1));
+  nb = nb > kAbsBlocks ? kAbsBlocks : (nb < 1 ? 1 : nb);
+  abs_sum_kernel<<<nb, 256, 0, s>>>(d_cir, n, d_mean_abs + 1);
+  abs_mean_kernel<<<1, 64, 0, s>>>(d_mean_abs, nb, n);
   leaf_features_kernel<<<(a.nleaves + 255) / 256, 256, 0, s>>>(a, leaf_cap, d_mean_abs, d_feat);
   FMM_HIP_TRY(hipGetLastError());
   return 0;

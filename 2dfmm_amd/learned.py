@@ -120,6 +120,7 @@ def evaluate(fmm, xy, cir, cor, weights, p_margin, cut_margin, judge=4000, seed=
         far, pairs = fmm.count_interactions()
         par = fmm.leaf_params()
         out[name] = {"rel_L2": float(np.linalg.norm(uv[idx] - ref) / np.linalg.norm(ref)),
+                     "rel_max": float(np.abs(uv[idx] - ref).max() / np.abs(ref).max()),
                      "eval_ms": float(np.min(t)), "pairs": int(pairs), "far": int(far),
                      "mean_p": float(par[:, 0].mean()), "mean_cut": float(par[:, 1].mean())}
     fmm.set_learned(None)

@@ -42,9 +42,12 @@ extern "C" {
 void make_tree(int nvor_b, const double *vor_b, const double *cir_b, const double *cor_b);
 
 /* Velocity (u, v) induced at (x, y) by the blobs of the last make_tree, by
- * the reference's treecode rule (fmm.c:69-118).  A blob position returns the
- * velocity computed during make_tree; any other point is evaluated on the GPU
- * one at a time (use fmm_eval_batch for many points). */
+ * the reference's treecode rule (fmm.c:69-118) in the treecode mode, or by the
+ * accurate modes (fmm_set_mode).  A blob position returns the velocity
+ * computed for all blobs (during make_tree, or on the first call); any other
+ * point is evaluated on the GPU one at a time (use fmm_eval_batch for many
+ * points).  Trees built by fmm_make_tree_device or the mirror variant are
+ * looked up the same way. */
 void biot_tree(double x, double y, double *u, double *v);
 
 /* ---- extensions --------------------------------------------------------- */
@@ -68,9 +71,11 @@ int fmm_get_params(int *nterms, int *num_p, double *theta, double *r_min);
  * contraction on MFMA, near leaves are summed directly.  MIXED runs the near
  * pairs in FP32 (coordinates relative to a leaf centre; groups of pairs
  * summed in FP32 and accumulated in FP64) and the contraction on FP32 MFMA;
- * expansions and the final accumulation stay FP64.  These modes
- * apply to the all-blob evaluation (make_tree's prefetch, fmm_eval_sources*);
- * other points are evaluated by the treecode rule on the same tree. */
+ * expansions and the final accumulation stay FP64.  At the blobs
+ * (make_tree's prefetch, fmm_eval_sources*, biot_tree at a blob position)
+ * these modes use the target-leaf kernels; other points (fmm_eval_batch*,
+ * biot_tree elsewhere) get the same far test with the point as a target leaf
+ * of size zero, multipoles evaluated at the point and FP64 near sums. */
 #define FMM2D_MODE_TREECODE 0
 #define FMM2D_MODE_FMM64 1
 #define FMM2D_MODE_MIXED 2
@@ -83,8 +88,11 @@ int fmm_set_fmm_params(int p, int leaf, double theta, double cut_sigmas);
  * plain loop on the device; *max_abs_err gets the largest difference. */
 int fmm_selftest_mfma(int fp32, double *max_abs_err);
 
-/* 0: make_tree does not evaluate at the blobs (biot_tree then evaluates on
- * demand); 1 (default): it does, asynchronously. */
+/* 0: make_tree does not evaluate at the blobs; 1 (default): it does,
+ * asynchronously.  Only the timing changes: with prefetch off, biot_tree at a
+ * blob position evaluates every blob on its first call in the accurate modes
+ * (the same values as the prefetch) and that one point in the treecode mode
+ * (the same rule). */
 int fmm_set_prefetch(int on);
 
 const char *fmm_last_error(void);

@@ -139,3 +139,76 @@ def test_fmm_varying_core_radii(fmm, oracle):
     du = oracle.direct(xy, cir, cu, xy[idx], style=1)
     assert rel_err(uu[idx], du)[0] <= 1e-6
     fmm.set_mode(fmm.MODE_TREECODE)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_points_off_blobs_vs_direct(fmm, oracle, mode):
+    """Accurate modes at points that are not blobs (the solver's sheet centres,
+    e_velo_mpi.f90:59-70; points in empty space; points outside the root
+    square) against the FP64 direct sum; at blob positions the point walk and
+    the target-leaf kernel agree to the accuracy of both."""
+    wl = workloads()
+    n = 40000
+    xy, cir, cor = wl.sheet(n, seed=71)
+    mxy, mcir, mcor = wl.gaussian_mixture(n, seed=72)
+    xy = np.concatenate([xy, mxy])
+    cir = np.concatenate([cir, mcir])
+    cor = np.concatenate([cor, np.full(n, 1e-3)])
+    centres = 0.5 * (xy[:n - 1][::40] + xy[1:n][::40])
+    rng = np.random.default_rng(73)
+    empty = rng.random((500, 2))
+    outside = np.array([[-0.3, 0.5], [1.7, -0.2], [0.5, 2.5], [-1.0, -1.0]])
+    pts = np.concatenate([centres, empty, outside])
+    fmm.set_mode(mode)
+    fmm.set_fmm_params(16, 64, 0.5, 4.0)
+    fmm.make_tree(xy, cir, cor)
+    uv_src = fmm.eval_sources()
+    got = fmm.eval_batch(pts)
+    d = oracle.direct(xy, cir, cor, pts, style=1)
+    l2, mx = rel_err(got, d)
+    _say(f"[points] mode {mode}: {len(pts)} off-blob points vs FP64 direct: rel_L2={l2:.2e} "
+         f"rel_max={mx:.2e}")
+    assert l2 <= 5e-7 and mx <= 5e-6
+    for j in range(0, len(pts), 97):
+        assert fmm.biot_tree(*pts[j]) == tuple(got[j])
+    idx = wl.sample_targets(2 * n, 500, seed=74)
+    at_blobs = fmm.eval_batch(xy[idx])
+    l2b, _ = rel_err(at_blobs, uv_src[idx])
+    _say(f"  point walk vs target-leaf kernel at 500 blobs: rel_L2={l2b:.2e}")
+    assert l2b <= 1e-6
+    fmm.set_mode(fmm.MODE_TREECODE)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_accurate_biot_tree_without_prefetch(fmm, mode):
+    """With the prefetch off, and for trees built from device arrays or with
+    mirror images, biot_tree at a blob returns the accurate all-blob value
+    (never the treecode rule)."""
+    import torch
+
+    wl = workloads()
+    n = 6000
+    xy, cir, cor = wl.gaussian_mixture(n, seed=75)
+    ref = _run(fmm, mode, xy, cir, cor)
+    fmm.set_prefetch(False)
+    try:
+        fmm.make_tree(xy, cir, cor)
+        for i in range(0, n, 251):
+            assert fmm.biot_tree(*xy[i]) == tuple(ref[i])
+        dev = torch.device("cuda:0")
+        d_xy = torch.from_numpy(xy).to(dev)
+        d_cir = torch.from_numpy(cir).to(dev)
+        d_cor = torch.from_numpy(cor).to(dev)
+        torch.cuda.synchronize()
+        fmm.make_tree_device(n, d_xy.data_ptr(), d_cir.data_ptr(), d_cor.data_ptr())
+        rng = np.random.default_rng(76)
+        for i in rng.permutation(n)[:40]:
+            assert fmm.biot_tree(*xy[i]) == tuple(ref[i])
+        # mirror images: the real blobs' values, looked up by their positions
+        fmm.make_tree_mirror_device(n, d_xy.data_ptr(), d_cir.data_ptr(), d_cor.data_ptr(), -0.5)
+        m_uv = fmm.eval_sources()
+        for i in range(0, n, 601):
+            assert fmm.biot_tree(*xy[i]) == tuple(m_uv[i])
+    finally:
+        fmm.set_prefetch(True)
+        fmm.set_mode(fmm.MODE_TREECODE)

@@ -290,3 +290,45 @@ def test_full_size_mixture_tree_invariants(fmm):
             assert t["cx"][kids[1]] == t["cx"][i] + h and t["cy"][kids[2]] == t["cy"][i] - h
     finally:
         fmm.set_prefetch(True)
+
+
+def test_snapshot_through_library(fmm, tmp_path):
+    """Snapshot files on the GPU path: a solver snapshot holding the blobs of
+    the reference's own sample run (golden sample_s00052) is written, read
+    back, evaluated blob by blob through biot_tree in the solver's loop order
+    (e_velo_mpi.f90:37-48), the velocities are stored in vor_vb(:, 1, :)
+    (f_data_out.f90:97-107) and the file is read again: the velocities are
+    the reference's to 1e-12, everything else is unchanged."""
+    import struct
+
+    from conftest import sfile
+
+    sf = sfile()
+    g = load_golden("sample_s00052")
+    n = g["xy"].shape[0]
+    nterms, num_p, theta = int(g["params"][0]), int(g["params"][1]), float(g["params"][2])
+    rng = np.random.default_rng(52)
+    head = [struct.pack("<2d", 3000.0, 0.015), struct.pack("<d", 0.78), struct.pack("<i", 52)]
+    tail = [struct.pack("<i", 0), struct.pack("<2d", 0.0, 0.0)]
+    snap = sf.Snapshot(xy=g["xy"].copy(), vel=rng.standard_normal((n, 3, 2)), cir=g["cir"].copy(),
+                       cor=g["cor"].copy(), blob_id=np.arange(n, dtype=np.int32) + 1,
+                       records=head + [struct.pack("<i", 0)] + tail, blob_record=len(head))
+    p0 = str(tmp_path / "s_00052_step.dat")
+    sf.write_snapshot(p0, snap)
+    s = sf.read_snapshot(p0)
+    assert np.array_equal(s.xy, g["xy"]) and np.array_equal(s.cir, g["cir"])
+    _say(f"[snapshot] N={n} p={nterms} num_p={num_p} theta={theta}")
+    fmm.set_params(nterms, num_p, theta)
+    fmm.make_tree(s.xy, s.cir, s.cor)
+    old_hist = s.vel[:, 1:].copy()
+    for i in range(n):
+        s.vel[i, 0] = fmm.biot_tree(*s.xy[i])
+    p1 = str(tmp_path / "s_00053_step.dat")
+    sf.write_snapshot(p1, s)
+    t = sf.read_snapshot(p1)
+    l2, mx = rel_err(t.vel[:, 0], g["uv"])
+    _say(f"  vor_vb(:,1,:) vs the reference's velocities: rel_L2={l2:.2e} rel_max={mx:.2e}")
+    assert mx <= TOL
+    assert np.array_equal(t.vel[:, 1:], old_hist) and np.array_equal(t.blob_id, s.blob_id)
+    assert np.array_equal(t.xy, g["xy"]) and np.array_equal(t.cor, g["cor"])
+    assert sf.read_records(p1)[:len(head)] == head and sf.read_records(p1)[-2:] == tail
