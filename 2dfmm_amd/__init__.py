@@ -68,10 +68,13 @@ lib.fmm_make_forest.argtypes = [C.c_int, _ip, _dp, _dp, _dp]
 lib.fmm_make_forest_device.argtypes = [C.c_int, _ip, _vp, _vp, _vp]
 lib.fmm_set_persistent.argtypes = [C.c_int, C.c_int]
 lib.fmm_set_split.argtypes = [C.c_int]
+lib.fmm_set_far_burst.argtypes = [C.c_int]
 lib.fmm_get_split.argtypes = []
 lib.fmm_set_smooth.argtypes = [C.c_double, C.c_int]
 lib.fmm_get_smooth.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
 lib.fmm_count_work.argtypes = [C.POINTER(C.c_longlong)] * 3
+lib.fmm_count_work_detail.argtypes = [np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                      C.c_int]
 lib.fmm_device_cus.argtypes = []
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 lib.fmm_set_learned.argtypes = [C.c_void_p, C.c_float, C.c_float]
@@ -314,6 +317,10 @@ def set_persistent(blocks: int, graph: bool = False) -> None:
     _check(lib.fmm_set_persistent(int(blocks), 1 if graph else 0), "set_persistent")
 
 
+def set_far_burst(cells: int) -> None:
+    _check(lib.fmm_set_far_burst(int(cells)), "set_far_burst")
+
+
 def set_split(budget: int) -> None:
     """Near work units per chunk of a split target leaf (0: no splitting)."""
     _check(lib.fmm_set_split(int(budget)), "set_split")
@@ -437,7 +444,8 @@ def direct_device_noalloc(ns, xy_ptr, cir_ptr, cor_ptr, nt, txy_ptr, uv_ptr, sty
            "direct_device_noalloc")
 
 
-PHASES = ("bbox_keys", "radix_sort", "tree_levels", "upward", "evaluate")
+PHASES = ("bbox_keys", "radix_sort", "tree_levels", "upward", "evaluate",
+          "levels", "leaf_order_gather", "s2m_m2m", "fused_pass", "split_passes")
 
 
 def set_timing(on: bool) -> None:
@@ -456,6 +464,16 @@ def count_interactions():
     a, b = C.c_longlong(), C.c_longlong()
     _check(lib.fmm_count_interactions(C.byref(a), C.byref(b)), "count_interactions")
     return a.value, b.value
+
+
+WORK_KEYS = ("far", "pairs", "smooth", "far_terms2", "l2p_terms", "target_leaves")
+
+
+def count_work_detail() -> dict:
+    """The evaluation's work in detail (see fmm_count_work_detail)."""
+    out = np.zeros(len(WORK_KEYS), np.int64)
+    _check(lib.fmm_count_work_detail(out, len(WORK_KEYS)), "count_work_detail")
+    return dict(zip(WORK_KEYS, (int(v) for v in out)))
 
 
 def count_work():

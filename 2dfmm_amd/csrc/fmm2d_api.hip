@@ -117,6 +117,7 @@ struct Context {
   int persist_blocks = 0;  // 0: one wave per leaf over a sized grid
   // splitting of target leaves with much near work (0: off)
   int split_budget = 8192;
+  int far_burst = 128;  // far cells per MFMA burst (64..128)
   DevBuf<SplitCk> ck;
   DevBuf<int32_t> nfirst;
   DevBuf<int2> leaf_ck;
@@ -140,10 +141,15 @@ struct Context {
   unsigned char graph_key[sizeof(FmmArgs) + 16] = {};
   DevBuf<double> xy, cir, cor;
   DevBuf<uint64_t> key, key_alt;
-  DevBuf<uint32_t> idx, idx_alt, hist, hist_scan, perm, flag, pos, total;
+  DevBuf<uint32_t> idx, idx_alt, hist, hist_scan, perm, total;
   DevBuf<double> sx, sy, sg, ss, bbox_part, uv_dev, tmp_in, tmp_out;
   DevBuf<NodeRec> nodes;
   DevBuf<int32_t> gparent, split;
+  DevBuf<int32_t> lt;         // level table of the tree build (see launch_tree_level)
+  DevBuf<uint32_t> lstat;     // status words of the levels' split-list scans
+  DevBuf<uint8_t> cflag;      // split flags of one level's children
+  int32_t *h_lt = nullptr;    // pinned copy of the level table
+  int depth_hint = 0;         // depth of the previous tree
   DevBuf<double2> coef;
   DevBuf<RootBox> root;
   std::vector<int> split_off;
@@ -158,10 +164,12 @@ struct Context {
   std::vector<int32_t> hash;
   size_t cursor = 0;
   // phase timing with events on the library stream:
-  // 0 start, 1 keys, 2 sorted, 3 tree, 4 expansions | 5, 6 around the last evaluation
+  // 0 start, 1 keys, 2 sorted, 7 levels, 3 tree (leaf order, gather), 8 S2M/M2M,
+  // 4 expansions (and the accurate modes' set-up) | 5, 9, 6 around the last
+  // evaluation (9: after the fused pass)
   bool timing = false;
-  hipEvent_t tev[7] = {};
-  bool tev_rec[7] = {};  // recorded since timing was switched on
+  hipEvent_t tev[10] = {};
+  bool tev_rec[10] = {};  // recorded since timing was switched on
 };
 
 static Context g;
@@ -177,6 +185,7 @@ static int ctx_init() {
   }
   FMM_HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   FMM_HIP_TRY(hipHostMalloc((void **)&g.h_total, 64, hipHostMallocDefault));
+  FMM_HIP_TRY(hipHostMalloc((void **)&g.h_lt, sizeof(int32_t) * kLtWords, hipHostMallocDefault));
   memset(g.h_total, 0, 64);
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_uv, hipEventDisableTiming));
   for (auto &e : g.tev) FMM_HIP_TRY(hipEventCreate(&e));
@@ -280,12 +289,14 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
       g.total.ensure(4, s) || g.hist.ensure(radix_sort_hist_words(n), s) ||
       g.hist_scan.ensure(radix_sort_hist_words(n), s))
     return -1;
-  size_t est = nroot_slots + 16 * ((size_t)n / (size_t)g.t_num_p + 1);
-  if (g.nodes.ensure_keep(est, 0, s) || g.gparent.ensure_keep(est / 4 + 1, 0, s) ||
-      g.split.ensure_keep(est / 4 + nsys + 1, 0, s))
-    return -1;
+  // node capacity: a first guess, doubled (and the build redone) whenever a
+  // level launch reports that it ran out of room
+  size_t est = std::max(g.nodes.cap, nroot_slots + 4 * ((size_t)n / (size_t)g.t_num_p + 1) + 64);
   int nroots = 1, child_base = 0;
-  for (int full = 0;; full = 1) {
+  for (int full = 0;;) {
+    if (g.nodes.ensure_keep(est, 0, s) || g.gparent.ensure_keep(g.nodes.cap / 4 + 1, 0, s) ||
+        g.split.ensure_keep(g.nodes.cap / 4 + nsys + 1, 0, s))
+      return -1;
     mark(0);
     if (nsys == 1) {
       if (launch_bbox(dxy, n, g.root.p, g.bbox_part.p, s)) return -1;
@@ -320,49 +331,73 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     mark(2);
     // (the sort may swap key/key_alt and idx/idx_alt; each pair is always
     // sized together, so their capacities are equal and need no fix-up)
-    // level-by-level build
-    g.split_off.assign({0, nroots});
-    child_base = nroot_slots;
-    int nsplit_prev = nroots, level = 1;
-    while (nsplit_prev > 0) {
-      if (level > D) {
-        set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D,
-                  g.t_num_p);
-        return -1;
+    // Levels 1..lsort, one launch each, with the counts kept on the device;
+    // the host reads the level table back once per batch of launches (the
+    // first batch reaches one level past the previous tree's depth).
+    const int node_cap = (int)std::min<size_t>(g.nodes.cap, (size_t)INT32_MAX);
+    const int split_cap = (int)std::min<size_t>(g.split.cap, (size_t)INT32_MAX);
+    const int stat_len = tree_level_tiles(node_cap) + 2;
+    if (g.lstat.ensure(2 * (size_t)stat_len, s) || g.lt.ensure(kLtWords, s) ||
+        g.cflag.ensure((size_t)node_cap, s))
+      return -1;
+    if (launch_tree_levels_init(g.lt.p, nroots, nroot_slots, g.lstat.p, stat_len, s)) return -1;
+    const int max_splits = (int)std::min<int64_t>((int64_t)n / std::max(1, g.t_num_p) + nsys,
+                                                  (int64_t)split_cap);
+    int launched = 0, depth = 0;
+    bool more = false, overflow = false;
+    for (;;) {
+      const int upto = launched == 0 ? std::min(lsort, std::max(g.depth_hint + 1, 6)) : lsort;
+      for (int level = launched + 1; level <= upto; level++) {
+        // split nodes of level - 1: at most nroots 4^(level-1), and at most
+        // one per num_p blobs
+        int64_t bound = nroots;
+        for (int k = 1; k < level && bound < max_splits; k++) bound *= 4;
+        bound = std::min<int64_t>(bound, max_splits);
+        if (launch_tree_level(g.key.p, 2 * D, level, D, g.t_num_p, (int)bound, g.nodes.p,
+                              g.gparent.p, g.split.p, node_cap, split_cap, g.lt.p, g.cflag.p,
+                              g.lstat.p, stat_len, s))
+          return -1;
       }
-      if (level > lsort) break;  // deeper than the sorted bits reach
-      const int nchild = 4 * nsplit_prev;
-      if (g.nodes.ensure_keep((size_t)child_base + nchild, child_base, s) ||
-          g.gparent.ensure_keep(((size_t)child_base + nchild) / 4, child_base / 4, s) ||
-          g.flag.ensure(nchild, s) || g.pos.ensure(nchild, s))
-        return -1;
-      if (launch_build_level(g.key.p, 2 * D, level, g.split.p + g.split_off[level - 1],
-                             nsplit_prev, child_base, g.t_num_p, g.nodes.p, g.gparent.p, g.flag.p,
-                             s))
-        return -1;
-      if (scan_u32(g.flag.p, g.pos.p, nchild, g.total.p, s)) return -1;
-      FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+      launched = upto;
+      FMM_HIP_TRY(hipMemcpyAsync(g.h_lt, g.lt.p, sizeof(int32_t) * kLtWords, hipMemcpyDeviceToHost,
                                  s));
       FMM_HIP_TRY(hipStreamSynchronize(s));
-      const int nsplit = (int)g.h_total[0];
-      if (nsplit > 0 && level == D) {
+      const int err = g.h_lt[kLtErr];
+      if (err & kLtErrDepth) {
         set_error("tree depth exceeds %d levels (are >= num_p=%d blobs coincident?)", D,
                   g.t_num_p);
         return -1;
       }
-      const int off = g.split_off[level];
-      if (g.split.ensure_keep((size_t)off + nsplit + 1, off, s)) return -1;
-      if (nsplit > 0 &&
-          launch_compact_split(g.flag.p, g.pos.p, nchild, child_base, g.split.p + off, s))
+      if (err & kLtErrLookback) {
+        set_error("tree build: a level's split-list scan did not complete");
         return -1;
-      g.split_off.push_back(off + nsplit);
-      child_base += nchild;
-      g.depth = level;
-      level++;
-      nsplit_prev = nsplit;
+      }
+      if (err & kLtErrOverflow) {
+        overflow = true;
+        break;
+      }
+      if (g.h_lt[kLtS + launched] > 0 && launched < lsort) continue;  // deeper: next batch
+      more = g.h_lt[kLtS + launched] > 0;  // deeper than the sorted bits reach
+      depth = 1;
+      while (depth < launched && g.h_lt[kLtS + depth] > 0) depth++;
+      break;
     }
-    if (nsplit_prev == 0) break;
+    if (overflow) {
+      est = 2 * g.nodes.cap;
+      continue;
+    }
+    if (more) {
+      full = 1;
+      continue;
+    }
+    g.depth = depth;
+    g.depth_hint = depth;
+    g.split_off.resize(depth + 2);
+    for (int l = 0; l <= depth + 1; l++) g.split_off[l] = g.h_lt[kLtOff + l];
+    child_base = g.h_lt[kLtCb + depth + 1];
+    break;
   }
+  mark(7);
   g.nnodes = child_base;
   g.nleaves = (child_base - nroot_slots + nroots) - g.split_off.back();
   if (launch_leaf_order(g.nodes.p, g.nnodes, g.idx.p, g.perm.p, g.t_num_p, s)) return -1;
@@ -378,6 +413,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     const int a = g.split_off[l], b = g.split_off[l + 1];
     if (launch_m2m_level(g.nodes.p, g.split.p + a, b - a, g.t_nterms, g.coef.p, s)) return -1;
   }
+  mark(8);
   if (g.t_mode != FMM2D_MODE_TREECODE) {
     // target leaves, leaf-relative FP32 sources, near-field radius
     if (g.leaves.ensure(n, s) || g.lat.ensure(n, s) || g.lflag.ensure(n, s) || g.lpos.ensure(n, s) ||
@@ -494,6 +530,7 @@ static FmmArgs fmm_args(double *d_uv) {
   a.scoef = g.scoef.p;
   a.srange = g.srange.p;
   a.sunits = g.sunits.p;
+  a.far_burst = g.far_burst;
   return a;
 }
 
@@ -595,6 +632,7 @@ static int eval_sources_dev(double *d_uv) {
     return 0;
   }
   mark(5);
+  g.tev_rec[9] = false;  // set again only if this evaluation records it
   int rc;
   const bool mixed = g.t_mode == FMM2D_MODE_MIXED;
   if (g.t_mode != FMM2D_MODE_TREECODE && ensure_split()) return -1;
@@ -607,7 +645,9 @@ static int eval_sources_dev(double *d_uv) {
                                     g.persist_blocks, g.split_ctr.p, split_blocks(), g.stream);
   else
     rc = launch_fmm_eval(fmm_args(d_uv), mixed, g.ntl_dev.p, g.split_ctr.p, split_blocks(),
-                         g.stream);
+                         g.stream, g.timing ? g.tev[9] : nullptr);
+  if (!rc && g.timing && g.t_mode != FMM2D_MODE_TREECODE && g.persist_blocks == 0)
+    g.tev_rec[9] = true;
   mark(6);
   if (!rc && g.t_mode != FMM2D_MODE_TREECODE && g.split_budget > 0)
     FMM_HIP_TRY(hipMemcpyAsync(g.h_total + 8, g.split_ctr.p, sizeof(uint32_t),
@@ -1097,6 +1137,16 @@ int fmm_set_split(int budget) {
   return 0;
 }
 
+int fmm_set_far_burst(int cells) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (cells < 64 || cells > 128) {
+    set_error("fmm_set_far_burst: %d cells (64..128)", cells);
+    return -1;
+  }
+  g.far_burst = cells;
+  return 0;
+}
+
 int fmm_get_split(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   return g.split_budget;
@@ -1580,8 +1630,9 @@ int fmm_get_timing(double *ms, int n) {
     return -1;
   }
   FMM_HIP_TRY(hipStreamSynchronize(g.stream));
-  const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {5, 6}};
-  for (int i = 0; i < n && i < 5; i++) {
+  const int pairs[10][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {5, 6},
+                            {2, 7}, {7, 3}, {3, 8}, {5, 9}, {9, 6}};
+  for (int i = 0; i < n && i < 10; i++) {
     float t = -1.f;
     // only events recorded since timing was switched on are queried, so no
     // error is left behind for a later launch check to report
@@ -1595,35 +1646,38 @@ int fmm_get_timing(double *ms, int n) {
   return 0;
 }
 
-// Work of the evaluation with all blobs as targets (see fmm_count_work).
-static int count_work(long long *m2s, long long *pairs, long long *smooth) {
+// Work of the evaluation with all blobs as targets: out[0] far cells (M2L
+// contractions of target leaves; treecode: M2S of (target, cell)), [1] near
+// pairs, [2] (target, smooth cell) evaluations, [3] sum over target leaves
+// of far cells x terms^2, [4] sum of targets x terms (L2P), [5] target leaves.
+constexpr int kWorkOut = 6;
+static int count_work(long long *out) {
   if (need_tree()) return -1;
-  *m2s = 0;
-  *pairs = 0;
-  *smooth = 0;
+  for (int i = 0; i < kWorkOut; i++) out[i] = 0;
   if (g.n == 0) return 0;
   if (g.t_mode != FMM2D_MODE_TREECODE) {
     DevBuf<int64_t> c;
-    if (c.ensure(3 * (size_t)std::max(g.ntleaves, 1), g.stream)) return -1;
-    std::vector<int64_t> h(3 * (size_t)g.ntleaves);
+    const size_t nw = (size_t)kCountWords * std::max(g.ntleaves, 1);
+    if (c.ensure(nw, g.stream)) return -1;
+    std::vector<int64_t> h(nw);
     int rc = launch_fmm_count(fmm_args(nullptr), c.p, g.stream);
-    if (!rc && (hipMemcpyAsync(h.data(), c.p, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost,
-                               g.stream) != hipSuccess ||
+    if (!rc && (hipMemcpyAsync(h.data(), c.p, sizeof(int64_t) * (size_t)kCountWords * g.ntleaves,
+                               hipMemcpyDeviceToHost, g.stream) != hipSuccess ||
                 hipStreamSynchronize(g.stream) != hipSuccess)) {
       set_error("count download failed");
       rc = -1;
     }
     c.release();
     if (rc) return rc;
-    long long a = 0, b = 0, sm = 0;
     for (int i = 0; i < g.ntleaves; i++) {
-      a += h[3 * i];
-      b += h[3 * i + 1];
-      sm += h[3 * i + 2];
+      const int64_t *o = &h[(size_t)kCountWords * i];
+      out[0] += o[0];
+      out[1] += o[1];
+      out[2] += o[2];
+      out[3] += o[0] * o[3] * o[3];
+      out[4] += o[4] * o[3];
     }
-    *m2s = a;
-    *pairs = b;
-    *smooth = sm;
+    out[5] = g.ntleaves;
     return 0;
   }
   DevBuf<int32_t> c;
@@ -1638,25 +1692,38 @@ static int count_work(long long *m2s, long long *pairs, long long *smooth) {
   }
   c.release();
   if (rc) return rc;
-  long long a = 0, b = 0;
   for (int i = 0; i < g.n; i++) {
-    a += h[2 * i];
-    b += h[2 * i + 1];
+    out[0] += h[2 * i];
+    out[1] += h[2 * i + 1];
   }
-  *m2s = a;
-  *pairs = b;
   return 0;
 }
 
 int fmm_count_interactions(long long *m2s, long long *pairs) {
   std::lock_guard<std::mutex> lk(g_mu);
-  long long sm = 0;
-  return count_work(m2s, pairs, &sm);
+  long long o[kWorkOut];
+  if (count_work(o)) return -1;
+  *m2s = o[0];
+  *pairs = o[1];
+  return 0;
 }
 
 int fmm_count_work(long long *m2s, long long *pairs, long long *smooth) {
   std::lock_guard<std::mutex> lk(g_mu);
-  return count_work(m2s, pairs, smooth);
+  long long o[kWorkOut];
+  if (count_work(o)) return -1;
+  *m2s = o[0];
+  *pairs = o[1];
+  *smooth = o[2];
+  return 0;
+}
+
+int fmm_count_work_detail(long long *out, int n) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  long long o[kWorkOut];
+  if (count_work(o)) return -1;
+  for (int i = 0; i < n && i < kWorkOut; i++) out[i] = o[i];
+  return 0;
 }
 
 int fmm_synchronize(void) {
@@ -1672,7 +1739,7 @@ void fmm_release(void) {
   (void)hipStreamSynchronize(g.stream);
   g.xy.release(); g.cir.release(); g.cor.release();
   g.key.release(); g.key_alt.release(); g.idx.release(); g.idx_alt.release();
-  g.hist.release(); g.hist_scan.release(); g.perm.release(); g.flag.release(); g.pos.release();
+  g.hist.release(); g.hist_scan.release(); g.perm.release(); g.lt.release(); g.lstat.release(); g.cflag.release();
   g.total.release(); g.sx.release(); g.sy.release(); g.sg.release(); g.ss.release();
   g.bbox_part.release(); g.uv_dev.release(); g.tmp_in.release(); g.tmp_out.release();
   g.nodes.release(); g.gparent.release(); g.split.release(); g.coef.release(); g.root.release();

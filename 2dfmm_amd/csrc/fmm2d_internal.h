@@ -75,11 +75,29 @@ int launch_forest_roots(const double *d_xy, int n, const int32_t *d_off, int nsy
                         int32_t *d_gparent, int nroot_slots, hipStream_t s);
 int launch_init_root(NodeRec *d_nodes, int32_t *d_gparent, const RootBox *d_root, int n,
                      int32_t *d_split, hipStream_t s);
-int launch_build_level(const uint64_t *d_key, int depth_bits, int level, const int32_t *d_split_prev,
-                       int nsplit_prev, int child_base, int num_p, NodeRec *d_nodes,
-                       int32_t *d_gparent, uint32_t *d_flag, hipStream_t s);
-int launch_compact_split(const uint32_t *d_flag, const uint32_t *d_pos, int nchild, int child_base,
-                         int32_t *d_split_next, hipStream_t s);
+// Level table of the tree build, on the device (int32): per level l the
+// number of split nodes S[l], the index of level l's first node CB[l], the
+// start of level l's split nodes in the split list OFF[l], the ticket counter
+// of level l's launch; and error flags.
+constexpr int kLtLevels = 40;
+constexpr int kLtS = 0, kLtCb = kLtLevels, kLtOff = 2 * kLtLevels, kLtTicket = 3 * kLtLevels,
+              kLtErr = 4 * kLtLevels, kLtWords = 4 * kLtLevels + 1;
+constexpr int kLtErrOverflow = 1, kLtErrDepth = 2, kLtErrLookback = 4;
+static_assert(kLtLevels >= kMaxDepth + 3, "level table");
+// Clears the level table and the two status arrays (stat_len words each); level
+// 0 = the nroots roots already in the split list, level 1 starts at nroot_slots.
+int launch_tree_levels_init(int32_t *d_lt, int nroots, int nroot_slots, uint32_t *d_stat,
+                            int stat_len, hipStream_t s);
+// Children (level `level`) of the split nodes of level-1, from the device
+// level table, and the next split list; split_bound >= the number of those
+// split nodes (sizes the grids).  Capacities: node_cap nodes (and d_cflag
+// bytes), split_cap split-list entries, stat_len compaction tiles
+// (tree_level_tiles of the children); past them the overflow flag is set.
+int launch_tree_level(const uint64_t *d_key, int depth_bits, int level, int max_level, int num_p,
+                      int split_bound, NodeRec *d_nodes, int32_t *d_gparent, int32_t *d_split,
+                      int node_cap, int split_cap, int32_t *d_lt, uint8_t *d_cflag,
+                      uint32_t *d_stat, int stat_len, hipStream_t s);
+int tree_level_tiles(int nchild);
 int launch_leaf_order(const NodeRec *d_nodes, int nnodes, const uint32_t *d_idx_sorted,
                       uint32_t *d_perm, int num_p, hipStream_t s);
 // Blobs into leaf order (d_rec: n records of scratch).
@@ -209,6 +227,9 @@ struct FmmArgs {
   // prefix sums over the nodes of the work units of smooth-level nodes (a
   // smooth cell kSmoothUnits, a small leaf its blobs + kStepUnits; nnodes + 1)
   const uint32_t *sunits;
+  // far cells listed before a burst sums them (64..kFarList; a walk step
+  // adds at most 64)
+  int far_burst;
 };
 // Work units of a smooth cell, and of a step's fixed cost (fmm_kernels.hip).
 constexpr int kSmoothUnits = 32;
@@ -233,8 +254,10 @@ int launch_minmax_reduce(const double *d_v, int n, double *d_out, hipStream_t s)
 int launch_node_first_blob(const NodeRec *d_nodes, int nnodes, int32_t *d_out, hipStream_t s);
 // One wave per target leaf; with splitting on, d_split_ctr holds two counters
 // and the continuation passes run on split_blocks x 256 threads.
+// mid (optional): an event recorded between the fused pass and the split passes.
 int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
-                    uint32_t *d_split_ctr, int split_blocks, hipStream_t s);
+                    uint32_t *d_split_ctr, int split_blocks, hipStream_t s,
+                    hipEvent_t mid = nullptr);
 
 // ---- time stepping around the evaluation (step_kernels.hip) ---------------
 int launch_drift_ab3(int n, double *d_xy, const double *d_hist, double dt, int mirror, double dm,
@@ -267,6 +290,9 @@ int launch_blob_leaf(const FmmArgs &a, int32_t *d_out, hipStream_t s);
 int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
                                uint32_t *d_counter, int nblocks, uint32_t *d_split_ctr,
                                int split_blocks, hipStream_t s);
+// Per target leaf kCountWords counts (far cells, pairs, smooth evaluations,
+// expansion terms, targets).
+constexpr int kCountWords = 5;
 int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s);
 // Accurate modes at nt arbitrary points (interleaved xy): lane per point,
 // far test with the point as a target leaf of size 0, FP64 (single tree).

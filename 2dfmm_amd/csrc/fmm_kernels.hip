@@ -1296,7 +1296,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   };
   auto far = [&](uint64_t fm, int c) {
     const int nf = __builtin_popcountll(fm);
-    if (nfar + nf > kFarList) far_flush();
+    if (nfar + nf > a.far_burst) far_flush();
     if ((fm >> lane) & 1)
       s_far[w][nfar + __builtin_amdgcn_mbcnt_hi((uint32_t)(fm >> 32),
                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)fm, 0))] = c;
@@ -1604,9 +1604,9 @@ int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *
 }
 
 // Work of the fused kernel, one thread per target leaf with the same walk:
-// out[3*leaf] = far cells (each one M2L into the leaf), out[3*leaf+1] = near
-// source-target pairs summed directly, out[3*leaf+2] = (target, smooth cell)
-// evaluations.
+// out[kCountWords*leaf + 0] = far cells (each one M2L into the leaf), + 1 = near
+// source-target pairs summed directly, + 2 = (target, smooth cell)
+// evaluations, + 3 = the leaf's expansion terms, + 4 = its targets.
 __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__restrict__ out) {
   const int li = blockIdx.x * 256 + threadIdx.x;
   if (li >= a.nleaves) return;
@@ -1657,9 +1657,12 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
       expand = false;
     }
   }
-  out[3 * li] = far;
-  out[3 * li + 1] = pairs;
-  out[3 * li + 2] = smooth;
+  int64_t *o = out + (size_t)kCountWords * li;
+  o[0] = far;
+  o[1] = pairs;
+  o[2] = smooth;
+  o[3] = a.leaf_par ? min(a.p, max(2, (int)a.leaf_par[li].x)) : a.p;
+  o[4] = cA;
 }
 
 // ---------------------------------------------------------------------------
@@ -1769,7 +1772,7 @@ int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s) {
 }
 
 int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
-                    uint32_t *d_split_ctr, int split_blocks, hipStream_t s) {
+                    uint32_t *d_split_ctr, int split_blocks, hipStream_t s, hipEvent_t mid) {
   if (args.nleaves <= 0) return 0;
   const bool split = args.split_budget > 0;
   if (split) {
@@ -1784,6 +1787,7 @@ int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
   else
     fmm_leaf_kernel<false, false><<<grid, 64, 0, s>>>(args);
   FMM_HIP_TRY(hipGetLastError());
+  if (mid) FMM_HIP_TRY(hipEventRecord(mid, s));
   if (split) return launch_split_passes(args, mixed, d_nleaves, d_split_ctr + 1, split_blocks, s);
   return 0;
 }

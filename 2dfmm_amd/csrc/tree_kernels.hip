@@ -549,74 +549,240 @@ __device__ inline int wave_lower_bound(const uint64_t *__restrict__ key, int lo,
   return m ? lo + __builtin_ctzll(m) : hi;
 }
 
+// Two launches per level, no host round trip: the counts of a level stay on
+// the device (the level table `lt`, see fmm2d_internal.h) and the next launch
+// reads them.
+//   children: one wave per split node of level l-1 (grid-stride over the
+//             device count): its three digit boundaries by wave searches, the
+//             four children's centres by quadtree.c:61-79, leaf iff count <
+//             num_p (quadtree.c:197-245), and one split flag per child;
+//   compact:  the split children, in Morton order, into the next split list:
+//             tiles of 4096 flags, a single-pass scan with decoupled
+//             look-back (each tile publishes its count, flag A, then its
+//             inclusive prefix, flag P, once the counts of its predecessors
+//             back to the first P are summed; one wave reads 64 predecessors
+//             per step).  A tile's predecessors hold earlier tickets of
+//             running workgroups, so every wait ends; it is also bounded
+//             (kLookbackSpins), past which the error flag is set and the host
+//             reports a failed build.
 // The range start of an internal node is kept in `first` until its own
-// children are built (then `first` becomes the child index).  One wave per
-// split node: its three digit boundaries by wave searches, lanes 0-3 write
-// the four children.
-__global__ __launch_bounds__(256) void build_level_kernel(const uint64_t *__restrict__ key,
-                                                          int shift, const int32_t *__restrict__ split_prev,
-                                                          int nsplit, int child_base, int num_p,
-                                                          NodeRec *__restrict__ nodes,
-                                                          int32_t *__restrict__ gparent,
-                                                          uint32_t *__restrict__ flag) {
-  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+// children are built (then `first` becomes the child index).
+constexpr uint32_t kStatAgg = 1u << 30, kStatPre = 2u << 30, kStatVal = (1u << 30) - 1u;
+constexpr int64_t kLookbackSpins = int64_t(1) << 24;
+constexpr int kCompactTile = 4096;  // flags per compaction tile (256 threads x 16)
+
+__device__ inline uint32_t ld_acquire(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_release(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void tree_children_kernel(
+    const uint64_t *__restrict__ key, int shift, int level, int max_level, int num_p,
+    NodeRec *__restrict__ nodes, int32_t *__restrict__ gparent, const int32_t *__restrict__ split,
+    int node_cap, int split_cap, int32_t *__restrict__ lt, uint8_t *__restrict__ cflag) {
   const int lane = threadIdx.x & 63;
-  if (j >= nsplit) return;
-  const int par = split_prev[j];
-  const NodeRec P = nodes[par];
-  const int lo = P.first, hi = lo + node_count(P.cnt);
-  const int b2 = wave_lower_bound(key, lo, hi, shift, 2u, lane);
-  const int b1 = wave_lower_bound(key, lo, b2, shift, 1u, lane);
-  const int b3 = wave_lower_bound(key, b2, hi, shift, 3u, lane);
-  const int cb = child_base + 4 * j;
-  if (lane < 4) {
-    const int q = lane;
-    const int bq = q == 0 ? lo : q == 1 ? b1 : q == 2 ? b2 : b3;
-    const int be = q == 0 ? b1 : q == 1 ? b2 : q == 2 ? b3 : hi;
-    const double h = 0.25 * P.region;
-    NodeRec c;
-    // quadtree.c:61-79
-    c.cx = (q & 1) ? P.cx + h : P.cx - h;
-    c.cy = (q < 2) ? P.cy + h : P.cy - h;
-    c.region = 0.5 * P.region;
-    const int cnt = be - bq;
-    c.first = bq;
-    const bool split = cnt >= num_p;
-    c.cnt = split ? (cnt | kInternalBit) : cnt;
-    nodes[cb + q] = c;
-    flag[4 * j + q] = split ? 1u : 0u;
+  if (lt[kLtErr] != 0) return;
+  const int Sp = lt[kLtS + level - 1];
+  if (Sp == 0) return;
+  const int cb = lt[kLtCb + level], offn = lt[kLtOff + level], offp = lt[kLtOff + level - 1];
+  const bool first_thread = blockIdx.x == 0 && threadIdx.x == 0;
+  if (level > max_level) {
+    if (first_thread) atomicOr(&lt[kLtErr], kLtErrDepth);
+    return;
   }
-  if (lane == 0) {
-    gparent[cb >> 2] = par;
-    nodes[par].first = cb;
+  if ((int64_t)cb + 4 * (int64_t)Sp > node_cap || (int64_t)offn + 4 * (int64_t)Sp > split_cap) {
+    if (first_thread) atomicOr(&lt[kLtErr], kLtErrOverflow);
+    return;
+  }
+  for (int j = blockIdx.x * 4 + (threadIdx.x >> 6); j < Sp; j += gridDim.x * 4) {
+    const int par = split[offp + j];
+    const NodeRec P = nodes[par];
+    const int lo = P.first, hi = lo + node_count(P.cnt);
+    const int b2 = wave_lower_bound(key, lo, hi, shift, 2u, lane);
+    const int b1 = wave_lower_bound(key, lo, b2, shift, 1u, lane);
+    const int b3 = wave_lower_bound(key, b2, hi, shift, 3u, lane);
+    const int c0 = cb + 4 * j;
+    if (lane < 4) {
+      const int q = lane;
+      const int bq = q == 0 ? lo : q == 1 ? b1 : q == 2 ? b2 : b3;
+      const int be = q == 0 ? b1 : q == 1 ? b2 : q == 2 ? b3 : hi;
+      const double h = 0.25 * P.region;
+      NodeRec c;
+      // quadtree.c:61-79
+      c.cx = (q & 1) ? P.cx + h : P.cx - h;
+      c.cy = (q < 2) ? P.cy + h : P.cy - h;
+      c.region = 0.5 * P.region;
+      const int cnt = be - bq;
+      c.first = bq;
+      const bool sp = cnt >= num_p;
+      c.cnt = sp ? (cnt | kInternalBit) : cnt;
+      nodes[c0 + q] = c;
+      cflag[4 * j + q] = sp ? 1 : 0;
+    }
+    if (lane == 0) {
+      gparent[c0 >> 2] = par;
+      nodes[par].first = c0;
+    }
   }
 }
 
-int launch_build_level(const uint64_t *d_key, int depth_bits, int level, const int32_t *d_split_prev,
-                       int nsplit_prev, int child_base, int num_p, NodeRec *d_nodes,
-                       int32_t *d_gparent, uint32_t *d_flag, hipStream_t s) {
+__global__ __launch_bounds__(256) void tree_compact_kernel(int level, int max_level,
+                                                           int32_t *__restrict__ split,
+                                                           int32_t *__restrict__ lt,
+                                                           const uint8_t *__restrict__ cflag,
+                                                           uint32_t *__restrict__ stat,
+                                                           uint32_t *__restrict__ stat_next,
+                                                           int stat_len) {
+  __shared__ uint32_t lds4[4];
+  __shared__ int s_ticket;
+  __shared__ uint32_t s_prefix;
+  const int lane = threadIdx.x & 63;
+  // the next level's status words start at zero (the previous level's
+  // compaction, the last user of that array, has finished)
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < stat_len; i += gridDim.x * 256) stat_next[i] = 0u;
+  if (lt[kLtErr] != 0) return;
+  const int Sp = lt[kLtS + level - 1];
+  const int cb = lt[kLtCb + level], offn = lt[kLtOff + level];
+  const int nflag = 4 * Sp;
+  const int ntiles = (nflag + kCompactTile - 1) / kCompactTile;
+  if (Sp == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      lt[kLtS + level] = 0;
+      lt[kLtOff + level + 1] = offn;
+      lt[kLtCb + level + 1] = cb;
+    }
+    return;
+  }
+  if (ntiles > stat_len) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&lt[kLtErr], kLtErrOverflow);
+    return;
+  }
+  for (;;) {
+    if (threadIdx.x == 0) s_ticket = atomicAdd(&lt[kLtTicket + level], 1);
+    __syncthreads();
+    const int t = s_ticket;
+    __syncthreads();
+    if (t >= ntiles) break;
+    const int base = t * kCompactTile + threadIdx.x * 16;
+    uint32_t f[16];
+    uint32_t mine = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      f[k] = (base + k < nflag) ? cflag[base + k] : 0u;
+      mine += f[k];
+    }
+    uint32_t agg = 0;
+    const uint32_t ex = block_excl_scan_256(mine, lds4, &agg);
+    if (threadIdx.x < 64) {
+      // wave 0: publish, then look back 64 tiles per step
+      uint32_t prefix = 0;
+      if (t == 0) {
+        if (lane == 0) st_release(&stat[0], kStatPre | agg);
+      } else {
+        if (lane == 0) st_release(&stat[t], kStatAgg | agg);
+        int k = t - 1;
+        int64_t spins = 0;
+        for (;;) {
+          const int idx = k - lane;
+          const uint32_t v = idx >= 0 ? ld_acquire(&stat[idx]) : kStatPre;
+          const uint32_t fl = v & ~kStatVal;
+          const uint64_t mp = __builtin_amdgcn_ballot_w64(fl == kStatPre);
+          const int fp = mp ? __builtin_ctzll(mp) : 64;  // nearest P
+          const uint64_t upto = fp >= 63 ? ~0ull : ((2ull << fp) - 1ull);
+          const uint64_t mx = __builtin_amdgcn_ballot_w64(fl == 0u) & upto;
+          if (mx) {
+            if (++spins > kLookbackSpins) {
+              if (lane == 0) atomicOr(&lt[kLtErr], kLtErrLookback);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+          uint32_t val = (lane <= fp) ? (v & kStatVal) : 0u;
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
+          prefix += val;
+          if (fp < 64) break;
+          k -= 64;
+        }
+        if (lane == 0) st_release(&stat[t], kStatPre | ((prefix + agg) & kStatVal));
+      }
+      if (lane == 0) {
+        s_prefix = prefix;
+        if (t == ntiles - 1) {
+          const int total = (int)(prefix + agg);
+          lt[kLtS + level] = total;
+          lt[kLtOff + level + 1] = offn + total;
+          lt[kLtCb + level + 1] = cb + 4 * Sp;
+          if (total > 0 && level == max_level) atomicOr(&lt[kLtErr], kLtErrDepth);
+        }
+      }
+    }
+    __syncthreads();
+    uint32_t pos = s_prefix + ex;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      if (f[k]) split[offn + (int)pos] = cb + base + k;
+      pos += f[k];
+    }
+    __syncthreads();
+  }
+}
+
+// lt: level 0 holds the nroots split roots (their list is already at
+// split[0..nroots)), level 1's children start at nroot_slots; tickets, error
+// flag and both status arrays cleared.
+__global__ __launch_bounds__(256) void tree_levels_init_kernel(int32_t *__restrict__ lt, int nroots,
+                                                               int nroot_slots,
+                                                               uint32_t *__restrict__ stat,
+                                                               int stat_words) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < stat_words; i += gridDim.x * 256) stat[i] = 0u;
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < kLtWords; i += 256) lt[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      lt[kLtS + 0] = nroots;
+      lt[kLtOff + 0] = 0;
+      lt[kLtOff + 1] = nroots;
+      lt[kLtCb + 1] = nroot_slots;
+    }
+  }
+}
+
+int launch_tree_levels_init(int32_t *d_lt, int nroots, int nroot_slots, uint32_t *d_stat,
+                            int stat_len, hipStream_t s) {
+  const int words = 2 * stat_len;
+  int nb = (words + 255) / 256;
+  nb = nb < 1 ? 1 : (nb > 1024 ? 1024 : nb);
+  tree_levels_init_kernel<<<nb, 256, 0, s>>>(d_lt, nroots, nroot_slots, d_stat, words);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int launch_tree_level(const uint64_t *d_key, int depth_bits, int level, int max_level, int num_p,
+                      int split_bound, NodeRec *d_nodes, int32_t *d_gparent, int32_t *d_split,
+                      int node_cap, int split_cap, int32_t *d_lt, uint8_t *d_cflag,
+                      uint32_t *d_stat, int stat_len, hipStream_t s) {
   const int shift = depth_bits - 2 * level;
-  build_level_kernel<<<(nsplit_prev + 3) / 4, 256, 0, s>>>(
-      d_key, shift, d_split_prev, nsplit_prev, child_base, num_p, d_nodes, d_gparent, d_flag);
+  // one wave per split node of level - 1 (grid-stride past 2048 workgroups)
+  int nb = (split_bound + 3) / 4;
+  nb = nb < 1 ? 1 : (nb > 2048 ? 2048 : nb);
+  tree_children_kernel<<<nb, 256, 0, s>>>(d_key, shift, level, max_level, num_p, d_nodes, d_gparent,
+                                          d_split, node_cap, split_cap, d_lt, d_cflag);
+  FMM_HIP_TRY(hipGetLastError());
+  uint32_t *cur = d_stat + (size_t)(level & 1) * stat_len;
+  uint32_t *next = d_stat + (size_t)((level + 1) & 1) * stat_len;
+  int nt = (4 * split_bound + kCompactTile - 1) / kCompactTile;
+  nt = nt < 1 ? 1 : (nt > 256 ? 256 : nt);
+  tree_compact_kernel<<<nt, 256, 0, s>>>(level, max_level, d_split, d_lt, d_cflag, cur, next,
+                                         stat_len);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
 
-__global__ __launch_bounds__(256) void compact_split_kernel(const uint32_t *__restrict__ flag,
-                                                            const uint32_t *__restrict__ pos,
-                                                            int nchild, int child_base,
-                                                            int32_t *__restrict__ out) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < nchild && flag[i]) out[pos[i]] = child_base + i;
-}
-
-int launch_compact_split(const uint32_t *d_flag, const uint32_t *d_pos, int nchild, int child_base,
-                         int32_t *d_split_next, hipStream_t s) {
-  compact_split_kernel<<<(nchild + 255) / 256, 256, 0, s>>>(d_flag, d_pos, nchild, child_base,
-                                                            d_split_next);
-  FMM_HIP_TRY(hipGetLastError());
-  return 0;
-}
+int tree_level_tiles(int nchild) { return (nchild + kCompactTile - 1) / kCompactTile; }
 
 // One wave per node: a leaf's blob ids are reordered ascending by ranking.
 __global__ __launch_bounds__(256) void leaf_order_kernel(const NodeRec *__restrict__ nodes,

@@ -10,12 +10,21 @@ radius 5e-4, circulation N(0, 1/N)), seeded.
 The headline `value` is the mixed-precision mode with learned per-leaf
 order and near-field radius (FP32 pair sums accumulated in FP64, smooth
 cells summed through their Taylor coefficients, FP32-MFMA far-field
-contraction, FP64 expansions) when it meets rel. L2 <= 1e-6, else
-the mixed mode with fixed parameters; the plain FP64 mode is timed on the
-same input, and each states its error against the FP64 direct sum on
-sampled targets.  The reference-rule treecode is timed beside them.  Inputs
-are resident in HBM for `value`; the rate through host arrays (PCIe copies
-included) is reported beside it as `host_inclusive`.
+contraction, FP64 expansions) when it meets rel. L2 <= 1e-6, else the mixed
+mode with fixed parameters.  Every mode states its error against the FP64
+direct sum on 10,000 sampled targets.  Beside it:
+  * the plain FP64 mode (the library's best FP64 algorithm) and the
+    reference-rule treecode on the same input;
+  * an ablation that separates precision, smooth cells and learning (mixed
+    and learned without smooth cells);
+  * the mixed mode at p = 8 and 12 (the error ladder of the precision form);
+  * two more 4M layouts of the same generator (seeds 5 and 11), each with its
+    own error line;
+  * the batching form (256 systems in one forest, autotuned);
+  * per-kernel rooflines: HBM bytes for keys, sort, tree levels and S2M; VALU
+    flops for the near pairs; MFMA flops for the far-field contraction.
+Inputs are resident in HBM for `value`; the rate through host arrays (PCIe
+copies included) is reported as `host_inclusive`.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--p P]
 
@@ -38,7 +47,9 @@ SMOOTH_COEFS = 45
 sys.path.insert(0, ROOT)
 
 METRIC = "target velocities/s at N=4M, rel.err≤1e-6 vs FP64 direct: mixed+learned vs plain"
-PEAK = {"f64": 78.6, "f32": 157.3}  # MI355X TFLOP/s; vector and matrix peaks coincide
+# MI355X published peaks: FP64 / FP32 TFLOP/s (vector and matrix coincide), HBM GB/s
+PEAK = {"f64": 78.6, "f32": 157.3}
+HBM_GBS = 8000.0
 
 
 def log(*a):
@@ -94,10 +105,11 @@ def cpu_baseline(xy, cir, cor, nterms, num_p, theta, n_targets, seed, threads):
 
 def bench_batch(args, fmm, wl, torch):
     """Batching form: 256 independent systems (N log-uniform in [2e3, 2e5],
-    uniform or 2-cluster layout, core 1e-3, p=12) in one forest, evaluated by
-    the persistent kernel replayed from a graph; configuration by the
-    autotuner.  Reports aggregate velocities/s, and the per-system latency of
-    the one-system-at-a-time path (p50/p99) beside it."""
+    uniform or 2-cluster layout, core 1e-3, p=12) in one forest; leaf size,
+    grid shape, graph replay, chunk size and far-field burst chosen by the
+    autotuner.  Returns the aggregate velocities/s, per-system accuracy, the
+    per-system latency inside the batch (the batch's wall time: every system
+    finishes with it) and the one-system-at-a-time latencies (p50/p99)."""
     tune = importlib.import_module("2dfmm_amd.autotune")
     systems = wl.batch_systems(256, seed=4)
     off, xy, cir, cor = fmm.concat_systems(systems)
@@ -108,6 +120,7 @@ def bench_batch(args, fmm, wl, torch):
     d_cor = torch.from_numpy(cor).to(dev)
     d_uv = torch.empty((n, 2), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+    fmm.set_learned(None)
     fmm.set_mode(fmm.MODE_MIXED)
     log(f"[batch] 256 systems, N total {n} (min {np.diff(off).min()}, max {np.diff(off).max()})")
     best, table = tune.tune_forest(fmm, off, d_xy.data_ptr(), d_cir.data_ptr(), d_cor.data_ptr(),
@@ -129,9 +142,9 @@ def bench_batch(args, fmm, wl, torch):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     uv = d_uv.cpu().numpy()
-    # accuracy: sampled targets of 16 systems against their own FP64 direct sums
+    # accuracy: sampled targets of 32 systems against their own FP64 direct sums
     errs = []
-    for s in range(0, 256, 16):
+    for s in range(0, 256, 8):
         a, b = int(off[s]), int(off[s + 1])
         t = np.arange(a, b, max(1, (b - a) // 300))
         ref = fmm.direct(xy[a:b], cir[a:b], cor[a:b], xy[t], style=1)
@@ -149,22 +162,66 @@ def bench_batch(args, fmm, wl, torch):
         fmm.synchronize()
         lat.append((time.perf_counter() - t1) * 1e3)
     lat = np.array(lat)
-    out = {
+    tune.reset(fmm)
+    return {
         "metric": "aggregate target velocities/s, 256 uneven systems in one forest",
-        "value": n / (ms * 1e-3), "unit": "target velocities/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-        "data": "synthetic (256 seeded systems)",
+        "value": n / (ms * 1e-3), "unit": "target velocities/s", "ms_per_step": ms,
+        "steps": args.steps, "warmup": args.warmup, "dtype": "f32+f64",
         "config": {"workload": "batch_systems_256", "n_blobs": n, "mode": "mixed", "p": 12,
                    "core_radius": 1e-3, "autotuned": best},
         "autotune_table": [[c, t] for c, t in table],
-        "accuracy": {"vs": "per-system FP64 direct sum", "worst_rel_L2": max(errs),
-                     "median_rel_L2": float(np.median(errs))},
-        "one_system_at_a_time": {"total_ms": float(lat.sum()), "p50_ms": float(np.percentile(lat, 50)),
+        "accuracy": {"vs": "per-system FP64 direct sum (32 systems, ~300 targets each)",
+                     "worst_rel_L2": max(errs), "median_rel_L2": float(np.median(errs))},
+        "latency_in_batch_ms": {"p50": ms, "p99": ms,
+                                "note": "every system's velocities are ready when the batch ends"},
+        "one_system_at_a_time": {"total_ms": float(lat.sum()),
+                                 "p50_ms": float(np.percentile(lat, 50)),
                                  "p99_ms": float(np.percentile(lat, 99))},
-        "batched_latency_ms": ms,
     }
-    print(json.dumps(out), flush=True)
+
+
+def per_kernel(res, n, p, fmm_mode):
+    """Per-kernel rooflines of one mode from its phase times and work counts:
+    HBM-bound build kernels against 8 TB/s (bytes by the formulas of the
+    algorithmic work), the evaluation's near pairs on the vector ALU and its
+    far-field contraction on MFMA against the FP32 (mixed) or FP64 peak."""
+    ph, w, tree = res["phases_ms"], res["work"], res["tree"]
+    out = []
+
+    def hbm(name, kernels, nbytes, ms, what):
+        if ms is None or ms <= 0:
+            return
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out.append({"kernel": kernels, "phase": name, "bound": "hbm", "bytes": nbytes,
+                    "bytes_per": what, "ms": ms, "achieved": gbs, "peak": HBM_GBS, "unit": "GB/s",
+                    "frac": gbs / HBM_GBS})
+
+    hbm("bbox_keys", "bbox_partial/final + keygen", 44 * n, ph.get("bbox_keys"),
+        "16 B read (bbox) + 16 B read, 12 B written (keys) per blob")
+    hbm("radix_sort", "rs_hist + scan + rs_scatter", 4 * 24 * n, ph.get("radix_sort"),
+        "4 passes of 8-bit digits (32 key bits) x 24 B per blob")
+    hbm("levels", "tree_children + tree_compact (one pair per level)",
+        8 * n + 32 * tree["nnodes"], ph.get("levels"),
+        "8 B of sorted key per blob + 32 B per node")
+    hbm("leaf_order_gather", "leaf_order + pack + gather", 136 * n, ph.get("leaf_order_gather"),
+        "8 B (ids) + 2 x 64 B (records read and written) per blob")
+    hbm("s2m_m2m", "s2m + m2m levels", 24 * n + 16 * p * tree["nleaves"], ph.get("s2m_m2m"),
+        "24 B per blob + 16 p B per leaf (S2M)")
+    if fmm_mode:
+        prec = res["precision"]
+        peak = PEAK[prec]
+        t = ph["evaluate"] * 1e-3
+        for name, fl, bound, note in (
+                ("p2p", 20 * w["pairs"], "valu", "20 flops per near pair"),
+                ("m2l", 8 * w["far_terms2"], "mfma", "8 p_A^2 flops per far cell (leaf's own p)"),
+                ("smooth", 8 * SMOOTH_COEFS * w["smooth"], "valu",
+                 "8 x 45 flops per (target, smooth cell)"),
+                ("l2p", 8 * w["l2p_terms"], "valu", "8 p_A flops per target")):
+            tf = fl / t / 1e12
+            out.append({"kernel": f"{name} in {res['kernel']}", "phase": "evaluate", "bound": bound,
+                        "flops": fl, "flops_per": note, "ms": ph["evaluate"], "achieved": tf,
+                        "peak": peak, "unit": "TFLOP/s", "frac": tf / peak})
+    return out
 
 
 def main():
@@ -178,10 +235,15 @@ def main():
     ap.add_argument("--leaf", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--cut", type=float, default=4.0, help="near-field radius in core radii")
-    ap.add_argument("--modes", default="learned,mixed,fmm64,treecode")
+    ap.add_argument("--modes",
+                    default="learned,mixed,fmm64,treecode,mixed_nosmooth,learned_nosmooth,"
+                            "mixed_p8,mixed_p12")
+    ap.add_argument("--layouts", default="5,11",
+                    help="more seeds of the same 4M generator, headline and mixed modes each")
     ap.add_argument("--judge-targets", type=int, default=10_000)
     ap.add_argument("--cpu-targets", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--workload", default="mixture4m", choices=["mixture4m", "batch"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device to device (a kernel of known traffic for "
@@ -211,7 +273,10 @@ def main():
     fmm = importlib.import_module("2dfmm_amd")
     wl = importlib.import_module("2dfmm_amd.workloads")
     if args.workload == "batch":
-        bench_batch(args, fmm, wl, torch)
+        out = bench_batch(args, fmm, wl, torch)
+        out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                    "data": "synthetic (256 seeded systems)"})
+        print(json.dumps(out), flush=True)
         return
 
     if args.copy_calibration:
@@ -225,61 +290,66 @@ def main():
         del a, b
 
     n = args.n
-    log(f"[bench] generating gaussian_mixture N={n} seed={args.seed}")
-    xy, cir, cor = wl.gaussian_mixture(n, seed=args.seed)
     dev = torch.device("cuda:0")
-    d_xy = torch.from_numpy(xy).to(dev)
-    d_cir = torch.from_numpy(cir).to(dev)
-    d_cor = torch.from_numpy(cor).to(dev)
-    d_uv = torch.empty((n, 2), dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
     fmm.set_timing(True)
     if args.persistent > 0:
         fmm.set_persistent(args.persistent * fmm.device_cus(), False)
     fmm.set_split(args.split)
-    fmm.set_smooth(args.smooth, args.smooth_min)
-
-    # FP64 direct sum on sampled targets (the judge)
-    idx = wl.sample_targets(n, args.judge_targets, seed=7)
-    d_t = torch.from_numpy(np.ascontiguousarray(xy[idx])).to(dev)
-    d_ref = torch.empty((len(idx), 2), dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    fmm.direct_device(n, d_xy.data_ptr(), d_cir.data_ptr(), d_cor.data_ptr(), len(idx),
-                      d_t.data_ptr(), d_ref.data_ptr(), 1)
-    fmm.synchronize()
-    log(f"[bench] FP64 direct judge on {len(idx)} targets: {time.perf_counter() - t0:.2f} s")
-    ref = d_ref.cpu().numpy()
-
-    def step():
-        fmm.make_tree_device(n, d_xy.data_ptr(), d_cir.data_ptr(), d_cor.data_ptr())
-        fmm.eval_sources_device(d_uv.data_ptr())
-
-    results = {}
     wpath = os.path.join(ROOT, "2dfmm_amd", "learned_weights.npz")
-    for mode in [m for m in args.modes.split(",") if m]:
+    have_weights = os.path.exists(wpath)
+
+    def layout(seed):
+        log(f"[bench] generating gaussian_mixture N={n} seed={seed}")
+        xy, cir, cor = wl.gaussian_mixture(n, seed=seed)
+        d = [torch.from_numpy(a).to(dev) for a in (xy, cir, cor)]
+        idx = wl.sample_targets(n, args.judge_targets, seed=7)
+        d_t = torch.from_numpy(np.ascontiguousarray(xy[idx])).to(dev)
+        d_ref = torch.empty((len(idx), 2), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fmm.direct_device(n, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), len(idx),
+                          d_t.data_ptr(), d_ref.data_ptr(), 1)
+        fmm.synchronize()
+        log(f"[bench] FP64 direct judge on {len(idx)} targets: {time.perf_counter() - t0:.2f} s")
+        return xy, cir, cor, d, idx, d_ref.cpu().numpy()
+
+    d_uv = torch.empty((n, 2), dtype=torch.float64, device=dev)
+
+    def run_mode(mode, lay):
+        """One mode on one layout: warmup, timed steps, error, work, rooflines."""
+        xy, cir, cor, d, idx, ref = lay
+        base = mode.split("_")[0]
+        p = args.p
+        smooth = args.smooth
+        for part in mode.split("_")[1:]:
+            if part.startswith("p"):
+                p = int(part[1:])
+            elif part == "nosmooth":
+                smooth = 0.0
         fmm.set_learned(None)
-        if mode == "learned" and not os.path.exists(wpath):
-            log("[bench] no trained weights; skipping the learned mode")
-            continue
-        if mode == "treecode":
+        fmm.set_smooth(smooth, args.smooth_min)
+        if base == "treecode":
             fmm.set_mode(fmm.MODE_TREECODE)
-            fmm.set_params(args.p, args.leaf, args.theta)
+            fmm.set_params(p, args.leaf, args.theta)
             prec, kernel = "f64", "parity_eval_kernel<0>"
         else:
-            mixed = mode in ("mixed", "learned")
+            mixed = base in ("mixed", "learned")
             fmm.set_mode(fmm.MODE_MIXED if mixed else fmm.MODE_FMM64)
-            fmm.set_fmm_params(args.p, args.leaf, args.theta, args.cut)
+            fmm.set_fmm_params(p, args.leaf, args.theta, args.cut)
             prec = "f32" if mixed else "f64"
             # <mixed, one core radius>: the workload has one core radius
             t = "true, true" if mixed else "false, false"
             kernel = f"fmm_leaf_kernel<{t}> + fmm_chunk_kernel<{t}> + fmm_split_sum_kernel"
-            if mode == "learned":
+            if base == "learned":
                 with np.load(wpath) as z:
                     fmm.set_learned(z["weights"], float(z["p_margin"]), float(z["cut_margin"]))
-        for w in range(args.warmup):
+
+        def step():
+            fmm.make_tree_device(n, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr())
+            fmm.eval_sources_device(d_uv.data_ptr())
+
+        for _ in range(args.warmup):
             step()
-            log(f"[bench] {mode} warmup {w + 1}/{args.warmup}: {fmm.get_timing()}")
         phases = []
         torch.cuda.synchronize()
         fmm.synchronize()
@@ -290,28 +360,40 @@ def main():
         fmm.synchronize()
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+        ph = {k: float(np.mean([q[k] for q in phases])) for k in phases[0]}
         got = d_uv.cpu().numpy()[idx]
         l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
         mx = float(np.abs(got - ref).max() / np.abs(ref).max())
-        p = args.p
-        if mode == "treecode":
-            far, pairs = fmm.count_interactions()
-            smooth = 0
-            flops = far * (8 * (p - 1) + 10) + pairs * 20
+        w = fmm.count_work_detail()
+        if base == "treecode":
+            flops = w["far"] * (8 * (p - 1) + 10) + w["pairs"] * 20
         else:
-            far, pairs, smooth = fmm.count_work()
-            # smooth cell at one target: 8 flops per complex coefficient
-            flops = far * 8 * p * p + pairs * 20 + n * 8 * p + smooth * 8 * SMOOTH_COEFS
+            # far cells at each target leaf's own p; a smooth cell at one target:
+            # 8 flops per complex coefficient
+            flops = (8 * w["far_terms2"] + 20 * w["pairs"] + 8 * w["l2p_terms"]
+                     + 8 * SMOOTH_COEFS * w["smooth"])
         ach = flops / (ph["evaluate"] * 1e-3) / 1e12
-        results[mode] = {"ms_per_step": ms, "value": n / (ms * 1e-3), "rel_L2": l2, "rel_max": mx,
-                         "phases_ms": ph, "tree": fmm.tree_size(), "far": far, "pairs": pairs,
-                         "smooth": smooth,
-                         "flops": flops, "achieved_tflops": ach, "precision": prec,
-                         "kernel": kernel}
+        r = {"ms_per_step": ms, "value": n / (ms * 1e-3), "rel_L2": l2, "rel_max": mx,
+             "meets_1e-6": l2 <= 1e-6, "p": p, "smooth_eta": smooth if base != "fmm64" else 0.0,
+             "phases_ms": ph, "tree": fmm.tree_size(), "work": w, "far": w["far"],
+             "pairs": w["pairs"], "smooth": w["smooth"], "flops": flops, "achieved_tflops": ach,
+             "precision": prec, "kernel": kernel}
+        if base == "learned":
+            par = fmm.leaf_params()
+            r["mean_leaf_p"] = float(par[:, 0].mean())
+            r["mean_leaf_cut"] = float(par[:, 1].mean())
         log(f"[bench] {mode}: {ms:.2f} ms/step ({n / (ms * 1e-3):.4g} vel/s), rel_L2={l2:.3e} "
-            f"rel_max={mx:.3e}, far={far} pairs={pairs} smooth={smooth}, eval {ph['evaluate']:.2f} ms "
-            f"= {ach:.2f} TFLOP/s; phases {ph}")
+            f"rel_max={mx:.3e}, work {w}, eval {ph['evaluate']:.2f} ms = {ach:.2f} TFLOP/s; "
+            f"phases {ph}")
+        return r
+
+    main_lay = layout(args.seed)
+    results = {}
+    for mode in [m for m in args.modes.split(",") if m]:
+        if mode.startswith("learned") and not have_weights:
+            log("[bench] no trained weights; skipping the learned modes")
+            continue
+        results[mode] = run_mode(mode, main_lay)
 
     # headline: mixed + learned when it meets the 1e-6 bar, else mixed alone
     if "learned" in results and results["learned"]["rel_L2"] <= 1e-6:
@@ -323,9 +405,35 @@ def main():
     head = results[head_mode]
     plain = results.get("fmm64")
 
+    def ratio(a, b):
+        return results[b]["ms_per_step"] / results[a]["ms_per_step"] \
+            if a in results and b in results else None
+
+    ablation = {
+        "note": ("ratios of ms/step on the same layout; the plain FP64 mode is the library's best "
+                 "FP64 algorithm (smooth cells exist only in the mixed form)"),
+        "headline_vs_plain_fp64": ratio(head_mode, "fmm64"),
+        "precision_alone (mixed_nosmooth vs fmm64)": ratio("mixed_nosmooth", "fmm64"),
+        "smooth_cells (mixed vs mixed_nosmooth)": ratio("mixed", "mixed_nosmooth"),
+        "learning_with_smooth (learned vs mixed)": ratio("learned", "mixed"),
+        "learning_without_smooth (learned_nosmooth vs mixed_nosmooth)":
+            ratio("learned_nosmooth", "mixed_nosmooth"),
+        "precision_and_learning_without_smooth (learned_nosmooth vs fmm64)":
+            ratio("learned_nosmooth", "fmm64"),
+    }
+
+    layouts = {}
+    for sd in [int(s) for s in args.layouts.split(",") if s]:
+        lay = layout(sd)
+        layouts[f"seed{sd}"] = {m: run_mode(m, lay) for m in (head_mode, "mixed")
+                                if not (m == "learned" and not have_weights)}
+        del lay
+    fmm.set_smooth(args.smooth, args.smooth_min)
+
     # the same step through host arrays: make_tree copies the blobs in, the
     # velocities come back to host memory (PCIe both ways); never `value`
     host = None
+    xy, cir, cor = main_lay[0], main_lay[1], main_lay[2]
     if head_mode in ("learned", "mixed", "fmm64"):
         fmm.set_mode(fmm.MODE_FMM64 if head_mode == "fmm64" else fmm.MODE_MIXED)
         fmm.set_fmm_params(args.p, args.leaf, args.theta, args.cut)
@@ -350,6 +458,13 @@ def main():
         log(f"[bench] {head_mode} through host arrays: {hms:.2f} ms/step")
     fmm.set_learned(None)
 
+    batch = None
+    if not args.no_batch:
+        fmm.set_timing(False)
+        batch = bench_batch(args, fmm, wl, torch)
+        log(f"[bench] batch: {batch['ms_per_step']:.2f} ms/step, {batch['value']:.4g} vel/s, "
+            f"accuracy {batch['accuracy']}")
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -365,9 +480,11 @@ def main():
     traffic = None
     if args.traffic and os.path.exists(args.traffic) and head_mode != "treecode":
         with open(args.traffic) as f:
-            t = json.load(f).get(head["precision"])
+            tr = json.load(f)
+        t = tr.get(head_mode) or tr.get(head["precision"])
         if t:
             traffic = t["bytes"]
+    pk = per_kernel(head, n, head["p"], head_mode != "treecode")
     out = {
         "metric": METRIC,
         "value": head["value"],
@@ -385,17 +502,25 @@ def main():
                    "core_radius": 5e-4, "mode": head_mode, "p": args.p, "leaf": args.leaf,
                    "theta": args.theta, "cut_core_radii": args.cut, "parallelism": "single GPU",
                    "persistent_blocks_per_cu": args.persistent, "split_budget": args.split},
-        "accuracy": {"vs": "FP64 direct sum (fmm.c:121-142 rule)", "targets": len(idx),
+        "accuracy": {"vs": "FP64 direct sum (fmm.c:121-142 rule)", "targets": len(main_lay[4]),
                      "rel_L2": head["rel_L2"], "rel_max": head["rel_max"],
                      "meets_1e-6": head["rel_L2"] <= 1e-6},
         "speedup_vs_plain_fp64": (plain["ms_per_step"] / head["ms_per_step"]) if plain else None,
+        "ablation": ablation,
         "modes": results,
-        "roofline": {"kernel": head["kernel"], "bound": "mfma", "achieved": head["achieved_tflops"],
-                     "peak": peak, "unit": "TFLOP/s", "frac": head["achieved_tflops"] / peak,
-                     "traffic": traffic, "flops_per_launch": head["flops"],
-                     "avg_launch_ms": head["phases_ms"]["evaluate"]},
+        "layouts": layouts,
+        "roofline": {"kernel": head["kernel"], "bound": "valu",
+                     "achieved": head["achieved_tflops"], "peak": peak, "unit": "TFLOP/s",
+                     "frac": head["achieved_tflops"] / peak, "traffic": traffic,
+                     "flops_per_launch": head["flops"], "avg_launch_ms": head["phases_ms"]["evaluate"],
+                     "note": ("the evaluation (fused pass + continuation chunks + chunk sums); "
+                              "near pairs and smooth cells run on the FP32 vector ALU, the far "
+                              "field on FP32 MFMA; the counters show the VALU busy, so the bound "
+                              "is the vector ALU; flops count each far cell at its leaf's own p"),
+                     "per_kernel": pk},
         "cpu_baseline": cpu,
         "host_inclusive": host,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

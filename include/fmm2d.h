@@ -125,6 +125,10 @@ int fmm_set_persistent(int blocks, int use_graph);
  * deterministic.  0 turns splitting off.  Default 8192. */
 int fmm_set_split(int budget);
 int fmm_get_split(void);
+/* Accurate modes: far cells a target leaf lists before one MFMA burst sums
+ * them into its local expansion (64..128, default 128; a knob of the
+ * autotuner: fewer cells per burst, smaller bursts at the end of the walk). */
+int fmm_set_far_burst(int cells);
 /* Smooth cells (no reference counterpart; mixed mode with one core radius
  * sigma).  A near node whose half side is at most eta sigma and that holds at
  * least min_blobs blobs is summed for every target through a degree-8 Taylor
@@ -234,9 +238,13 @@ int fmm_direct_device_noalloc(int ns, const double *d_xy, const double *d_cir, c
 
 /* Measurement (no reference counterpart: the reference has no timers).
  * With timing on, HIP events on the library's stream bracket the phases of
- * the last build and evaluation; fmm_get_timing fills up to 5 values in ms:
+ * the last build and evaluation; fmm_get_timing fills up to 10 values in ms:
  * [0] bbox + keys, [1] radix sort, [2] tree levels + leaf order + gather,
- * [3] S2M + M2M, [4] the last evaluation kernel over all blobs. */
+ * [3] S2M + M2M and the accurate modes' set-up (smooth cells, learned
+ * parameters), [4] the last evaluation over all blobs, [5] tree levels alone,
+ * [6] leaf order + gather, [7] S2M + M2M alone, [8] the fused pass of the
+ * accurate modes, [9] their split (continuation) passes; -1 where a phase
+ * did not run. */
 int fmm_set_timing(int on);
 int fmm_get_timing(double *ms, int n);
 /* Treecode interactions of the last tree with all blobs as targets: far cells
@@ -245,6 +253,13 @@ int fmm_count_interactions(long long *m2s, long long *pairs);
 /* Same walk, with the smooth cells of the mixed form counted apart: far
  * cells, pairs summed directly, and (target, smooth cell) evaluations. */
 int fmm_count_work(long long *m2s, long long *pairs, long long *smooth);
+/* The work of that walk in detail (up to n values): [0] far cells (M2L
+ * contractions of target leaves; treecode: M2S evaluations), [1] near pairs,
+ * [2] (target, smooth cell) evaluations, [3] sum over target leaves of far
+ * cells x the leaf's terms^2 (the contractions' size), [4] sum over target
+ * leaves of targets x terms (L2P), [5] target leaves; accurate modes only for
+ * [2..5] (0 in the treecode mode). */
+int fmm_count_work_detail(long long *out, int n);
 
 /* Wait for all work of the library; release all device memory. */
 int fmm_synchronize(void);

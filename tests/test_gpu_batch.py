@@ -4,6 +4,7 @@ evaluated by one persistent launch that can be replayed from a HIP graph.
 Pins: every system's velocities equal, bit for bit, those of the
 single-system path with the same parameters; the persistent and graph-replayed
 evaluations equal the one-wave-per-leaf launch bit for bit."""
+import importlib
 import sys
 
 import numpy as np
@@ -100,3 +101,36 @@ def test_forest_accuracy(fmm, oracle):
     _say(f"  worst per-system rel_L2 (mixed, p=12): {worst:.2e}")
     # p = 12 at theta = 0.5: truncation (theta / (2 - theta))^12 = 1.9e-6 per far cell
     assert worst <= 5e-6
+
+
+def test_autotuner_knobs(fmm):
+    """The autotuner's knobs (leaf, grid, graph, chunk size, far-field burst)
+    each keep the accuracy; it returns the fastest configuration it timed and
+    leaves it applied."""
+    tune = importlib.import_module("2dfmm_amd.autotune")
+    import torch
+
+    systems = _systems(12, 63, 0.05)
+    off, xy, cir, cor = fmm.concat_systems(systems)
+    n = int(off[-1])
+    dev = torch.device("cuda:0")
+    d = [torch.from_numpy(a).to(dev) for a in (xy, cir, cor)]
+    d_uv = torch.empty((n, 2), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    fmm.set_mode(fmm.MODE_MIXED)
+    best, table = tune.tune_forest(fmm, off, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(),
+                                   d_uv.data_ptr(), p=12, leaves=(32, 64), blocks_per_cu=(0, 2),
+                                   splits=(512, 8192), bursts=(64, 128), reps=1,
+                                   sync=torch.cuda.synchronize)
+    _say(f"[autotune] best {best} of {len(table)}")
+    assert best == min(table, key=lambda t: t[1])[0]
+    assert {c["burst"] for c, _ in table} == {64, 128} and {c["split"] for c, _ in table} == {512, 8192}
+    s = 1 if off[2] > off[1] else 2
+    a, b = int(off[s]), int(off[s + 1])
+    ref = fmm.direct(xy[a:b], cir[a:b], cor[a:b], xy[a:b], style=1)
+    for cfg, _ in table:
+        tune.apply(fmm, cfg)
+        fmm.make_forest(systems)
+        got = fmm.eval_sources()[a:b]
+        assert rel_err(got, ref)[0] <= 5e-6, cfg
+    tune.reset(fmm)
