@@ -69,6 +69,7 @@ lib.fmm_make_forest_device.argtypes = [C.c_int, _ip, _vp, _vp, _vp]
 lib.fmm_set_persistent.argtypes = [C.c_int, C.c_int]
 lib.fmm_set_split.argtypes = [C.c_int]
 lib.fmm_set_far_burst.argtypes = [C.c_int]
+lib.fmm_set_l2l.argtypes = [C.c_int]
 lib.fmm_get_split.argtypes = []
 lib.fmm_set_smooth.argtypes = [C.c_double, C.c_int]
 lib.fmm_get_smooth.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -317,6 +318,14 @@ def set_persistent(blocks: int, graph: bool = False) -> None:
     _check(lib.fmm_set_persistent(int(blocks), 1 if graph else 0), "set_persistent")
 
 
+def set_l2l(min_children) -> None:
+    """Local expansions inherited from the parent (accurate modes): parents
+    with at least min_children target-leaf children (True = 2, False = off)."""
+    if min_children is True:
+        min_children = 2
+    _check(lib.fmm_set_l2l(int(min_children)), "set_l2l")
+
+
 def set_far_burst(cells: int) -> None:
     _check(lib.fmm_set_far_burst(int(cells)), "set_far_burst")
 
@@ -466,7 +475,8 @@ def count_interactions():
     return a.value, b.value
 
 
-WORK_KEYS = ("far", "pairs", "smooth", "far_terms2", "l2p_terms", "target_leaves")
+WORK_KEYS = ("far", "pairs", "smooth", "far_terms2", "l2p_terms", "target_leaves",
+             "parent_far", "parent_far_terms2", "parents", "l2l_shifts")
 
 
 def count_work_detail() -> dict:

@@ -118,6 +118,9 @@ struct Context {
   // splitting of target leaves with much near work (0: off)
   int split_budget = 8192;
   int far_burst = 128;  // far cells per MFMA burst (64..128)
+  int l2l = 2;          // parent local expansions for parents of >= l2l target leaves (0: off)
+  DevBuf<double2> plocal;
+  DevBuf<int32_t> nodeli;
   DevBuf<SplitCk> ck;
   DevBuf<int32_t> nfirst;
   DevBuf<int2> leaf_ck;
@@ -419,7 +422,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     if (g.leaves.ensure(n, s) || g.lat.ensure(n, s) || g.lflag.ensure(n, s) || g.lpos.ensure(n, s) ||
         g.loc.ensure(n, s) || g.loc64.ensure(n, s) || g.smax.ensure(2, s))
       return -1;
-    if (g.ntl_dev.ensure(1, s) || g.counter.ensure(1, s)) return -1;
+    if (g.ntl_dev.ensure(2, s) || g.counter.ensure(1, s)) return -1;
     if (launch_leaf_list(g.nodes.p, g.nnodes, n, g.lat.p, g.lflag.p, g.lpos.p, g.leaves.p,
                          g.ntl_dev.p, s))
       return -1;
@@ -438,6 +441,10 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
                           g.loc.p, g.loc64.p, g.sw_h, s))
       return -1;
     if (g.nfirst.ensure(g.nnodes, s) || launch_node_first_blob(g.nodes.p, g.nnodes, g.nfirst.p, s))
+      return -1;
+    // L2L: node -> target-leaf index, sibling groups, parent expansion slots
+    if (g.nodeli.ensure(g.nnodes, s) || g.plocal.ensure((size_t)(g.nnodes / 4 + 1) * 16, s) ||
+        launch_l2l_setup(g.leaves.p, g.ntleaves, g.nodeli.p, g.ntl_dev.p + 1, g.nnodes / 4, s))
       return -1;
     // smooth cells: mixed form with one core radius only
     if (g.t_mode == FMM2D_MODE_MIXED && g.sw_h > 0.0 && g.smooth_eta > 0.0) {
@@ -531,6 +538,11 @@ static FmmArgs fmm_args(double *d_uv) {
   a.srange = g.srange.p;
   a.sunits = g.sunits.p;
   a.far_burst = g.far_burst;
+  a.l2l = (g.t_mode != FMM2D_MODE_TREECODE && g.plocal.p) ? g.l2l : 0;
+  a.plocal = g.plocal.p;
+  a.nodeli = g.nodeli.p;
+  a.ngroups = g.ntl_dev.p ? g.ntl_dev.p + 1 : nullptr;
+  a.ngroups_h = g.nnodes / 4;
   return a;
 }
 
@@ -587,7 +599,8 @@ static int eval_fmm_graph(const FmmArgs &a, bool mixed) {
   unsigned char key[sizeof g.graph_key];
   memset(key, 0, sizeof key);
   FmmArgs ka = a;
-  ka.nleaves = 0;  // the persistent kernel reads the count from the device
+  ka.nleaves = 0;    // the persistent kernel reads the count from the device
+  ka.ngroups_h = 0;  // and so does the parent-expansion pass in this path
   memcpy(key, &ka, sizeof(FmmArgs));
   key[sizeof(FmmArgs)] = mixed ? 1 : 0;
   memcpy(key + sizeof(FmmArgs) + 4, &g.persist_blocks, sizeof(int));
@@ -1147,6 +1160,16 @@ int fmm_set_far_burst(int cells) {
   return 0;
 }
 
+int fmm_set_l2l(int min_children) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (min_children < 0 || min_children > 4) {
+    set_error("fmm_set_l2l: %d (0 = off, 1..4)", min_children);
+    return -1;
+  }
+  g.l2l = min_children == 1 ? 2 : min_children;
+  return 0;
+}
+
 int fmm_get_split(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   return g.split_budget;
@@ -1649,8 +1672,11 @@ int fmm_get_timing(double *ms, int n) {
 // Work of the evaluation with all blobs as targets: out[0] far cells (M2L
 // contractions of target leaves; treecode: M2S of (target, cell)), [1] near
 // pairs, [2] (target, smooth cell) evaluations, [3] sum over target leaves
-// of far cells x terms^2, [4] sum of targets x terms (L2P), [5] target leaves.
-constexpr int kWorkOut = 6;
+// of far cells x terms^2, [4] sum of targets x terms (L2P), [5] target leaves,
+// [6] far cells of parent expansions (L2L), [7] their sum of far cells x the
+// parent's terms^2, [8] parent expansions, [9] L2L shifts (target leaves that
+// inherit one).
+constexpr int kWorkOut = 10;
 static int count_work(long long *out) {
   if (need_tree()) return -1;
   for (int i = 0; i < kWorkOut; i++) out[i] = 0;
@@ -1678,6 +1704,28 @@ static int count_work(long long *out) {
       out[4] += o[4] * o[3];
     }
     out[5] = g.ntleaves;
+    const FmmArgs a = fmm_args(nullptr);
+    if (a.l2l) {
+      const size_t ng = (size_t)(g.nnodes / 4);
+      DevBuf<int64_t> pc;
+      std::vector<int64_t> hp(3 * ng);
+      if (pc.ensure(3 * ng + 3, g.stream)) return -1;
+      rc = launch_parent_local(a, g.t_mode == FMM2D_MODE_MIXED, pc.p, split_blocks(), g.stream);
+      if (!rc && (hipMemcpyAsync(hp.data(), pc.p, sizeof(int64_t) * 3 * ng, hipMemcpyDeviceToHost,
+                                 g.stream) != hipSuccess ||
+                  hipStreamSynchronize(g.stream) != hipSuccess)) {
+        set_error("count download failed");
+        rc = -1;
+      }
+      pc.release();
+      if (rc) return rc;
+      for (size_t k = 0; k < ng; k++) {
+        out[6] += hp[3 * k];
+        out[7] += hp[3 * k] * hp[3 * k + 1] * hp[3 * k + 1];
+        out[8] += hp[3 * k + 2] > 0 ? 1 : 0;
+        out[9] += hp[3 * k + 2];
+      }
+    }
     return 0;
   }
   DevBuf<int32_t> c;
@@ -1748,6 +1796,8 @@ void fmm_release(void) {
   g.doff.release(); g.ntl_dev.release(); g.counter.release();
   g.ck.release(); g.leaf_ck.release(); g.part.release(); g.split_ctr.release();
   g.nfirst.release();
+  g.plocal.release();
+  g.nodeli.release();
   g.scoef.release();
   g.sflag.release();
   g.srange.release();

@@ -230,7 +230,25 @@ struct FmmArgs {
   // far cells listed before a burst sums them (64..kFarList; a walk step
   // adds at most 64)
   int far_burst;
+  // local expansions inherited from the parent (l2l = the fewest target-leaf
+  // children of a parent that takes part, 2..4; 0: off): per sibling group
+  // g, the parent's scaled local expansion plocal[16 g ...]; nodeli[node] =
+  // index in `leaves` of a target leaf; *ngroups = sibling groups (nnodes / 4)
+  int l2l;
+  double2 *plocal;
+  const int32_t *nodeli;
+  const uint32_t *ngroups;
+  int ngroups_h;  // the same count on the host (sizes a grid; 0: read it on the device only)
 };
+// Node -> leaf-list index of the target leaves, and the group count on the
+// device (the inputs of the parent expansions).
+int launch_l2l_setup(const int32_t *d_leaves, int nleaves, int32_t *d_nodeli,
+                     uint32_t *d_ngroups, int ngroups, hipStream_t s);
+// Parent local expansions of the L2L form (no-op with l2l off); d_pcount
+// (optional, 3 per sibling group) gets each group's far cells, the parent
+// expansion's terms and its target-leaf children.  nblocks x 256 threads.
+int launch_parent_local(const FmmArgs &args, bool mixed, int64_t *d_pcount, int nblocks,
+                        hipStream_t s);
 // Work units of a smooth cell, and of a step's fixed cost (fmm_kernels.hip).
 constexpr int kSmoothUnits = 32;
 constexpr int kStepUnits = 8;

@@ -29,6 +29,8 @@
 // L2P accumulation stay in FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fmm2d_internal.h"
 
 namespace fmm2d {
@@ -1034,18 +1036,162 @@ __device__ inline bool far_cell(double inv_theta, double RA, double rcut, const 
 // entries per level, which kStackMargin covers.
 constexpr int kStack = 384;
 constexpr int kStackMargin = 96;
-template <bool SMOOTH, typename FarFn, typename ItemFn>
+
+// ---- local expansions inherited from the parent (L2L) ----------------------
+// A far cell of target leaf A that is also far from A's parent P is summed
+// once into P's local expansion (parent_local_kernel) and reaches A, and every
+// other target-leaf child of P, through the exact shift L2L.  To count each
+// cell exactly once the parent takes only cells that every one of its
+// target-leaf children would have taken as far itself: P opens a node iff all
+// of them open it (the walk's own rule), and lists an opened node's child C iff
+// it is far from P with rcut_P = the largest near radius of those children.
+// Far from P implies far from each child (a child's centre is R_P / 2 from
+// P's, R_A = R_P / 2 and theta <= 1), so every child, whose walk meets C below
+// a node all of them opened, would have listed it: its walk skips such a C
+// ("covered") instead.  Stack entries carry the flag "the parent opened this
+// node" in bit 31.  Only parents with at least two target-leaf children take
+// part.
+constexpr int kCovBit = int(0x80000000u);
+// The parent's box and the target-leaf siblings of A, kept in the wave's LDS
+// (read only where a walk step meets a node the parent opened).
+struct ParentBox {
+  double npx, npy, nregion, RP, rcutP;
+  double sx[4], sy[4], srcut[4];
+};
+struct ParentCtx {
+  bool on = false;
+  int g = 0;               // the sibling group of A (the parent's expansion slot)
+  int pP = 0;              // terms of the parent's expansion
+  uint32_t sib = 0;        // A's other target-leaf siblings (mask over q)
+  const ParentBox *box = nullptr;
+};
+
+// Near radius and terms of target leaf li.
+__device__ inline void leaf_rcut_p(const FmmArgs &a, int li, double &rcut, int &p) {
+  rcut = a.rcut;
+  p = a.p;
+  if (a.leaf_par) {
+    const float2 lp = a.leaf_par[li];
+    p = min(a.p, max(2, (int)lp.x));
+    rcut = (double)lp.y * a.smax;
+  }
+}
+
+// The target-leaf children of sibling group g (mask over q), their near radii
+// and terms (only those of the mask are set).
+__device__ inline uint32_t group_targets(const FmmArgs &a, int g, NodeRec C[4], double rc[4],
+                                         int pq[4]) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    C[q] = ldro(a.nodes + 4 * g + q);
+    rc[q] = 0.0;
+    pq[q] = 0;
+    if (!node_internal(C[q].cnt) && node_count(C[q].cnt) > 0) {
+      m |= 1u << q;
+      leaf_rcut_p(a, rfl(ldro(a.nodeli + 4 * g + q)), rc[q], pq[q]);
+    }
+  }
+  return m;
+}
+
+__device__ inline ParentCtx parent_ctx(const FmmArgs &a, int li, int lane, ParentBox *box) {
+  ParentCtx pc;
+  if (!a.l2l) return pc;
+  const int iA = rfl(ldro(a.leaves + li));
+  const int g = iA >> 2;
+  const int par = rfl(ldro(a.gparent + g));
+  if (par < 0) return pc;
+  NodeRec C[4];
+  double rc[4];
+  int pq[4];
+  const uint32_t m = group_targets(a, g, C, rc, pq);
+  if (__builtin_popcount(m) < a.l2l) return pc;
+  pc.on = true;
+  pc.g = g;
+  pc.sib = m & ~(1u << (iA & 3));
+  pc.box = box;
+  const NodeRec NP = ldro(a.nodes + par);
+  double rcutP = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    if ((m >> q) & 1) {
+      rcutP = fmax(rcutP, rc[q]);
+      pc.pP = max(pc.pP, pq[q]);
+    }
+  }
+  if (lane == 0) {
+    box->npx = NP.cx;
+    box->npy = NP.cy;
+    box->nregion = NP.region;
+    box->RP = NP.region * 0.70710678118654752;
+    box->rcutP = rcutP;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      box->sx[q] = C[q].cx;
+      box->sy[q] = C[q].cy;
+      box->srcut[q] = rc[q];
+    }
+  }
+  wave_sync();
+  return pc;
+}
+
+__device__ inline NodeRec parent_node(const ParentCtx &pc) {
+  NodeRec N = {};
+  N.cx = pc.box->npx;
+  N.cy = pc.box->npy;
+  N.region = pc.box->nregion;
+  return N;
+}
+
+// Whether a leaf box of half diagonal RA at (x, y) with near radius rcut opens
+// internal node c (the walk's rule: not far, not a smooth cell, not a whole
+// near subtree taken as one item).
+__device__ inline bool leaf_opens(const FmmArgs &a, bool smooth_on, double inv_theta, double RA,
+                                  double rcut, double x, double y, double region, int c,
+                                  const NodeRec &C) {
+  NodeRec B = {};
+  B.cx = x;
+  B.cy = y;
+  B.region = region;
+  if (far_cell(inv_theta, RA, rcut, B, C)) return false;
+  if (smooth_on && smooth_cell(a, C)) return false;
+  if (whole_near(inv_theta, RA, rcut, B, C) && (!smooth_on || a.srange[c].x != -2)) return false;
+  return true;
+}
+
+// The walk of one target leaf, level-parallel: `stack` (this wave's LDS,
+// kStack entries) holds the first-child index of every open node still to be
+// expanded.  A step takes up to 16 of them (the latest pushed) and classifies
+// their 64 children at once, lane = child: far cells go to far(mask, c) (the
+// lanes of `mask` hold one each) and near items to item(c, C, kind) in lane
+// order; open children are pushed.
+// item returning false ends the walk.  The order depends on the tree and the
+// target leaf only, so every pass over a leaf (fused, continuation, inline)
+// meets the same items in the same order (cells covered by the parent are far
+// cells, never items).  Pops shrink to one node per step near the top of the
+// stack; a depth-first descent then adds at most 3 entries per level, which
+// kStackMargin covers.  With L2L, children of nodes the parent opened that are
+// far from the parent are skipped (the parent's expansion has them).
+template <bool SMOOTH, bool L2L, typename FarFn, typename ItemFn>
 __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, int *stack,
-                                 FarFn &&far, ItemFn &&item) {
+                                 FarFn &&far, ItemFn &&item, const ParentCtx &pc) {
   const bool smooth_on = SMOOTH && a.smooth_region > 0.0;
+  const bool cov_on = L2L && pc.on;
   const int root = rfl(root_of(a, L.NA.first));
   int ns = 1;
-  if (lane == 0) stack[0] = ldro(a.nodes + root).first;
+  if (lane == 0) stack[0] = ldro(a.nodes + root).first | (cov_on ? kCovBit : 0);
   wave_sync();
   while (ns > 0) {
     const int k = min(min(16, ns), max(1, (kStack - kStackMargin - ns) / 3));
     int c = -1;
-    if ((lane >> 2) < k) c = stack[ns - k + (lane >> 2)] + (lane & 3);
+    bool cov = false;
+    if ((lane >> 2) < k) {
+      const int e = stack[ns - k + (lane >> 2)];
+      cov = e < 0;
+      c = (e & 0x7fffffff) + (lane & 3);
+    }
     wave_sync();
     ns -= k;
     NodeRec C = {};
@@ -1053,7 +1199,9 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
     bool f = false, it = false, op = false;
     int kind = kItemLeaf;
     if (node_count(C.cnt) > 0) {
-      if (far_cell(L.inv_theta, L.RA, L.rcut, L.NA, C)) {
+      if (cov_on && cov && far_cell(L.inv_theta, pc.box->RP, pc.box->rcutP, parent_node(pc), C)) {
+        // covered by the parent's expansion
+      } else if (far_cell(L.inv_theta, L.RA, L.rcut, L.NA, C)) {
         f = true;
       } else if (smooth_on && smooth_cell(a, C)) {
         it = true;
@@ -1078,9 +1226,20 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
     const uint64_t sm = __builtin_amdgcn_ballot_w64(
         it && (kind == kItemLeaf || kind == kItemRange) && needs_series(a, L, C));
     if (op) {
+      int e = C.first;
+      if (cov_on && cov) {
+        // the parent opens C iff every target-leaf child does
+        bool all = true;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if ((pc.sib >> q) & 1)
+            all = all && leaf_opens(a, smooth_on, L.inv_theta, L.RA, pc.box->srcut[q],
+                                    pc.box->sx[q], pc.box->sy[q], L.NA.region, c, C);
+        if (all) e |= kCovBit;
+      }
       const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(om >> 32),
                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)om, 0));
-      if (ns + below < kStack) stack[ns + below] = C.first;
+      if (ns + below < kStack) stack[ns + below] = e;
     }
     ns += __builtin_popcountll(om);
     wave_sync();
@@ -1092,6 +1251,36 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
       if (!item(lane_of(c, b), node_of_lane(C, b), lane_of(kind, b), (sm >> b) & 1)) return;
     }
   }
+}
+
+// A's local expansion at the start of its walk: with L2L, the parent's
+// (scaled by s_P about c_P) shifted to A's centre,
+//   Lt_A[m] = 2^-m sum_{l=m}^{pP-1} Lt_P[l] C(l, m) t^(l-m),  t = (c_A - c_P) / s_P
+// (s_A = s_P / 2; t = (+-1/4, +-1/4) exactly), else zero.
+__device__ inline void l2l_init(const FmmArgs &a, const ParentCtx &pc, const LeafCtx &L, int lane,
+                                double2 *sloc) {
+  if (lane < kFmmMaxP) {
+    double2 v = make_double2(0.0, 0.0);
+    if (pc.on && lane < pc.pP) {
+      const double2 *lp = a.plocal + (size_t)pc.g * kFmmMaxP;
+      const ParentBox *b = pc.box;
+      const double tr = (L.NA.cx - b->npx) / b->nregion, ti = (L.NA.cy - b->npy) / b->nregion;
+      const int m = lane;
+      double sr = 0.0, si = 0.0;
+      for (int l = pc.pP - 1; l >= m; l--) {
+        const double2 q = lp[l];
+        const double b = kM2lBinom.v[m][l - m];  // C(l, m)
+        const double nr = sr * tr - si * ti + b * q.x;
+        const double ni = sr * ti + si * tr + b * q.y;
+        sr = nr;
+        si = ni;
+      }
+      const double sc = ldexp(1.0, -m);
+      v = make_double2(sr * sc, si * sc);
+    }
+    sloc[lane] = v;
+  }
+  wave_sync();
 }
 
 // Work units of tiles [0, t) of item c (nt tiles): near_units per tile; a
@@ -1172,7 +1361,7 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t sta
       }
     }
   };
-  walk_wide<SM>(a, L, lane, stack, [](uint64_t, int) {},
+  walk_wide<SM, false>(a, L, lane, stack, [](uint64_t, int) {},
                 [&](int c, const NodeRec &C, int kind, bool ser) -> bool {
     const int ki = k++;
     if (ki < k0) return true;
@@ -1182,10 +1371,102 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t sta
     const int tb = ki == k1 ? min(t1, nt) : nt;
     if (ta < tb) item_steps<MIXED, UNI>(a, L, c, C, kind, ser, ta, tb, lane, st, ax, ay, push);
     return ki < k1;
-  });
+  }, ParentCtx{});
   if constexpr (SM) {
     if (snb > 0) smooth_batch(a, L, scell, snb, lane, st, ax, ay);
   }
+}
+
+// Far cells of a box X (a target leaf, or the parent whose local expansion
+// its leaf children inherit), listed in LDS by a walk, summed in one burst
+// into X's scaled local expansion sloc[l] (l < kFmmMaxP).  Per quad of cells,
+// lane group mk (16 lanes, row i = mi) forms (s_X w)^(i+1) and
+// (-1)^(i+1) a_i w^i of cell 4q + mk once; each MFMA step then takes the real
+// and imaginary parts of two cells (K slots cell re, cell im, cell' re, cell'
+// im) from the groups that formed them (lane permutes).  The next quad's cell
+// data is loaded before this quad's arithmetic.  The burst adds its part of
+// the local expansion, Lt_l = (1/s_X) sum_k C(l+k,k) G[l][k], to sloc.
+// sXpow = s_X^(mi+1) of this lane.  Ends with a wave-level fence.
+template <bool MIXED>
+__device__ inline void far_contract(const FmmArgs &a, const NodeRec &NA, double sA, int pA,
+                                   double2 sApow, int lane, const int *far_list, int nfar,
+                                   double2 *sloc) {
+  const int mi = lane & 15, mk = lane >> 4;
+  const bool row_ok = mi < pA;
+  FarAcc<MIXED> acc;
+  const int nq = (nfar + 3) >> 2;
+  // centre and coefficient of the quad's cell, loaded a quad ahead; a lane
+  // without a cell (past the list, or row i >= p) loads node 0 and its
+  // result is dropped, so the loads and the arithmetic take no branches
+  double2 cc, ak;
+  bool ok;
+  auto fetch = [&](int q, double2 &c2, double2 &k, bool &v) {
+    const int idx = 4 * q + mk;
+    const int cell = idx < nfar ? far_list[idx] : -1;
+    v = cell >= 0 && row_ok;
+    const int cs = v ? cell : 0;
+    c2 = make_double2(a.nodes[cs].cx, a.nodes[cs].cy);
+    k = a.coef[(size_t)cs * a.p + (v ? mi : 0)];
+  };
+  fetch(0, cc, ak, ok);
+  for (int q = 0; q < nq; q++) {
+    double2 ncc = cc, nak = ak;
+    bool nok = false;
+    if (q + 1 < nq) fetch(q + 1, ncc, nak, nok);
+    // w = 1 / (dr + i di); 1 / |d|^2 from the hardware reciprocal and two
+    // Newton steps
+    const double dr = cc.x - NA.cx, di = cc.y - NA.cy;
+    const double r2 = fma(dr, dr, di * di);
+    double inv = __builtin_amdgcn_rcp(r2);
+    inv = inv * fma(-r2, inv, 2.0);
+    inv = inv * fma(-r2, inv, 2.0);
+    const double2 wv = make_double2(dr * inv, -di * inv);
+    const double2 wi = cpow16(wv, mi);                          // w^i
+    const double2 wi1 = cmul_fma(wi, wv);
+    double2 pw = make_double2(wi1.x * sApow.x, wi1.y * sApow.x);  // (s_A w)^(i+1)
+    double2 qv = cmul_fma(ak, wi);                                // a_i w^i
+    if ((mi & 1) == 0) qv = make_double2(-qv.x, -qv.y);           // (-1)^(i+1)
+    if (!ok) {
+      pw = make_double2(0.0, 0.0);
+      qv = make_double2(0.0, 0.0);
+    }
+    // operands rounded to the MFMA type before they cross lanes
+    typedef typename FarAcc<MIXED>::T FT;
+    const FT pwx = (FT)pw.x, pwy = (FT)pw.y, qvx = (FT)qv.x, qvy = (FT)qv.y;
+    const bool im = (mk & 1) != 0;
+    for (int st = 0; st < 2; st++) {
+      if (4 * q + 2 * st >= nfar) continue;  // wave-uniform
+      const int src = mi + 16 * (2 * st + (mk >> 1));
+      const FT px = __shfl(pwx, src, 64), py = __shfl(pwy, src, 64);
+      const FT qx = __shfl(qvx, src, 64), qy = __shfl(qvy, src, 64);
+      acc.add(im ? py : px, im ? -qy : qx, im ? qx : qy);
+      acc.tick();
+    }
+    cc = ncc;
+    ak = nak;
+    ok = nok;
+  }
+  acc.fold();
+  // lane holds rows acc.row(mk, r), column mi of G (real part in r0,
+  // imaginary in r1)
+  for (int r = 0; r < 4; r++) {
+    const int row = acc.row(mk, r);
+    const double bc = kM2lBinom.v[row][mi];  // C(row + mi, mi)
+    double gr = (mi < pA) ? acc.r0(r) * bc : 0.0;
+    double gi = (mi < pA) ? acc.r1(r) * bc : 0.0;
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      gr += __shfl_xor(gr, o, 64);
+      gi += __shfl_xor(gi, o, 64);
+    }
+    if (mi == 0 && row < kFmmMaxP) {
+      double2 v = sloc[row];
+      v.x += gr / sA;
+      v.y += gi / sA;
+      sloc[row] = v;
+    }
+  }
+  wave_sync();
 }
 
 // The fused pass for target leaf li (index in the leaf list), one wave.
@@ -1194,104 +1475,27 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
                                  double2 (*s_loc)[kFmmMaxP], double2 (*s_red)[64],
                                  typename StageOf<MIXED>::T (*s_stage)[stage_slots<MIXED>()],
                                  int64_t (*s_ck)[kMaxCk], int32_t (*s_cku)[kMaxCk],
-                                 int (*s_stack)[kStack], int (*s_far)[kFarList]) {
+                                 int (*s_stack)[kStack], int (*s_far)[kFarList],
+                                 ParentBox *s_pbox) {
   li = rfl(li);
   const LeafCtx L = leaf_ctx<MIXED, UNI>(a, li, lane);
   const NodeRec &NA = L.NA;
   const double sA = L.sA;
   const int pA = L.pA;
   double ax = 0.0, ay = 0.0;
-  // MFMA lane roles: row / column index i, K slot k
-  const int mi = lane & 15, mk = lane >> 4;
-  const bool row_ok = mi < pA;
-  const double2 sApow = cpow_u(make_double2(sA, 0.0), mi + 1);  // s_A^(i+1), real
+  // s_A^(i+1) of this lane's MFMA row i = lane mod 16 (real)
+  const double2 sApow = cpow_u(make_double2(sA, 0.0), (lane & 15) + 1);
 
-  // Far cells are listed in LDS (s_far) by the walk and summed in bursts, at
-  // the end or when the list is full, so the MFMA accumulators live only
-  // there.  Per quad of cells, lane group mk (16 lanes, row i = mi) forms
-  // (s_A w)^(i+1) and (-1)^(i+1) a_i w^i of cell 4q + mk once; each MFMA step
-  // then takes the real and imaginary parts of two cells (K slots cell re,
-  // cell im, cell' re, cell' im) from the groups that formed them (lane
-  // permutes).  The next quad's cell data is loaded before this quad's
-  // arithmetic.  Each burst adds its part of the local expansion,
-  // Lt_l = (1/s_A) sum_k C(l+k,k) G[l][k], to s_loc.
-  if (lane < kFmmMaxP) s_loc[w][lane] = make_double2(0.0, 0.0);
+  // The local expansion starts from the parent's, shifted to A's centre, when
+  // the parent's far cells are summed once for its target-leaf children
+  // (l2l_init); far cells of A's own are listed in LDS (s_far) by the walk and
+  // summed in bursts (far_contract), at the end or when the list is full, so
+  // the MFMA accumulators live only there.
+  const ParentCtx pc = parent_ctx(a, li, lane, &s_pbox[w]);
+  l2l_init(a, pc, L, lane, s_loc[w]);
   int nfar = 0;
   auto far_flush = [&]() {
-    FarAcc<MIXED> acc;
-    const int nq = (nfar + 3) >> 2;
-    // centre and coefficient of the quad's cell, loaded a quad ahead; a lane
-    // without a cell (past the list, or row i >= p) loads node 0 and its
-    // result is dropped, so the loads and the arithmetic take no branches
-    double2 cc, ak;
-    bool ok;
-    auto fetch = [&](int q, double2 &c2, double2 &k, bool &v) {
-      const int idx = 4 * q + mk;
-      const int cell = idx < nfar ? s_far[w][idx] : -1;
-      v = cell >= 0 && row_ok;
-      const int cs = v ? cell : 0;
-      c2 = make_double2(a.nodes[cs].cx, a.nodes[cs].cy);
-      k = a.coef[(size_t)cs * a.p + (v ? mi : 0)];
-    };
-    fetch(0, cc, ak, ok);
-    for (int q = 0; q < nq; q++) {
-      double2 ncc = cc, nak = ak;
-      bool nok = false;
-      if (q + 1 < nq) fetch(q + 1, ncc, nak, nok);
-      // w = 1 / (dr + i di); 1 / |d|^2 from the hardware reciprocal and two
-      // Newton steps
-      const double dr = cc.x - NA.cx, di = cc.y - NA.cy;
-      const double r2 = fma(dr, dr, di * di);
-      double inv = __builtin_amdgcn_rcp(r2);
-      inv = inv * fma(-r2, inv, 2.0);
-      inv = inv * fma(-r2, inv, 2.0);
-      const double2 wv = make_double2(dr * inv, -di * inv);
-      const double2 wi = cpow16(wv, mi);                          // w^i
-      const double2 wi1 = cmul_fma(wi, wv);
-      double2 pw = make_double2(wi1.x * sApow.x, wi1.y * sApow.x);  // (s_A w)^(i+1)
-      double2 qv = cmul_fma(ak, wi);                                // a_i w^i
-      if ((mi & 1) == 0) qv = make_double2(-qv.x, -qv.y);           // (-1)^(i+1)
-      if (!ok) {
-        pw = make_double2(0.0, 0.0);
-        qv = make_double2(0.0, 0.0);
-      }
-      // operands rounded to the MFMA type before they cross lanes
-      typedef typename FarAcc<MIXED>::T FT;
-      const FT pwx = (FT)pw.x, pwy = (FT)pw.y, qvx = (FT)qv.x, qvy = (FT)qv.y;
-      const bool im = (mk & 1) != 0;
-      for (int st = 0; st < 2; st++) {
-        if (4 * q + 2 * st >= nfar) continue;  // wave-uniform
-        const int src = mi + 16 * (2 * st + (mk >> 1));
-        const FT px = __shfl(pwx, src, 64), py = __shfl(pwy, src, 64);
-        const FT qx = __shfl(qvx, src, 64), qy = __shfl(qvy, src, 64);
-        acc.add(im ? py : px, im ? -qy : qx, im ? qx : qy);
-        acc.tick();
-      }
-      cc = ncc;
-      ak = nak;
-      ok = nok;
-    }
-    acc.fold();
-    // lane holds rows acc.row(mk, r), column mi of G (real part in r0,
-    // imaginary in r1)
-    for (int r = 0; r < 4; r++) {
-      const int row = acc.row(mk, r);
-      const double bc = kM2lBinom.v[row][mi];  // C(row + mi, mi)
-      double gr = (mi < pA) ? acc.r0(r) * bc : 0.0;
-      double gi = (mi < pA) ? acc.r1(r) * bc : 0.0;
-#pragma unroll
-      for (int o = 8; o > 0; o >>= 1) {
-        gr += __shfl_xor(gr, o, 64);
-        gi += __shfl_xor(gi, o, 64);
-      }
-      if (mi == 0 && row < kFmmMaxP) {
-        double2 v = s_loc[w][row];
-        v.x += gr / sA;
-        v.y += gi / sA;
-        s_loc[w][row] = v;
-      }
-    }
-    wave_sync();
+    far_contract<MIXED>(a, NA, sA, pA, sApow, lane, s_far[w], nfar, s_loc[w]);
     nfar = 0;
   };
   auto far = [&](uint64_t fm, int c) {
@@ -1376,7 +1580,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
     return true;
   };
   int *stack = s_stack[w];
-  walk_wide<SM>(a, L, lane, stack, far, item);
+  walk_wide<SM, true>(a, L, lane, stack, far, item, pc);
   if constexpr (SM) {
     if (snb > 0) smooth_batch(a, L, scell, snb, lane, s_stage[w], ax, ay);
   }
@@ -1477,10 +1681,165 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fm
   __shared__ int s_stack[1][kStack];
   __shared__ int s_far[1][kFarList];
   __shared__ int32_t s_cku[1][kMaxCk];
+  __shared__ ParentBox s_pbox[1];
   const int li = xcd_leaf(blockIdx.x, a.nleaves);
   if (li >= a.nleaves) return;
   fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck, s_cku, s_stack,
-                               s_far);
+                               s_far, s_pbox);
+}
+
+// Parent-level far field (L2L; see ParentCtx): one wave per sibling group g
+// whose parent has at least two target-leaf children.  The parent's walk
+// opens a node iff every one of those children opens it and lists an opened
+// node's child iff it is far from the parent (R_P, the largest near radius of
+// the children); one far_contract with the largest order of the children puts
+// the parent's scaled local expansion into plocal[g].  Grid-stride over the
+// groups, their number read from the device (a launch that can be captured
+// once and replayed on rebuilt trees).  pcount (optional): per group its far
+// cells, the parent expansion's terms and its target-leaf children (zeros for a
+// group without a parent expansion).
+template <bool MIXED, bool UNI, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void parent_local_kernel(
+    FmmArgs a, int64_t *__restrict__ pcount) {
+  __shared__ double2 s_loc[W][kFmmMaxP];
+  __shared__ int s_stack[W][kStack];
+  __shared__ int s_far[W][kFarList];
+  const int w = W == 1 ? 0 : (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ng = (int)*a.ngroups;
+  const bool smooth_on = MIXED && UNI && a.smooth_region > 0.0;
+  const double inv_theta = 1.0 / a.theta;
+  for (int g = blockIdx.x * W + w; g < ng; g += gridDim.x * W) {
+    const int par = rfl(ldro(a.gparent + g));
+    NodeRec C4[4];
+    double rc[4];
+    int pq[4];
+    const uint32_t m = par >= 0 ? group_targets(a, g, C4, rc, pq) : 0u;
+    if (__builtin_popcount(m) < a.l2l) {
+      if (pcount && lane == 0) pcount[3 * (size_t)g] = pcount[3 * (size_t)g + 1] = pcount[3 * (size_t)g + 2] = 0;
+      continue;
+    }
+    const NodeRec NP = ldro(a.nodes + par);
+    const double RP = NP.region * 0.70710678118654752;
+    double rcutP = 0.0;
+    int pP = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if ((m >> q) & 1) {
+        rcutP = fmax(rcutP, rc[q]);
+        pP = max(pP, pq[q]);
+      }
+    }
+    // the children of one parent share a region; the first target leaf's first
+    // blob names the tree (forests)
+    const int q0 = __builtin_ctz(m);
+    const double region = 0.5 * NP.region, RA = region * 0.70710678118654752;
+    const int first0 = (q0 == 0) ? C4[0].first : (q0 == 1) ? C4[1].first : (q0 == 2) ? C4[2].first
+                                                                                       : C4[3].first;
+    const double2 sPpow = cpow_u(make_double2(NP.region, 0.0), (lane & 15) + 1);
+    if (lane < kFmmMaxP) s_loc[w][lane] = make_double2(0.0, 0.0);
+    int *stack = s_stack[w];
+    const int root = rfl(root_of(a, first0));
+    int ns = 1, nfar = 0;
+    int64_t total = 0;
+    if (lane == 0) stack[0] = ldro(a.nodes + root).first;
+    wave_sync();
+    while (ns > 0) {
+      const int k = min(min(16, ns), max(1, (kStack - kStackMargin - ns) / 3));
+      int c = -1;
+      if ((lane >> 2) < k) c = stack[ns - k + (lane >> 2)] + (lane & 3);
+      wave_sync();
+      ns -= k;
+      NodeRec C = {};
+      if (c >= 0) C = a.nodes[c];
+      bool f = false, op = false;
+      if (node_count(C.cnt) > 0) {
+        if (far_cell(inv_theta, RP, rcutP, NP, C)) {
+          f = true;
+        } else if (node_internal(C.cnt)) {
+          bool all = true;
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if ((m >> q) & 1)
+              all = all && leaf_opens(a, smooth_on, inv_theta, RA, rc[q], C4[q].cx, C4[q].cy,
+                                      region, c, C);
+          op = all;
+        }
+      }
+      const uint64_t fm = __builtin_amdgcn_ballot_w64(f);
+      const uint64_t om = __builtin_amdgcn_ballot_w64(op);
+      if (op) {
+        const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(om >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)om, 0));
+        if (ns + below < kStack) stack[ns + below] = C.first;
+      }
+      ns += __builtin_popcountll(om);
+      wave_sync();
+      if (fm) {
+        const int nf = __builtin_popcountll(fm);
+        if (nfar + nf > a.far_burst) {
+          far_contract<MIXED>(a, NP, NP.region, pP, sPpow, lane, s_far[w], nfar, s_loc[w]);
+          nfar = 0;
+        }
+        if (f)
+          s_far[w][nfar + __builtin_amdgcn_mbcnt_hi((uint32_t)(fm >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)fm, 0))] = c;
+        nfar += nf;
+        total += nf;
+        wave_sync();
+      }
+    }
+    if (nfar > 0) far_contract<MIXED>(a, NP, NP.region, pP, sPpow, lane, s_far[w], nfar, s_loc[w]);
+    if (lane < kFmmMaxP) a.plocal[(size_t)g * kFmmMaxP + lane] = s_loc[w][lane];
+    if (pcount && lane == 0) {
+      pcount[3 * (size_t)g] = total;
+      pcount[3 * (size_t)g + 1] = pP;
+      pcount[3 * (size_t)g + 2] = __builtin_popcount(m);
+    }
+    wave_sync();
+  }
+}
+
+__global__ __launch_bounds__(256) void l2l_setup_kernel(const int32_t *__restrict__ leaves,
+                                                        int nleaves, int32_t *__restrict__ nodeli,
+                                                        uint32_t *__restrict__ ngroups_dev,
+                                                        uint32_t ngroups) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) *ngroups_dev = ngroups;
+  if (i < nleaves) nodeli[leaves[i]] = i;
+}
+
+int launch_l2l_setup(const int32_t *d_leaves, int nleaves, int32_t *d_nodeli,
+                     uint32_t *d_ngroups, int ngroups, hipStream_t s) {
+  l2l_setup_kernel<<<(std::max(nleaves, 1) + 255) / 256, 256, 0, s>>>(d_leaves, nleaves, d_nodeli,
+                                                                      d_ngroups, (uint32_t)ngroups);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int launch_parent_local(const FmmArgs &args, bool mixed, int64_t *d_pcount, int nblocks,
+                        hipStream_t s) {
+  if (!args.l2l) return 0;
+  if (args.ngroups_h > 0) {
+    // one wave per sibling group, one wave per workgroup (a finished wave's
+    // slot is refilled at once)
+    const int nb = args.ngroups_h;
+    if (mixed && args.sw > 0.f)
+      parent_local_kernel<true, true, 1><<<nb, 64, 0, s>>>(args, d_pcount);
+    else if (mixed)
+      parent_local_kernel<true, false, 1><<<nb, 64, 0, s>>>(args, d_pcount);
+    else
+      parent_local_kernel<false, false, 1><<<nb, 64, 0, s>>>(args, d_pcount);
+  } else {
+    // the group count only on the device (graph replay): grid-stride
+    if (mixed && args.sw > 0.f)
+      parent_local_kernel<true, true, 4><<<nblocks, 256, 0, s>>>(args, d_pcount);
+    else if (mixed)
+      parent_local_kernel<true, false, 4><<<nblocks, 256, 0, s>>>(args, d_pcount);
+    else
+      parent_local_kernel<false, false, 4><<<nblocks, 256, 0, s>>>(args, d_pcount);
+  }
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
 }
 
 // Persistent form: a fixed grid whose waves take target leaves from a work
@@ -1499,6 +1858,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
   __shared__ int s_stack[4][kStack];
   __shared__ int s_far[4][kFarList];
   __shared__ int32_t s_cku[4][kMaxCk];
+  __shared__ ParentBox s_pbox[4];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t total = *nleaves;
   for (;;) {
@@ -1507,7 +1867,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
     li = __shfl(li, 0, 64);
     if (li >= total) break;
     fmm_process_leaf<MIXED, UNI>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck, s_cku,
-                                 s_stack, s_far);
+                                 s_stack, s_far, s_pbox);
   }
 }
 
@@ -1585,6 +1945,9 @@ static int launch_split_passes(const FmmArgs &args, bool mixed, const uint32_t *
 int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
                                uint32_t *d_counter, int nblocks, uint32_t *d_split_ctr,
                                int split_blocks, hipStream_t s) {
+  FmmArgs pa = args;
+  pa.ngroups_h = 0;  // the launch may be captured and replayed on another tree
+  if (launch_parent_local(pa, mixed, nullptr, split_blocks, s)) return -1;
   zero_counters_kernel<<<1, 64, 0, s>>>(d_counter, 1);
   FMM_HIP_TRY(hipGetLastError());
   const bool split = args.split_budget > 0;
@@ -1610,15 +1973,47 @@ int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *
 __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__restrict__ out) {
   const int li = blockIdx.x * 256 + threadIdx.x;
   if (li >= a.nleaves) return;
-  const NodeRec NA = a.nodes[a.leaves[li]];
+  const int iA = a.leaves[li];
+  const NodeRec NA = a.nodes[iA];
   const int cA = node_count(NA.cnt);
   const double RA = NA.region * 0.70710678118654752;
   const double rcut = a.leaf_par ? (double)a.leaf_par[li].y * a.smax : a.rcut;
   const double inv_theta = 1.0 / a.theta;
+  const bool smooth_on = a.smooth_region > 0.0;
+  // the parent's part (L2L, see ParentCtx): the same rules as the fused walk
+  bool pon = false;
+  NodeRec NP = {};
+  double RP = 0.0, rcutP = 0.0, sx[3] = {0, 0, 0}, sy[3] = {0, 0, 0}, srcut[3] = {0, 0, 0};
+  int nsib = 0;
+  if (a.l2l && a.gparent[iA >> 2] >= 0) {
+    const int g = iA >> 2;
+    int nt = 0;
+    for (int q = 0; q < 4; q++) {
+      const NodeRec C = a.nodes[4 * g + q];
+      if (node_internal(C.cnt) || node_count(C.cnt) == 0) continue;
+      double rc;
+      int pq;
+      leaf_rcut_p(a, a.nodeli[4 * g + q], rc, pq);
+      rcutP = fmax(rcutP, rc);
+      nt++;
+      if (q != (iA & 3)) {
+        sx[nsib] = C.cx;
+        sy[nsib] = C.cy;
+        srcut[nsib] = rc;
+        nsib++;
+      }
+    }
+    if (nt >= a.l2l) {
+      pon = true;
+      NP = a.nodes[a.gparent[g]];
+      RP = NP.region * 0.70710678118654752;
+    }
+  }
   int64_t far = 0, pairs = 0, smooth = 0;
   Slots2 sl;
+  uint32_t covd = 0;  // bit d: the node expanded at depth d was opened by the parent
   int d = 0, cur = 0, node = root_of(a, NA.first);
-  bool expand = true;
+  bool expand = true, fl = pon;
   for (;;) {
     if (expand) {
       const int base = a.nodes[node].first;
@@ -1626,11 +2021,13 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
       for (int q = 0; q < 4; q++) {
         const NodeRec C = a.nodes[base + q];
         if (node_count(C.cnt) == 0) continue;
+        if (fl && far_cell(inv_theta, RP, rcutP, NP, C)) continue;  // the parent's
         if (far_cell(inv_theta, RA, rcut, NA, C)) far++;
         else m |= 1u << q;
       }
       d++;
       sl.set(d, m);
+      covd = fl ? (covd | (1u << d)) : (covd & ~(1u << d));
       cur = base;
     }
     uint32_t m;
@@ -1652,6 +2049,17 @@ __global__ __launch_bounds__(256) void fmm_count_kernel(FmmArgs a, int64_t *__re
       expand = false;
     } else if (node_internal(Cn.cnt)) {
       expand = true;
+      // the children of Cn are the parent's iff the parent opened Cn: every
+      // target-leaf child opens it (A's walk opens it unless it is a whole near
+      // subtree taken as one item)
+      fl = false;
+      if ((covd >> d) & 1u) {
+        fl = !(whole_near(inv_theta, RA, rcut, NA, Cn) &&
+               (!smooth_on || a.srange[node].x != -2));
+        for (int s2 = 0; s2 < nsib && fl; s2++)
+          fl = leaf_opens(a, smooth_on, inv_theta, RA, srcut[s2], sx[s2], sy[s2], NA.region, node,
+                          Cn);
+      }
     } else {
       pairs += (int64_t)cA * node_count(Cn.cnt);
       expand = false;
@@ -1774,6 +2182,7 @@ int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s) {
 int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
                     uint32_t *d_split_ctr, int split_blocks, hipStream_t s, hipEvent_t mid) {
   if (args.nleaves <= 0) return 0;
+  if (launch_parent_local(args, mixed, nullptr, split_blocks, s)) return -1;
   const bool split = args.split_budget > 0;
   if (split) {
     zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 2);

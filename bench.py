@@ -213,7 +213,9 @@ def per_kernel(res, n, p, fmm_mode):
         t = ph["evaluate"] * 1e-3
         for name, fl, bound, note in (
                 ("p2p", 20 * w["pairs"], "valu", "20 flops per near pair"),
-                ("m2l", 8 * w["far_terms2"], "mfma", "8 p_A^2 flops per far cell (leaf's own p)"),
+                ("m2l", 8 * (w["far_terms2"] + w["parent_far_terms2"]), "mfma",
+                 "8 p^2 flops per far cell of a target leaf (its own p) or of a parent"),
+                ("l2l", 8 * p * p * w["l2l_shifts"], "valu", "8 p^2 flops per shift to a child"),
                 ("smooth", 8 * SMOOTH_COEFS * w["smooth"], "valu",
                  "8 x 45 flops per (target, smooth cell)"),
                 ("l2p", 8 * w["l2p_terms"], "valu", "8 p_A flops per target")):
@@ -253,6 +255,9 @@ def main():
     ap.add_argument("--persistent", type=int, default=0,
                     help="accurate modes: persistent fused pass with this many workgroups per CU "
                          "(0: one wave per target leaf over a sized grid)")
+    ap.add_argument("--l2l", type=int, default=2,
+                    help="accurate modes: parents of at least this many target leaves pass their "
+                         "local expansion down (0: off)")
     ap.add_argument("--smooth", type=float, default=0.3,
                     help="mixed modes: smooth cells of half side <= SMOOTH core radii (0: off)")
     ap.add_argument("--smooth-min", type=int, default=24,
@@ -321,12 +326,18 @@ def main():
         base = mode.split("_")[0]
         p = args.p
         smooth = args.smooth
+        l2l = args.l2l
         for part in mode.split("_")[1:]:
             if part.startswith("p"):
                 p = int(part[1:])
             elif part == "nosmooth":
                 smooth = 0.0
+            elif part == "nol2l":
+                l2l = 0
+            elif part.startswith("l2l"):
+                l2l = int(part[3:])
         fmm.set_learned(None)
+        fmm.set_l2l(l2l)
         fmm.set_smooth(smooth, args.smooth_min)
         if base == "treecode":
             fmm.set_mode(fmm.MODE_TREECODE)
@@ -370,7 +381,8 @@ def main():
         else:
             # far cells at each target leaf's own p; a smooth cell at one target:
             # 8 flops per complex coefficient
-            flops = (8 * w["far_terms2"] + 20 * w["pairs"] + 8 * w["l2p_terms"]
+            flops = (8 * (w["far_terms2"] + w["parent_far_terms2"]) + 20 * w["pairs"]
+                     + 8 * w["l2p_terms"] + 8 * p * p * w["l2l_shifts"]
                      + 8 * SMOOTH_COEFS * w["smooth"])
         ach = flops / (ph["evaluate"] * 1e-3) / 1e12
         r = {"ms_per_step": ms, "value": n / (ms * 1e-3), "rel_L2": l2, "rel_max": mx,
@@ -429,6 +441,7 @@ def main():
                                 if not (m == "learned" and not have_weights)}
         del lay
     fmm.set_smooth(args.smooth, args.smooth_min)
+    fmm.set_l2l(args.l2l)
 
     # the same step through host arrays: make_tree copies the blobs in, the
     # velocities come back to host memory (PCIe both ways); never `value`

@@ -129,6 +129,15 @@ int fmm_get_split(void);
  * them into its local expansion (64..128, default 128; a knob of the
  * autotuner: fewer cells per burst, smaller bursts at the end of the walk). */
 int fmm_set_far_burst(int cells);
+/* Accurate modes: local expansions inherited from the parent.  A far cell
+ * that is also far from a target leaf's parent, and that every target-leaf
+ * child of that parent would take as far, is summed once into the parent's
+ * local expansion, which reaches each of those children through the exact
+ * shift L2L (no reference counterpart; the reference's ver-fortran variant
+ * shifts locals in fmm.f90:142-168).  min_children: parents with at least
+ * this many target-leaf children take part (1 or 2: all that can, 3, 4;
+ * default 2); 0: every target leaf contracts all of its far cells itself. */
+int fmm_set_l2l(int min_children);
 /* Smooth cells (no reference counterpart; mixed mode with one core radius
  * sigma).  A near node whose half side is at most eta sigma and that holds at
  * least min_blobs blobs is summed for every target through a degree-8 Taylor
@@ -257,8 +266,10 @@ int fmm_count_work(long long *m2s, long long *pairs, long long *smooth);
  * contractions of target leaves; treecode: M2S evaluations), [1] near pairs,
  * [2] (target, smooth cell) evaluations, [3] sum over target leaves of far
  * cells x the leaf's terms^2 (the contractions' size), [4] sum over target
- * leaves of targets x terms (L2P), [5] target leaves; accurate modes only for
- * [2..5] (0 in the treecode mode). */
+ * leaves of targets x terms (L2P), [5] target leaves, [6] far cells summed
+ * into parent expansions (L2L), [7] their far cells x the parent's terms^2,
+ * [8] parent expansions, [9] L2L shifts; accurate modes only for [2..9] (0 in
+ * the treecode mode). */
 int fmm_count_work_detail(long long *out, int n);
 
 /* Wait for all work of the library; release all device memory. */

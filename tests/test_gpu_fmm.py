@@ -212,3 +212,41 @@ def test_accurate_biot_tree_without_prefetch(fmm, mode):
     finally:
         fmm.set_prefetch(True)
         fmm.set_mode(fmm.MODE_TREECODE)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_l2l_parent_expansions(fmm, oracle, mode):
+    """Local expansions inherited from the parent: with them on, the target
+    leaves contract far fewer cells (the parents take the cells far from
+    them once), the accuracy against the FP64 direct sum stays where it was,
+    and the results do not depend on the launch shape (grid, persistent,
+    graph replay)."""
+    wl = workloads()
+    n = 60000
+    xy, cir, cor = wl.gaussian_mixture(n, seed=81)
+    idx = wl.sample_targets(n, 1500, seed=81)
+    d = oracle.direct(xy, cir, cor, xy[idx], style=1)
+    out = {}
+    for on in (False, True):
+        fmm.set_l2l(on)
+        uv = _run(fmm, mode, xy, cir, cor)
+        w = fmm.count_work_detail()
+        out[on] = (rel_err(uv[idx], d), w, uv)
+        _say(f"[l2l] mode {mode} l2l={on}: rel_L2={out[on][0][0]:.2e} rel_max={out[on][0][1]:.2e} "
+             f"leaf far={w['far']} parent far={w['parent_far']} parents={w['parents']} "
+             f"shifts={w['l2l_shifts']}")
+    fmm.set_l2l(True)
+    e0, w0, _ = out[False]
+    e1, w1, u1 = out[True]
+    assert w0["parent_far"] == 0 and w1["parents"] > 0 and w1["l2l_shifts"] >= 2 * w1["parents"]
+    assert w1["far"] <= 0.7 * w0["far"]
+    assert w1["far"] + w1["parent_far"] < w0["far"]
+    assert w1["pairs"] == w0["pairs"] and w1["smooth"] == w0["smooth"]
+    assert e1[0] <= max(1.5 * e0[0], 2e-7) and e1[1] <= max(1.5 * e0[1], 5e-7)
+    for blocks, graph in ((256, False), (256, True)):
+        fmm.set_persistent(blocks, graph)
+        fmm.make_tree(xy, cir, cor)
+        got = fmm.eval_sources()
+        assert np.array_equal(got.view(np.uint64), u1.view(np.uint64)), (blocks, graph)
+    fmm.set_persistent(0, False)
+    fmm.set_mode(fmm.MODE_TREECODE)
