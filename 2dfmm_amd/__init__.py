@@ -79,6 +79,7 @@ lib.fmm_count_work_detail.argtypes = [np.ctypeslib.ndpointer(dtype=np.int64, fla
 lib.fmm_device_cus.argtypes = []
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 lib.fmm_set_learned.argtypes = [C.c_void_p, C.c_float, C.c_float]
+lib.fmm_set_learned_floor.argtypes = [C.c_float]
 lib.fmm_num_target_leaves.argtypes = []
 lib.fmm_leaf_features.argtypes = [_fp]
 lib.fmm_blob_leaf.argtypes = [_ip]
@@ -239,6 +240,11 @@ def core_spread_device(n, cor_ptr, dt, re) -> None:
 
 def _f32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def set_learned_floor(cut_min: float) -> None:
+    """Floor of the learned near-field radius in core radii (default 2.5)."""
+    _check(lib.fmm_set_learned_floor(float(cut_min)), "set_learned_floor")
 
 
 def set_learned(weights, p_margin: float = 0.0, cut_margin: float = 0.0) -> None:

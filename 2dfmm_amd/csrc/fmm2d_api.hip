@@ -101,6 +101,9 @@ struct Context {
   // learned per-leaf parameters
   bool learned = false;
   float p_margin = 0.f, cut_margin = 0.f;
+  // floor of the learned near radius (core radii): a far pair at c core radii
+  // carries exp(-c^2) of its velocity as the neglected core term
+  float cut_min = 2.5f;
   DevBuf<float> mlp_w, feat, mlp_out;
   DevBuf<double> mean_abs;
   DevBuf<float2> leaf_par;   // learned (or explicitly set) per-leaf (p, cut)
@@ -584,7 +587,8 @@ static int apply_learned() {
   FmmArgs a = fmm_args(nullptr);
   if (launch_leaf_features(a, g.t_num_p, g.sg.p, g.n, g.mean_abs.p, g.feat.p, s)) return -1;
   if (launch_mlp_forward(g.mlp_w.p, nl, g.feat.p, g.mlp_out.p, s)) return -1;
-  if (launch_leaf_params(g.mlp_out.p, nl, g.p_margin, g.cut_margin, g.t_nterms, (float)g.fmm_cut,
+  if (launch_leaf_params(g.mlp_out.p, nl, g.p_margin, g.cut_margin, g.cut_min, g.t_nterms,
+                         (float)g.fmm_cut,
                          g.leaf_par.p, s))
     return -1;
   g.leaf_par_on = true;
@@ -1209,6 +1213,16 @@ int fmm_set_learned(const float *weights, float p_margin, float cut_margin) {
   g.learned = true;
   g.p_margin = p_margin;
   g.cut_margin = cut_margin;
+  return 0;
+}
+
+int fmm_set_learned_floor(float cut_min) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!(cut_min >= 1.f && cut_min <= 6.f)) {
+    set_error("fmm_set_learned_floor: cut_min=%g (1..6 core radii)", (double)cut_min);
+    return -1;
+  }
+  g.cut_min = cut_min;
   return 0;
 }
 

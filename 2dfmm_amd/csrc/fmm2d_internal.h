@@ -300,8 +300,8 @@ int launch_mlp_grad(const float *d_w, int n, const float *d_x, const float *d_y,
                     float *d_grad, float *d_loss, hipStream_t s);
 int launch_adam(float *d_w, const float *d_g, float *d_m, float *d_v, float lr, int t,
                 hipStream_t s);
-int launch_leaf_params(const float *d_y, int n, float p_margin, float cut_margin, int p_max,
-                       float cut_fallback, float2 *d_par, hipStream_t s);
+int launch_leaf_params(const float *d_y, int n, float p_margin, float cut_margin, float cut_min,
+                       int p_max, float cut_fallback, float2 *d_par, hipStream_t s);
 int launch_blob_leaf(const FmmArgs &a, int32_t *d_out, hipStream_t s);
 // Persistent grid of nblocks x 256 threads over a work counter; the number of
 // target leaves is read from *d_nleaves (graph-capturable).

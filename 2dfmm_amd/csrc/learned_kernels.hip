@@ -72,12 +72,16 @@ __global__ __launch_bounds__(256) void abs_sum_kernel(const double *__restrict__
   if (threadIdx.x == 0) part[blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
 }
 
-// out[0] = (sum of the nb partials at out[1..nb], in order) / n
+// out[0] = (sum of the nb <= 1024 partials at out[1..nb]) / n, in a fixed
+// order: lane l adds partials l, l + 64, ... in turn, then a butterfly across
+// the wave -- the same additions in the same order on every run.
 __global__ __launch_bounds__(64) void abs_mean_kernel(double *__restrict__ out, int nb, int n) {
-  if (threadIdx.x != 0) return;
+  const int lane = threadIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nb; b++) s += out[1 + b];
-  out[0] = s / (double)(n > 0 ? n : 1);
+  for (int b = lane; b < nb; b += 64) s += out[1 + b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) out[0] = s / (double)(n > 0 ? n : 1);
 }
 
 int launch_leaf_features(const FmmArgs &a, int leaf_cap, const double *d_cir, int n,
@@ -263,7 +267,8 @@ int launch_blob_leaf(const FmmArgs &a, int32_t *d_out, hipStream_t s) {
 // fixed order and radius.
 __global__ __launch_bounds__(256) void leaf_params_kernel(const float *__restrict__ y, int n,
                                                           float p_margin, float cut_margin,
-                                                          int p_max, float cut_fallback,
+                                                          float cut_min, int p_max,
+                                                          float cut_fallback,
                                                           float2 *__restrict__ par) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -276,15 +281,15 @@ __global__ __launch_bounds__(256) void leaf_params_kernel(const float *__restric
     p = ceilf(11.f + 5.f * y0 + p_margin);
     p = fminf(fmaxf(p, 4.f), (float)p_max);
     c = 3.f + 2.f * y1 + cut_margin;
-    c = fminf(fmaxf(c, 1.f), 6.f);
+    c = fminf(fmaxf(c, cut_min), 6.f);
   }
   par[i] = make_float2(p, c);
 }
 
-int launch_leaf_params(const float *d_y, int n, float p_margin, float cut_margin, int p_max,
-                       float cut_fallback, float2 *d_par, hipStream_t s) {
+int launch_leaf_params(const float *d_y, int n, float p_margin, float cut_margin, float cut_min,
+                       int p_max, float cut_fallback, float2 *d_par, hipStream_t s) {
   if (n <= 0) return 0;
-  leaf_params_kernel<<<(n + 255) / 256, 256, 0, s>>>(d_y, n, p_margin, cut_margin, p_max,
+  leaf_params_kernel<<<(n + 255) / 256, 256, 0, s>>>(d_y, n, p_margin, cut_margin, cut_min, p_max,
                                                      cut_fallback, d_par);
   FMM_HIP_TRY(hipGetLastError());
   return 0;

@@ -155,9 +155,15 @@ int fmm_device_cus(void);
  * build computes 8 features per target leaf, runs the FP32 MLP 8->64->64->2
  * with weights w (4866 floats: W1[64][8] b1[64] W2[64][64] b2[64] W3[2][64]
  * b3[2], ReLU) and takes p = ceil(11 + 5 y0 + p_margin), cut = 3 + 2 y1 +
- * cut_margin (clamped to [4, p] and [1, 6]); a non-finite output falls back to
- * the fixed p and cut.  w = NULL switches it off. */
+ * cut_margin (clamped to [4, p] and [floor, 6], fmm_set_learned_floor); a
+ * non-finite output falls back to the fixed p and cut.  w = NULL switches it
+ * off. */
 int fmm_set_learned(const float *weights, float p_margin, float cut_margin);
+/* Floor of the learned near-field radius, in core radii (1..6, default 2.5):
+ * a far pair at c core radii carries exp(-c^2) of its own velocity as the
+ * neglected core term (0.2% at 2.5), which bounds what a single leaf's
+ * prediction can cost. */
+int fmm_set_learned_floor(float cut_min);
 /* The current tree's target leaves (accurate modes): count; features
  * (count x 8 floats); the target-leaf index of every blob (caller order); the
  * per-leaf (p, cut) in force (count x 2); or set them explicitly (until the
