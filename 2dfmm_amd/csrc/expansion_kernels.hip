@@ -36,9 +36,11 @@ __device__ inline void halve(double (&v)[NV], int lane, int o) {
 }
 
 // NV = 2 x (terms rounded up to 16 or 32) values per lane.
+// Nodes that get no expansion otherwise -- empty leaves and the roots (never
+// shifted into, fmm.c:41) -- get zeros here, so no clearing pass is needed.
 template <int NV>
 __global__ __launch_bounds__(256) void s2m_kernel(const NodeRec *__restrict__ nodes, int nnodes,
-                                                  const double *__restrict__ sx,
+                                                  int nroot_slots, const double *__restrict__ sx,
                                                   const double *__restrict__ sy,
                                                   const double *__restrict__ sg, int nterms,
                                                   double2 *__restrict__ coef) {
@@ -47,7 +49,11 @@ __global__ __launch_bounds__(256) void s2m_kernel(const NodeRec *__restrict__ no
   if (node >= nnodes) return;
   const NodeRec r = nodes[node];
   const int c = node_count(r.cnt);
-  if (node_internal(r.cnt) || c == 0) return;
+  if (node < nroot_slots || (!node_internal(r.cnt) && c == 0)) {
+    if (lane < nterms) coef[(size_t)node * nterms + lane] = make_double2(0.0, 0.0);
+    return;
+  }
+  if (node_internal(r.cnt)) return;
   // v[2k], v[2k+1] = cir (z - z_c)^k (real, imaginary) of this lane's blob;
   // after the reduce-scatter lane l holds the wave's sum of v[l * NV / 64]
   double acc = 0.0;
@@ -85,12 +91,14 @@ __global__ __launch_bounds__(256) void s2m_kernel(const NodeRec *__restrict__ no
     reinterpret_cast<double *>(coef)[2 * ((size_t)node * nterms + k) + (j & 1)] = acc;
 }
 
-int launch_s2m(const NodeRec *d_nodes, int nnodes, const double *sx, const double *sy,
-               const double *sg, int nterms, double2 *d_coef, hipStream_t s) {
+int launch_s2m(const NodeRec *d_nodes, int nnodes, int nroot_slots, const double *sx,
+               const double *sy, const double *sg, int nterms, double2 *d_coef, hipStream_t s) {
   if (nterms <= 16)
-    s2m_kernel<32><<<(nnodes + 3) / 4, 256, 0, s>>>(d_nodes, nnodes, sx, sy, sg, nterms, d_coef);
+    s2m_kernel<32><<<(nnodes + 3) / 4, 256, 0, s>>>(d_nodes, nnodes, nroot_slots, sx, sy, sg,
+                                                    nterms, d_coef);
   else
-    s2m_kernel<64><<<(nnodes + 3) / 4, 256, 0, s>>>(d_nodes, nnodes, sx, sy, sg, nterms, d_coef);
+    s2m_kernel<64><<<(nnodes + 3) / 4, 256, 0, s>>>(d_nodes, nnodes, nroot_slots, sx, sy, sg,
+                                                    nterms, d_coef);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -413,8 +413,9 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   mark(3);
   const size_t ncoef = (size_t)g.nnodes * g.t_nterms;
   if (g.coef.ensure(ncoef, s)) return -1;
-  FMM_HIP_TRY(hipMemsetAsync(g.coef.p, 0, ncoef * sizeof(double2), s));
-  if (launch_s2m(g.nodes.p, g.nnodes, g.sx.p, g.sy.p, g.sg.p, g.t_nterms, g.coef.p, s)) return -1;
+  if (launch_s2m(g.nodes.p, g.nnodes, nroot_slots, g.sx.p, g.sy.p, g.sg.p, g.t_nterms, g.coef.p,
+                 s))
+    return -1;
   for (int l = g.depth - 1; l >= 1; l--) {
     const int a = g.split_off[l], b = g.split_off[l + 1];
     if (launch_m2m_level(g.nodes.p, g.split.p + a, b - a, g.t_nterms, g.coef.p, s)) return -1;

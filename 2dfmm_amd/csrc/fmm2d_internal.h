@@ -106,8 +106,10 @@ int launch_gather(const uint32_t *d_perm, int n, const double *d_xy, const doubl
                   double *ss, hipStream_t s);
 
 // ---- expansions (expansion_kernels.hip) ----------------------------------
-int launch_s2m(const NodeRec *d_nodes, int nnodes, const double *sx, const double *sy,
-               const double *sg, int nterms, double2 *d_coef, hipStream_t s);
+// S2M of every non-empty leaf; zeros for empty leaves and the first
+// nroot_slots nodes (roots); internal nodes are left to M2M.
+int launch_s2m(const NodeRec *d_nodes, int nnodes, int nroot_slots, const double *sx,
+               const double *sy, const double *sg, int nterms, double2 *d_coef, hipStream_t s);
 int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int nparents, int nterms,
                      double2 *d_coef, hipStream_t s);
 

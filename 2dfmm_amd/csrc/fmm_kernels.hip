@@ -1937,7 +1937,10 @@ static int launch_split_passes(const FmmArgs &args, bool mixed, const uint32_t *
   else
     fmm_chunk_kernel<false, false><<<nblocks, 256, 0, s>>>(args, d_chunk_counter);
   FMM_HIP_TRY(hipGetLastError());
-  fmm_split_sum_kernel<<<nblocks, 256, 0, s>>>(args, d_nleaves);
+  // four target leaves per workgroup (grid-stride past the host count, which a
+  // replayed graph may hold from an older tree)
+  const int sb = std::max(nblocks, (args.nleaves + 3) / 4);
+  fmm_split_sum_kernel<<<sb, 256, 0, s>>>(args, d_nleaves);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
