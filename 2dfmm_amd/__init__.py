@@ -69,12 +69,15 @@ lib.fmm_make_forest_device.argtypes = [C.c_int, _ip, _vp, _vp, _vp]
 lib.fmm_set_persistent.argtypes = [C.c_int, C.c_int]
 lib.fmm_set_split.argtypes = [C.c_int]
 lib.fmm_set_far_burst.argtypes = [C.c_int]
+lib.fmm_set_treecode_walk.argtypes = [C.c_int]
 lib.fmm_set_l2l.argtypes = [C.c_int]
 lib.fmm_get_split.argtypes = []
 lib.fmm_set_smooth.argtypes = [C.c_double, C.c_int]
 lib.fmm_get_smooth.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
 lib.fmm_count_work.argtypes = [C.POINTER(C.c_longlong)] * 3
 lib.fmm_count_work_detail.argtypes = [np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                      C.c_int]
+lib.fmm_count_work_leaves.argtypes = [np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                                       C.c_int]
 lib.fmm_device_cus.argtypes = []
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -332,6 +335,11 @@ def set_l2l(min_children) -> None:
     _check(lib.fmm_set_l2l(int(min_children)), "set_l2l")
 
 
+def set_treecode_walk(per_leaf: bool) -> None:
+    """Treecode at the blobs: a wave per leaf (True) or a lane per target."""
+    _check(lib.fmm_set_treecode_walk(1 if per_leaf else 0), "set_treecode_walk")
+
+
 def set_far_burst(cells: int) -> None:
     _check(lib.fmm_set_far_burst(int(cells)), "set_far_burst")
 
@@ -490,6 +498,14 @@ def count_work_detail() -> dict:
     out = np.zeros(len(WORK_KEYS), np.int64)
     _check(lib.fmm_count_work_detail(out, len(WORK_KEYS)), "count_work_detail")
     return dict(zip(WORK_KEYS, (int(v) for v in out)))
+
+
+def count_work_leaves() -> np.ndarray:
+    """Per target leaf: (far cells, near pairs, smooth evaluations, terms, targets)."""
+    nl = num_target_leaves()
+    out = np.zeros((max(nl, 1), 5), np.int64)
+    _check(lib.fmm_count_work_leaves(out.ravel(), nl), "count_work_leaves")
+    return out[:nl]
 
 
 def count_work():

@@ -180,6 +180,102 @@ __global__ __launch_bounds__(256) void parity_eval_kernel(TreeView tv, int nt,
   }
 }
 
+// Treecode at the blobs, one wave per leaf (its blobs are the targets, 64 at a
+// time): the wave walks the union of the nodes its targets open, depth first
+// in the reference's order -- children tested in quadrant order 0..3, opened
+// ones visited 3..0 -- each stack entry carrying the mask of lanes that opened
+// it.  At a popped internal node every lane of the mask tests the four
+// children itself and evaluates the ones it does not open by M2S at once; at a
+// popped leaf the lanes of the mask sum it directly.  Restricted to one lane
+// the events are that target's own walk (parity_eval_kernel) in the same
+// order with the same arithmetic, so the velocities are bit for bit the same;
+// here node records and coefficients are read at wave-uniform addresses and
+// the lanes of a leaf walk together.  The stack holds at most 3 entries per
+// level plus one.
+constexpr int kParStack = 3 * kMaxDepth + 4;
+__global__ __launch_bounds__(64) void parity_leaf_kernel(TreeView tv, int nnodes,
+                                                         const uint32_t *__restrict__ perm,
+                                                         double2 *__restrict__ uv) {
+  __shared__ int s_node[kParStack];
+  __shared__ uint64_t s_mask[kParStack];
+  const int leaf = blockIdx.x, lane = threadIdx.x;
+  if (leaf >= nnodes) return;
+  const NodeRec Lf = tv.nodes[leaf];
+  const int cnt = node_count(Lf.cnt);
+  if (node_internal(Lf.cnt) || cnt == 0) return;
+  const NodeRec *__restrict__ nodes = tv.nodes;
+  const int p = tv.nterms;
+  const double theta = tv.theta;
+  // root: 0 for a single tree; in a forest, the system of the leaf's blobs
+  const int root = tv.root_shift < 0 ? 0 : (int)(tv.key[Lf.first] >> tv.root_shift);
+  for (int b0 = 0; b0 < cnt; b0 += 64) {
+    const int t = Lf.first + b0 + lane;
+    const bool act = b0 + lane < cnt;
+    const double x = act ? tv.sx[t] : 0.0, y = act ? tv.sy[t] : 0.0;
+    double phr = 0.0, phi = 0.0, ud = 0.0, vd = 0.0;
+    int ns = 1;
+    if (lane == 0) {
+      s_node[0] = root;
+      s_mask[0] = cnt - b0 >= 64 ? ~0ull : ((1ull << (cnt - b0)) - 1ull);  // the active lanes
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    while (ns > 0) {
+      ns--;
+      const int node = s_node[ns];
+      const uint64_t m = s_mask[ns];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const bool mine = (m >> lane) & 1;
+      const NodeRec N = nodes[node];
+      if (!node_internal(N.cnt)) {
+        if (mine) p2p_leaf(tv, N.first, node_count(N.cnt), x, y, ud, vd);
+        continue;
+      }
+      const int base = N.first;
+      uint64_t om[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int c = base + q;
+        const NodeRec C = nodes[c];
+        bool op = false;
+        if (mine) {
+          const double dx = x - C.cx, dy = y - C.cy;
+          const double r = sqrt(dx * dx + dy * dy);
+          if (C.region > theta * r)
+            op = true;
+          else if (node_count(C.cnt) > 0)
+            m2s_horner(tv.coef + (size_t)c * p, p, dx, dy, phr, phi);
+        }
+        om[q] = __builtin_amdgcn_ballot_w64(op);
+      }
+      // pushed in quadrant order, so the highest opened quadrant is next
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (om[q]) {
+          if (lane == 0) {
+            s_node[ns] = base + q;
+            s_mask[ns] = om[q];
+          }
+          ns++;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    // fmm.c:115-117
+    if (act) uv[perm[t]] = make_double2(0.5 * phi / M_PI + ud, 0.5 * phr / M_PI + vd);
+  }
+}
+
+int launch_parity_leaves(const TreeView &tv, int nnodes, const uint32_t *d_perm, double *d_uv,
+                         hipStream_t s) {
+  if (nnodes <= 0) return 0;
+  parity_leaf_kernel<<<nnodes, 64, 0, s>>>(tv, nnodes, d_perm, reinterpret_cast<double2 *>(d_uv));
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int launch_parity_eval(const TreeView &tv, int nt, const double *d_txy, const uint32_t *d_perm,
                        double *d_uv, hipStream_t s) {
   if (nt <= 0) return 0;

@@ -258,7 +258,7 @@ __device__ inline int smooth_slot(int i, int j) {
 // t < (q+1)(q+2)/2 adds the products of term t = (m, n) over the chunk (FP64).
 // Lane k < kSmoothC then forms output coefficient k.
 __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restrict__ nodes,
-                                                         int nnodes,
+                                                         int node0, int nnodes,
                                                          const int32_t *__restrict__ nfirst,
                                                          const double *__restrict__ sx,
                                                          const double *__restrict__ sy,
@@ -269,9 +269,10 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
   constexpr int NT = (Q + 1) * (Q + 2) / 2;
   __shared__ double2 mu[NT];
   __shared__ double2 cz[kSmoothC];   // c_ij lam^(i+j) in storage order
-  __shared__ double2 pm[Q + 1][64];  // cir d^m
-  __shared__ double2 pn[Q + 1][64];  // conj(d)^n
-  const int node = blockIdx.x;
+  constexpr int B = 32;               // blobs per chunk (LDS: 4 waves' worth per 64 KB)
+  __shared__ double2 pm[Q + 1][B];  // cir d^m
+  __shared__ double2 pn[Q + 1][B];  // conj(d)^n
+  const int node = node0 + blockIdx.x;
   if (node >= nnodes) return;
   const NodeRec C = nodes[node];
   const int cnt = node_count(C.cnt);
@@ -287,8 +288,8 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
   }
   const int f = nfirst[node];
   double ar = 0.0, ai = 0.0;
-  for (int j0 = 0; j0 < cnt; j0 += 64) {
-    const int nb = min(64, cnt - j0);
+  for (int j0 = 0; j0 < cnt; j0 += B) {
+    const int nb = min(B, cnt - j0);
     if (lane < nb) {
       const int j = f + j0 + lane;
       const double dr = (C.cx - sx[j]) * inv_sigma, di = (C.cy - sy[j]) * inv_sigma;
@@ -379,12 +380,12 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
         make_float2((float)(vr * inv_sigma), (float)(vi * inv_sigma));
 }
 
-int launch_smooth_coef(const NodeRec *d_nodes, int nnodes, const int32_t *d_nfirst,
+int launch_smooth_coef(const NodeRec *d_nodes, int node0, int node1, const int32_t *d_nfirst,
                        const double *sx, const double *sy, const double *sg, double sigma,
                        double region, int smin, float4 *d_out, hipStream_t s) {
-  if (nnodes <= 0) return 0;
-  smooth_coef_kernel<<<nnodes, 64, 0, s>>>(d_nodes, nnodes, d_nfirst, sx, sy, sg, 1.0 / sigma,
-                                           region, smin, d_out);
+  if (node1 <= node0) return 0;
+  smooth_coef_kernel<<<node1 - node0, 64, 0, s>>>(d_nodes, node0, node1, d_nfirst, sx, sy, sg,
+                                                  1.0 / sigma, region, smin, d_out);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -121,6 +121,7 @@ struct Context {
   // splitting of target leaves with much near work (0: off)
   int split_budget = 8192;
   int far_burst = 128;  // far cells per MFMA burst (64..128)
+  bool parity_leaves = true;  // treecode at the blobs: a wave per leaf (else a lane per target)
   int l2l = 2;          // parent local expansions for parents of >= l2l target leaves (0: off)
   DevBuf<double2> plocal;
   DevBuf<int32_t> nodeli;
@@ -134,7 +135,7 @@ struct Context {
   // at least smooth_min blobs; smooth_region_h = 2 eta sigma of the current
   // tree (0: none built)
   double smooth_eta = 0.3;
-  int smooth_min = 24;
+  int smooth_min = 16;
   double smooth_region_h = 0.0;
   int t_smooth_min = 0;  // smooth_min the current tree's coefficients used
   DevBuf<float4> scoef;
@@ -156,6 +157,7 @@ struct Context {
   DevBuf<uint8_t> cflag;      // split flags of one level's children
   int32_t *h_lt = nullptr;    // pinned copy of the level table
   int depth_hint = 0;         // depth of the previous tree
+  RootBox h_root = {};        // root square of the current single tree (read with the levels)
   DevBuf<double2> coef;
   DevBuf<RootBox> root;
   std::vector<int> split_off;
@@ -367,6 +369,8 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
       launched = upto;
       FMM_HIP_TRY(hipMemcpyAsync(g.h_lt, g.lt.p, sizeof(int32_t) * kLtWords, hipMemcpyDeviceToHost,
                                  s));
+      if (nsys == 1)
+        FMM_HIP_TRY(hipMemcpyAsync(&g.h_root, g.root.p, sizeof(RootBox), hipMemcpyDeviceToHost, s));
       FMM_HIP_TRY(hipStreamSynchronize(s));
       const int err = g.h_lt[kLtErr];
       if (err & kLtErrDepth) {
@@ -455,8 +459,21 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
       const double region = 2.0 * g.smooth_eta * smm[0];
       // every internal node (>= num_p blobs) of the smooth level is a smooth cell
       const int smin = std::min(g.smooth_min, g.t_num_p);
+      // a single tree's smooth level is one contiguous range of nodes (every
+      // level halves the root's side exactly); a forest's roots differ
+      int n0 = 0, n1 = g.nnodes;
+      if (nsys == 1) {
+        double rl = g.h_root.region;
+        int l = 0;
+        while (l < g.depth && rl > region) {
+          rl *= 0.5;
+          l++;
+        }
+        n0 = l == 0 ? 0 : g.h_lt[kLtCb + l];
+        n1 = l == 0 ? nroot_slots : g.h_lt[kLtCb + l + 1];
+      }
       if (g.scoef.ensure((size_t)g.nnodes * kSmoothF4, s) ||
-          launch_smooth_coef(g.nodes.p, g.nnodes, g.nfirst.p, g.sx.p, g.sy.p, g.sg.p, smm[0],
+          launch_smooth_coef(g.nodes.p, n0, n1, g.nfirst.p, g.sx.p, g.sy.p, g.sg.p, smm[0],
                              region, smin, g.scoef.p, s))
         return -1;
       if (g.sflag.ensure(2 * (size_t)g.nnodes, s) || g.srange.ensure(g.nnodes, s) ||
@@ -655,7 +672,8 @@ static int eval_sources_dev(double *d_uv) {
   const bool mixed = g.t_mode == FMM2D_MODE_MIXED;
   if (g.t_mode != FMM2D_MODE_TREECODE && ensure_split()) return -1;
   if (g.t_mode == FMM2D_MODE_TREECODE)
-    rc = launch_parity_eval(view(), g.n, nullptr, g.perm.p, d_uv, g.stream);
+    rc = g.parity_leaves ? launch_parity_leaves(view(), g.nnodes, g.perm.p, d_uv, g.stream)
+                         : launch_parity_eval(view(), g.n, nullptr, g.perm.p, d_uv, g.stream);
   else if (g.persist_blocks > 0 && g.use_graph)
     rc = eval_fmm_graph(fmm_args(d_uv), mixed);
   else if (g.persist_blocks > 0)
@@ -1152,6 +1170,12 @@ int fmm_set_split(int budget) {
     return -1;
   }
   g.split_budget = budget;
+  return 0;
+}
+
+int fmm_set_treecode_walk(int per_leaf) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g.parity_leaves = per_leaf != 0;
   return 0;
 }
 
@@ -1779,6 +1803,28 @@ int fmm_count_work(long long *m2s, long long *pairs, long long *smooth) {
   *pairs = o[1];
   *smooth = o[2];
   return 0;
+}
+
+int fmm_count_work_leaves(long long *out, int nleaves) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (need_fmm_tree()) return -1;
+  if (nleaves != g.ntleaves) {
+    set_error("fmm_count_work_leaves: %d entries for %d target leaves", nleaves, g.ntleaves);
+    return -1;
+  }
+  if (nleaves == 0) return 0;
+  DevBuf<int64_t> c;
+  const size_t nw = (size_t)kCountWords * nleaves;
+  if (c.ensure(nw, g.stream)) return -1;
+  int rc = launch_fmm_count(fmm_args(nullptr), c.p, g.stream);
+  if (!rc && (hipMemcpyAsync(out, c.p, sizeof(int64_t) * nw, hipMemcpyDeviceToHost, g.stream) !=
+                  hipSuccess ||
+              hipStreamSynchronize(g.stream) != hipSuccess)) {
+    set_error("count download failed");
+    rc = -1;
+  }
+  c.release();
+  return rc;
 }
 
 int fmm_count_work_detail(long long *out, int n) {

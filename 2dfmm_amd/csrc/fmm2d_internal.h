@@ -124,10 +124,10 @@ int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int npare
 constexpr int kSmoothQ = 8;
 constexpr int kSmoothC = (kSmoothQ + 1) + kSmoothQ * (kSmoothQ + 1) / 2;
 constexpr int kSmoothF4 = (kSmoothC + 1) / 2;  // float4 per cell
-// Coefficients of every node of region in (region/2, region] with at least
-// smin blobs (those of node k are sx..sg[nfirst[k] ...]), at
-// d_out[node * kSmoothF4]; other nodes are left as they are.
-int launch_smooth_coef(const NodeRec *d_nodes, int nnodes, const int32_t *d_nfirst,
+// Coefficients of every node in [node0, node1) of region in (region/2,
+// region] with at least smin blobs (those of node k are sx..sg[nfirst[k] ...]),
+// at d_out[node * kSmoothF4]; other nodes are left as they are.
+int launch_smooth_coef(const NodeRec *d_nodes, int node0, int node1, const int32_t *d_nfirst,
                        const double *sx, const double *sy, const double *sg, double sigma,
                        double region, int smin, float4 *d_out, hipStream_t s);
 // Smooth runs: for every internal node above the smooth level (region >
@@ -154,6 +154,10 @@ struct TreeView {
 };
 // Treecode velocities at targets (txy interleaved; or the sorted blobs
 // themselves when txy is null, then results are scattered through perm).
+// Treecode velocities at every blob, one wave per leaf walking the union of
+// its targets' opened nodes (bitwise those of launch_parity_eval at the blobs).
+int launch_parity_leaves(const TreeView &tv, int nnodes, const uint32_t *d_perm, double *d_uv,
+                         hipStream_t s);
 int launch_parity_eval(const TreeView &tv, int nt, const double *d_txy, const uint32_t *d_perm,
                        double *d_uv, hipStream_t s);
 // Interaction lists of targets: node index (>= 0) and kind (0 P2P, 1 M2S).

@@ -25,23 +25,20 @@ constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH
               oW3 = oB2 + kH, oB3 = oW3 + kOut * kH, kNW = oB3 + kOut;
 static_assert(kNW == kMlpWeights, "weight count");
 
-// One wave per target leaf: the lanes add |cir| of the leaf's blobs (a fixed
-// butterfly), lane 0 writes the features.
+// One thread per target leaf (the features' logarithms and roots run on all
+// lanes; the leaf's |cir| are summed in blob order).
 __global__ __launch_bounds__(256) void leaf_features_kernel(const FmmArgs a, int leaf_cap,
                                                             const double *__restrict__ mean_abs,
                                                             float *__restrict__ feat) {
-  const int li = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int li = blockIdx.x * 256 + threadIdx.x;
   if (li >= a.nleaves) return;
   const int A = a.leaves[li];
   const NodeRec NA = a.nodes[A];
   const int cA = node_count(NA.cnt);
-  double sabs = 0.0;
-  for (int j = lane; j < cA; j += 64) sabs += fabs(a.loc64[NA.first + j].z);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) sabs += __shfl_xor(sabs, o, 64);
-  if (lane != 0) return;
   const int root = a.root_shift < 0 ? 0 : (int)(a.key[NA.first] >> a.root_shift);
   const double side = NA.region, rside = a.nodes[root].region;
+  double sabs = 0.0;
+  for (int j = 0; j < cA; j++) sabs += fabs(a.loc64[NA.first + j].z);
   const double2 a0 = a.coef[(size_t)A * a.p], a1 = a.coef[(size_t)A * a.p + 1];
   const int par = a.gparent[A >> 2];
   const int cpar = par >= 0 ? node_count(a.nodes[par].cnt) : cA;
@@ -96,7 +93,7 @@ int launch_leaf_features(const FmmArgs &a, int leaf_cap, const double *d_cir, in
   nb = nb > kAbsBlocks ? kAbsBlocks : (nb < 1 ? 1 : nb);
   abs_sum_kernel<<<nb, 256, 0, s>>>(d_cir, n, d_mean_abs + 1);
   abs_mean_kernel<<<1, 64, 0, s>>>(d_mean_abs, nb, n);
-  leaf_features_kernel<<<(a.nleaves + 3) / 4, 256, 0, s>>>(a, leaf_cap, d_mean_abs, d_feat);
+  leaf_features_kernel<<<(a.nleaves + 255) / 256, 256, 0, s>>>(a, leaf_cap, d_mean_abs, d_feat);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -263,7 +263,7 @@ def main():
                     help="learned mode: floor of the near-field radius in core radii")
     ap.add_argument("--smooth", type=float, default=0.3,
                     help="mixed modes: smooth cells of half side <= SMOOTH core radii (0: off)")
-    ap.add_argument("--smooth-min", type=int, default=24,
+    ap.add_argument("--smooth-min", type=int, default=16,
                     help="mixed modes: fewest blobs of a smooth cell")
     ap.add_argument("--traffic",
                     default=os.path.join(ROOT, "profiles", "fmm_eval_gaussian_mixture_4M_traffic.txt"),

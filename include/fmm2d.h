@@ -88,6 +88,12 @@ int fmm_set_fmm_params(int p, int leaf, double theta, double cut_sigmas);
  * plain loop on the device; *max_abs_err gets the largest difference. */
 int fmm_selftest_mfma(int fp32, double *max_abs_err);
 
+/* Treecode mode at the blobs: 1 (default) one wave per leaf walking the union
+ * of its targets' opened nodes, 0 one lane per target walking alone; the
+ * results are the same bits (each target meets its own events in the
+ * reference's order either way). */
+int fmm_set_treecode_walk(int per_leaf);
+
 /* 0: make_tree does not evaluate at the blobs; 1 (default): it does,
  * asynchronously.  Only the timing changes: with prefetch off, biot_tree at a
  * blob position evaluates every blob on its first call in the accurate modes
@@ -143,7 +149,7 @@ int fmm_set_l2l(int min_children);
  * least min_blobs blobs is summed for every target through a degree-8 Taylor
  * expansion of the regularised kernel about its centre (coefficients computed
  * at make_tree) instead of pair by pair.  eta in [0, 0.35]; 0 turns it off.
- * Takes effect at the next make_tree.  Default eta 0.3, min_blobs 24. */
+ * Takes effect at the next make_tree.  Default eta 0.3, min_blobs 16. */
 int fmm_set_smooth(double eta, int min_blobs);
 int fmm_get_smooth(double *eta, int *min_blobs);
 /* Compute units of the current device (for sizing persistent grids). */
@@ -277,6 +283,10 @@ int fmm_count_work(long long *m2s, long long *pairs, long long *smooth);
  * [8] parent expansions, [9] L2L shifts; accurate modes only for [2..9] (0 in
  * the treecode mode). */
 int fmm_count_work_detail(long long *out, int n);
+/* The same walk per target leaf (accurate modes): out[5 i + 0..4] = far cells,
+ * near pairs, (target, smooth cell) evaluations, terms and targets of target
+ * leaf i (nleaves = fmm_num_target_leaves()). */
+int fmm_count_work_leaves(long long *out, int nleaves);
 
 /* Wait for all work of the library; release all device memory. */
 int fmm_synchronize(void);

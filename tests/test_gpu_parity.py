@@ -332,3 +332,24 @@ def test_snapshot_through_library(fmm, tmp_path):
     assert np.array_equal(t.vel[:, 1:], old_hist) and np.array_equal(t.blob_id, s.blob_id)
     assert np.array_equal(t.xy, g["xy"]) and np.array_equal(t.cor, g["cor"])
     assert sf.read_records(p1)[:len(head)] == head and sf.read_records(p1)[-2:] == tail
+
+
+@pytest.mark.parametrize("kind,n,seed,nterms,num_p", [("uniform", 20000, 91, 10, 32),
+                                                      ("mixture", 200000, 92, 16, 64),
+                                                      ("sheet", 50000, 93, 12, 100)])
+def test_treecode_walk_per_leaf_bitwise(fmm, kind, n, seed, nterms, num_p):
+    """The treecode at the blobs, one wave per leaf (the union walk), gives the
+    same bits as one lane per target; leaves of more than 64 blobs (num_p
+    100) take several passes."""
+    wl = workloads()
+    xy, cir, cor = {"uniform": wl.uniform_square, "sheet": wl.sheet,
+                    "mixture": wl.gaussian_mixture}[kind](n, seed=seed)
+    fmm.set_params(nterms, num_p, 0.5)
+    out = []
+    for per_leaf in (False, True):
+        fmm.set_treecode_walk(per_leaf)
+        fmm.make_tree(xy, cir, cor)
+        out.append(fmm.eval_sources())
+    fmm.set_treecode_walk(True)
+    _say(f"[treecode walk] {kind} N={n}: max |diff| {np.abs(out[0] - out[1]).max():.3e}")
+    assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 def fmm():
     m = fmm_module()
     yield m
-    m.set_smooth(0.3, 24)
+    m.set_smooth(0.3, 16)
     m.set_split(8192)
     m.set_persistent(0, False)
     m.set_mode(m.MODE_TREECODE)
@@ -33,7 +33,7 @@ def _say(*a):
 
 
 def _run(fmm, xy, cir, cor, eta, mode=2, budget=8192, blocks=0, graph=False):
-    fmm.set_smooth(eta, 24)
+    fmm.set_smooth(eta, 16)
     fmm.set_mode(mode)
     fmm.set_fmm_params(16, 64, 0.5, 4.0)
     fmm.set_split(budget)
