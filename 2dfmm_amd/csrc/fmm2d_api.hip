@@ -157,7 +157,7 @@ struct Context {
   DevBuf<uint8_t> cflag;      // split flags of one level's children
   int32_t *h_lt = nullptr;    // pinned copy of the level table
   int depth_hint = 0;         // depth of the previous tree
-  RootBox h_root = {};        // root square of the current single tree (read with the levels)
+  RootBox *h_root = nullptr;  // root square of the current single tree (pinned, read with the levels)
   DevBuf<double2> coef;
   DevBuf<RootBox> root;
   std::vector<int> split_off;
@@ -193,7 +193,11 @@ static int ctx_init() {
   }
   FMM_HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   FMM_HIP_TRY(hipHostMalloc((void **)&g.h_total, 64, hipHostMallocDefault));
-  FMM_HIP_TRY(hipHostMalloc((void **)&g.h_lt, sizeof(int32_t) * kLtWords, hipHostMallocDefault));
+  // the level table, then the root square (read back together, pinned)
+  FMM_HIP_TRY(hipHostMalloc((void **)&g.h_lt, sizeof(int32_t) * kLtWords + 64 + sizeof(RootBox),
+                            hipHostMallocDefault));
+  g.h_root = reinterpret_cast<RootBox *>(
+      reinterpret_cast<char *>(g.h_lt) + ((sizeof(int32_t) * kLtWords + 63) & ~size_t(63)));
   memset(g.h_total, 0, 64);
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_uv, hipEventDisableTiming));
   for (auto &e : g.tev) FMM_HIP_TRY(hipEventCreate(&e));
@@ -370,7 +374,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
       FMM_HIP_TRY(hipMemcpyAsync(g.h_lt, g.lt.p, sizeof(int32_t) * kLtWords, hipMemcpyDeviceToHost,
                                  s));
       if (nsys == 1)
-        FMM_HIP_TRY(hipMemcpyAsync(&g.h_root, g.root.p, sizeof(RootBox), hipMemcpyDeviceToHost, s));
+        FMM_HIP_TRY(hipMemcpyAsync(g.h_root, g.root.p, sizeof(RootBox), hipMemcpyDeviceToHost, s));
       FMM_HIP_TRY(hipStreamSynchronize(s));
       const int err = g.h_lt[kLtErr];
       if (err & kLtErrDepth) {
@@ -463,7 +467,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
       // level halves the root's side exactly); a forest's roots differ
       int n0 = 0, n1 = g.nnodes;
       if (nsys == 1) {
-        double rl = g.h_root.region;
+        double rl = g.h_root->region;
         int l = 0;
         while (l < g.depth && rl > region) {
           rl *= 0.5;
