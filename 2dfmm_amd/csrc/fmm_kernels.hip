@@ -891,9 +891,24 @@ __device__ inline void smooth_eval(float zx, float zy, const float4 *row, float 
 
 // Work units of one smooth evaluation: kSmoothUnits (about its cost in pair
 // steps).
-constexpr int kSmoothRow = kSmoothF4 + 1;  // coefficients + the cell's offset
-static_assert(kSmoothBatch * kSmoothRow <= kStageSlots, "smooth batch exceeds the stage");
-static_assert((kSmoothBatch + 1) * 64 * 8 <= kSmoothResSlots * 16, "smooth values exceed the stage");
+//
+// Stage layout of the mixed form (float4 slots of one wave's slice):
+//   [0, 160)    near-step sources (slots < kStageSlots) while a near item is
+//               summed; the values of one smooth batch (float2 per (target,
+//               cell) evaluation) while a batch is summed -- never both;
+//   [160, 275)  coefficients of the pending smooth cells, kSmoothF4 per cell,
+//               written by direct global-to-LDS loads when a cell is found;
+//   [276, 279)  their offsets from the target leaf (float2 per cell);
+//   [288, 320)  the targets' coordinates (float2 per target).
+constexpr int kSmoothCoef = 160;
+constexpr int kSmoothOff = 276;
+constexpr int kSmoothTl = 288;
+static_assert(kStageSlots <= kSmoothCoef && 2 * kSmoothCoef >= kSmoothBatch * 64,
+              "near stage or smooth values overlap the coefficients");
+static_assert(kSmoothCoef + kSmoothBatch * kSmoothF4 <= kSmoothOff &&
+                  2 * kSmoothOff + kSmoothBatch <= 2 * kSmoothTl &&
+                  kSmoothTl + 32 <= kStageSlots + kSmoothResSlots,
+              "smooth stage exceeds the slice");
 // Node k (lane k) of tile t of smooth run c: its record (cnt 0 past the end).
 __device__ inline NodeRec srun_node(const FmmArgs &a, int c, int t, int lane, int &idx) {
   const int2 r = ldro(a.srange + c);
@@ -903,41 +918,44 @@ __device__ inline NodeRec srun_node(const FmmArgs &a, int c, int t, int lane, in
   return C;
 }
 
-// The nb cells held by lanes 0..nb-1 in `cell`: their coefficients and offsets
-// from the target leaf (FP64 difference, then FP32 scaled) are staged in the
-// wave's LDS slice st, kSmoothRow float4 per cell.  The nb x cA evaluations
-// e = (target e mod cA, cell e / cA) are dealt to the lanes in rounds (lane e
-// starts at its own (ti, rep)); each value goes to st's result slots, and
-// the lane of target t (replica 0) adds its nb values in cell order to its
-// FP64 sums -- a cell adds the same whatever batch it falls in.
-__device__ inline void smooth_batch(const FmmArgs &a, const LeafCtx &L, int cell, int nb, int lane,
-                                    float4 *st, double &ax, double &ay) {
-  const int k0 = lane / kSmoothRow, k1 = (lane + 64) / kSmoothRow;
-  const int c0 = __shfl(cell, k0, 64), c1 = __shfl(cell, k1, 64);
-  auto stage = [&](int e, int k, int c) {
-    if (e < nb * kSmoothRow) {
-      const int j = e - k * kSmoothRow;
-      if (j < kSmoothF4) {
-        st[e] = a.scoef[(size_t)c * kSmoothF4 + j];
-      } else {
-        const NodeRec C = a.nodes[c];
-        st[e] = make_float4((float)((C.cx - L.NA.cx) * a.sw), (float)((C.cy - L.NA.cy) * a.sw),
-                            0.f, 0.f);
-      }
-    }
-  };
-  float2 *res = reinterpret_cast<float2 *>(st + kStageSlots);
-  float2 *tl = res + kSmoothBatch * 64;  // target coordinates
-  stage(lane, k0, c0);
-  stage(lane + 64, k1, c1);
+// Smooth cell C (node index cell) becomes cell k of the pending batch: lanes
+// 0..kSmoothF4-1 load its coefficients straight into the stage (global to
+// LDS, no registers; they land while the walk goes on), lane 0 writes its
+// offset from the target leaf (FP64 difference, then FP32 scaled).
+__device__ inline void smooth_fetch(const FmmArgs &a, const LeafCtx &L, int cell, const NodeRec &C,
+                                    int k, int lane, float4 *st) {
+  if (lane == 0)
+    reinterpret_cast<float2 *>(st + kSmoothOff)[k] =
+        make_float2((float)((C.cx - L.NA.cx) * a.sw), (float)((C.cy - L.NA.cy) * a.sw));
+  if (lane < kSmoothF4)
+    __builtin_amdgcn_global_load_lds(
+        (const void *)(a.scoef + (size_t)cell * kSmoothF4 + lane),
+        (__attribute__((address_space(3))) void *)(st + kSmoothCoef + k * kSmoothF4), 16, 0, 0);
+}
+
+// The nb pending cells (coefficients and offsets in the stage, see
+// smooth_fetch): the nb x cA evaluations e = (target e mod cA, cell e / cA)
+// are dealt to the lanes in rounds (lane e starts at its own (ti, rep)); each
+// value goes to the stage's value slots, and the lane of target t (replica 0)
+// adds its nb values in cell order to its FP64 sums -- a cell adds the same
+// whatever batch it falls in.
+__device__ inline void smooth_batch(const LeafCtx &L, int nb, int lane, float4 *st, double &ax,
+                                    double &ay) {
+  float2 *res = reinterpret_cast<float2 *>(st);
+  const float4 *coef = st + kSmoothCoef;
+  const float2 *off = reinterpret_cast<const float2 *>(st + kSmoothOff);
+  float2 *tl = reinterpret_cast<float2 *>(st + kSmoothTl);
   if (L.rep == 0) tl[L.ti] = make_float2(L.lx, L.ly);
+  // the direct loads of the coefficients have landed (vmcnt 0), then the
+  // wave's LDS writes are visible to all its lanes
+  __builtin_amdgcn_s_waitcnt(0x0F70);
   wave_sync();
   const int cA = L.cA, ne = nb * cA;
   const int q64 = L.nrep, r64 = 64 - q64 * cA;  // 64 / cA, 64 % cA (cA <= 64)
   int t = L.ti, k = L.rep;
   for (int e = lane; e < ne; e += 64) {
-    const float4 *row = st + k * kSmoothRow;
-    const float4 o = row[kSmoothF4];
+    const float4 *row = coef + k * kSmoothF4;
+    const float2 o = off[k];
     const float2 p = tl[t];
     float gx = 0.f, gy = 0.f;
     smooth_eval(p.x - o.x, p.y - o.y, row, gx, gy);
@@ -960,6 +978,18 @@ __device__ inline void smooth_batch(const FmmArgs &a, const LeafCtx &L, int cell
   wave_sync();
 }
 
+// Steps ta..tb-1 of near item c.
+template <bool MIXED, bool UNI>
+__device__ inline void near_steps(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
+                                  bool range, bool ser, int ta, int tb, int lane,
+                                  typename StageOf<MIXED>::T *st, double &ax,
+                                  double &ay) {
+  if (range)
+    near_range<MIXED, UNI>(a, L, c, C, ser, ta, tb, lane, st, ax, ay);
+  else
+    near_leaf<MIXED, UNI>(a, L, C, ser, lane, st, ax, ay);
+}
+
 // Steps ta..tb-1 of near item c of any kind; smooth cells go to push(cell)
 // (the caller's batch), the small leaves of a smooth run are summed here.
 template <bool MIXED, bool UNI, typename Push>
@@ -969,7 +999,7 @@ __device__ inline void item_steps(const FmmArgs &a, const LeafCtx &L, int c, con
                                   Push &&push) {
   if constexpr (MIXED && UNI) {
     if (kind == kItemSmooth) {
-      push(c);
+      push(c, C);
       return;
     }
     if (kind == kItemSmoothRun) {
@@ -987,7 +1017,7 @@ __device__ inline void item_steps(const FmmArgs &a, const LeafCtx &L, int c, con
           m &= m - 1;
           const int node = lane_of(idx, k);
           if ((sm >> k) & 1) {
-            push(node);
+            push(node, node_of_lane(N, k));
           } else {
             near_leaf<MIXED, UNI>(a, L, node_of_lane(N, k), (serm >> k) & 1, lane, st, ax, ay);
           }
@@ -999,17 +1029,6 @@ __device__ inline void item_steps(const FmmArgs &a, const LeafCtx &L, int c, con
   near_steps<MIXED, UNI>(a, L, c, C, kind == kItemRange, ser, ta, tb, lane, st, ax, ay);
 }
 
-// Steps ta..tb-1 of near item c.
-template <bool MIXED, bool UNI>
-__device__ inline void near_steps(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
-                                  bool range, bool ser, int ta, int tb, int lane,
-                                  typename StageOf<MIXED>::T *st, double &ax,
-                                  double &ay) {
-  if (range)
-    near_range<MIXED, UNI>(a, L, c, C, ser, ta, tb, lane, st, ax, ay);
-  else
-    near_leaf<MIXED, UNI>(a, L, C, ser, lane, st, ax, ay);
-}
 
 // The far test of a cell C for target leaf A: R_A + R_C <= theta d and
 // d - R_A - R_C >= r_cut, i.e. d >= max((R_A + R_C) / theta, R_A + R_C + r_cut),
@@ -1350,13 +1369,13 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t sta
   const int k0 = (int)(start >> 32), t0 = (int)(uint32_t)start;
   const int k1 = (int)(end >> 32), t1 = (int)(uint32_t)end;
   int k = 0;
-  int scell = 0, snb = 0;
+  int snb = 0;
   const int sbsz = L.sbsz;
-  auto push = [&](int cell) {
+  auto push = [&](int cell, const NodeRec &C) {
     if constexpr (SM) {
-      if (lane == snb) scell = cell;
+      smooth_fetch(a, L, cell, C, snb, lane, st);
       if (++snb == sbsz) {
-        smooth_batch(a, L, scell, snb, lane, st, ax, ay);
+        smooth_batch(L, snb, lane, st, ax, ay);
         snb = 0;
       }
     }
@@ -1373,7 +1392,7 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t sta
     return ki < k1;
   }, ParentCtx{});
   if constexpr (SM) {
-    if (snb > 0) smooth_batch(a, L, scell, snb, lane, st, ax, ay);
+    if (snb > 0) smooth_batch(L, snb, lane, st, ax, ay);
   }
 }
 
@@ -1521,13 +1540,13 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   constexpr bool SM = MIXED && UNI;
   // pending smooth cells (lane k holds cell k of the batch); only the summing
   // part of the walk adds cells, so the last batch is summed after the walk
-  int scell = 0, snb = 0;
+  int snb = 0;
   const int sbsz = L.sbsz;
-  auto push = [&](int cell) {
+  auto push = [&](int cell, const NodeRec &C) {
     if constexpr (SM) {
-      if (lane == snb) scell = cell;
+      smooth_fetch(a, L, cell, C, snb, lane, s_stage[w]);
       if (++snb == sbsz) {
-        smooth_batch(a, L, scell, snb, lane, s_stage[w], ax, ay);
+        smooth_batch(L, snb, lane, s_stage[w], ax, ay);
         snb = 0;
       }
     }
@@ -1582,7 +1601,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   int *stack = s_stack[w];
   walk_wide<SM, true>(a, L, lane, stack, far, item, pc);
   if constexpr (SM) {
-    if (snb > 0) smooth_batch(a, L, scell, snb, lane, s_stage[w], ax, ay);
+    if (snb > 0) smooth_batch(L, snb, lane, s_stage[w], ax, ay);
   }
   if (nfar > 0) far_flush();
 
