@@ -17,16 +17,24 @@
 //     and the scaled local expansion is  Lt_l = (1/s_A) sum_k C(l+k,k) G[l][k],
 //     i.e. the M2L of every far cell of A folded into one rank-K update of a
 //     16 x 32 real matrix (K = 2 x far cells); the potential at a target is
-//     sum_l Lt_l ((z - c_A)/s_A)^l (L2P).
+//     sum_l Lt_l ((z - c_A)/s_A)^l (L2P);
+//   * L2L: cells far from every target-leaf child of a parent P are
+//     contracted once into P's local expansion (parent_local_kernel), which
+//     each child starts from, shifted to its own centre; the child's walk
+//     skips them;
+//   * smooth cells (mixed form, one core radius): near nodes of the smooth
+//     level are summed for every target through a Taylor expansion of the
+//     regularised kernel about the cell centre (smooth_eval).
 // The traversal decisions are uniform across the wave (they depend on the
 // target leaf only), so the wave walks the tree together; lanes are
 // (target, replica) pairs for the direct part.
 //
 // Precision: FP64 everywhere for the plain form.  The mixed form runs the
 // direct part in FP32 on coordinates relative to the target leaf's centre
-// (sums of four pairs in FP32, accumulated in FP64) and the contraction on
-// FP32 MFMA (promoted to FP64 every 8 cells); expansions (S2M/M2M) and the
-// L2P accumulation stay in FP64.
+// (sums of eight pairs in FP32, accumulated in FP64), the smooth cells in
+// FP32 (each value added to the FP64 sums), and the contraction on FP32 MFMA
+// (promoted to FP64 every 8 cells); expansions (S2M/M2M), L2L and the L2P
+// accumulation stay in FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -509,8 +517,11 @@ __device__ inline LeafCtx leaf_ctx(const FmmArgs &a, int li, int lane) {
   L.spos = (lane % L.nrep) * L.stride + lane / L.nrep;
   L.tact = L.rep < L.nrep;
   L.tj = L.NA.first + L.ti;
-  L.tx = a.sx[L.tj];
-  L.ty = a.sy[L.tj];
+  // FP64 target coordinates: the plain form's pairs; the mixed form needs
+  // them only for the L2P at the end and loads them there (nothing to keep
+  // live through the walk)
+  L.tx = MIXED ? 0.0 : a.sx[L.tj];
+  L.ty = MIXED ? 0.0 : a.sy[L.tj];
   L.lx = 0.f;
   L.ly = 0.f;
   if (MIXED) {
@@ -1612,7 +1623,12 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
   double phr = 0.0, phi = 0.0;
   if (L.tact && L.rep == 0) {
     // L2P: sum_l Lt_l u^l, u = (z - c_A)/s_A
-    const double ur = (L.tx - NA.cx) / sA, ui = (L.ty - NA.cy) / sA;
+    // the target's index formed anew (opaque to the compiler), so that no
+    // copy of it is kept through the walk
+    int tj = L.tj;
+    asm volatile("" : "+v"(tj));
+    const double tx = MIXED ? a.sx[tj] : L.tx, ty = MIXED ? a.sy[tj] : L.ty;
+    const double ur = (tx - NA.cx) / sA, ui = (ty - NA.cy) / sA;
     double2 s = s_loc[w][pA - 1];
     for (int l = pA - 2; l >= 0; l--) {
       const double2 Lc = s_loc[w][l];
@@ -1668,7 +1684,9 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
       u = u + 0.5 * extra.x / M_PI;
       v = v + 0.5 * extra.y / M_PI;
     }
-    a.uv[a.perm[L.tj]] = make_double2(u, v);
+    int tj = L.tj;
+    asm volatile("" : "+v"(tj));
+    a.uv[a.perm[tj]] = make_double2(u, v);
   }
   wave_sync();
 }

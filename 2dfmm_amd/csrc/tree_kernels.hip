@@ -5,9 +5,11 @@
 //             with the parent->child centre recurrence of quadtree.c:61-79 and
 //             the tie rule x >= cx, y <= cy of quadtree.c:29-35
 //   radix     stable LSD sort of (key, blob id), 8-bit digits, LDS ranking
-//   levels    one launch per level: children of every split node found by
-//             binary search on the sorted keys; leaf iff count < num_p
-//             (quadtree.c:197-245)
+//   levels    two launches per level with the counts kept on the device:
+//             children of every split node by wave searches on the sorted
+//             keys, leaf iff count < num_p (quadtree.c:197-245); the split
+//             children compacted into the next level's list by a single-pass
+//             scan with decoupled look-back
 //   leaf      within every leaf, blobs in ascending id (the stable partition
 //             order of quadtree.c:219-225)
 #include <hip/hip_runtime.h>
