@@ -253,6 +253,19 @@ __device__ inline int smooth_slot(int i, int j) {
   return Q + 1 + (Q - 1 - j) * (Q - j) / 2 + (Q - 1 - j - i);
 }
 
+// k! for k < 2 kSmoothQ + 2 (exact in FP64: 17! < 2^53), a constant table
+// rather than a per-lane array (which would live in scratch memory)
+struct SmoothFact {
+  double v[2 * kSmoothQ + 2];
+};
+constexpr SmoothFact make_smooth_fact() {
+  SmoothFact t{};
+  t.v[0] = 1.0;
+  for (int k = 1; k < 2 * kSmoothQ + 2; k++) t.v[k] = t.v[k - 1] * (double)k;
+  return t;
+}
+__constant__ SmoothFact kSmoothFact = make_smooth_fact();
+
 // One wave per node.  The node's blobs go through in chunks of 64: lane j
 // forms cir_j d_j^m and conj(d_j)^n (m, n <= q) of blob j in LDS, then lane
 // t < (q+1)(q+2)/2 adds the products of term t = (m, n) over the chunk (FP64).
@@ -278,9 +291,7 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
   const int cnt = node_count(C.cnt);
   if (!(C.region <= region && 2.0 * C.region > region && cnt >= smin)) return;
   const int lane = threadIdx.x;
-  double fact[2 * Q + 2];
-  fact[0] = 1.0;
-  for (int k = 1; k < 2 * Q + 2; k++) fact[k] = fact[k - 1] * (double)k;
+  const double *fact = kSmoothFact.v;
   int m = 0, n = 0;
   if (lane < NT) {
     while (lane >= smooth_tri(m + 1)) m++;

@@ -295,8 +295,9 @@ int launch_mirror_images(int n, const double *d_xy, const double *d_cir, double 
 
 // ---- learned per-leaf parameters (learned_kernels.hip) --------------------
 constexpr int kMlpWeights = 4866;  // 8 -> 64 -> 64 -> 2
-// d_mean_abs: kMeanAbsWords doubles; [0] gets mean |cir| (computed in a fixed
-// order), the rest is scratch for block partials.
+// d_cir: the n circulations in leaf order; d_mean_abs: kMeanAbsWords
+// doubles, [0] gets mean |cir| (computed in a fixed order), the rest is
+// scratch for block partials.
 constexpr int kMeanAbsWords = 1 + 1024;
 int launch_leaf_features(const FmmArgs &a, int leaf_cap, const double *d_cir, int n,
                          double *d_mean_abs, float *d_feat, hipStream_t s);
