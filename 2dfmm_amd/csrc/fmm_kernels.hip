@@ -720,16 +720,14 @@ __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeR
 }
 
 // Direct sums of the targets of L against tiles ta..tb-1 (64 sources each)
-// of near subtree c, whose sources are one contiguous run spanning many
-// leaves.  Each tile is staged in LDS -- the mixed form converts each source
+// of a contiguous run of cnt sources from s0 spanning several leaves (a whole
+// near subtree, or consecutive near leaves of one parent).  Each tile is staged in LDS -- the mixed form converts each source
 // once, to FP32 coordinates relative to the target leaf's centre -- and the
 // next tile's loads are issued before the current tile's sums.
 template <bool MIXED, bool UNI>
-__device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
-                                  bool ser, int ta, int tb, int lane,
-                                  typename StageOf<MIXED>::T *st, double &ax, double &ay) {
-  const int s0 = rfl(ldro(a.nfirst + c));
-  const int cnt = node_count(C.cnt);
+__device__ inline void near_range(const FmmArgs &a, const LeafCtx &L, int s0, int cnt, bool ser,
+                                  int ta, int tb, int lane, typename StageOf<MIXED>::T *st,
+                                  double &ax, double &ay) {
   double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
   float2 zw = make_float2(0.f, 0.f);
   auto load = [&](int t) {
@@ -786,16 +784,21 @@ __device__ inline bool whole_near(double inv_theta, double RA, double rcut, cons
 
 // Near items are leaves (one step), whole near subtrees (one step per tile
 // of 64 sources), smooth cells (one step; summed in batches, see
-// smooth_batch) and smooth runs: a whole near subtree whose blobs all lie in
+// smooth_batch), smooth runs: a whole near subtree whose blobs all lie in
 // smooth-level nodes, one step per tile of 64 consecutive smooth-level nodes
-// (smooth cells and small leaves).
-enum ItemKind { kItemLeaf = 0, kItemRange = 1, kItemSmooth = 2, kItemSmoothRun = 3 };
+// (smooth cells and small leaves), and leaf runs: near leaves that follow
+// each other among the children of one node (only empty children between
+// them), whose sources are one contiguous run -- summed like a whole near
+// subtree, one step per tile of 64 (the record passed for a leaf run holds
+// the first blob and the total count).
+enum ItemKind { kItemLeaf = 0, kItemRange = 1, kItemSmooth = 2, kItemSmoothRun = 3, kItemRun = 4 };
+__device__ inline bool tiled_run(int kind) { return kind == kItemRange || kind == kItemRun; }
 __device__ inline int item_tiles(const FmmArgs &a, int c, const NodeRec &C, int kind) {
   if (kind == kItemSmoothRun) {
     const int2 r = ldro(a.srange + c);
     return (rfl(r.y) - rfl(r.x) + 63) >> 6;
   }
-  return kind == kItemRange ? (node_count(C.cnt) + 63) >> 6 : 1;
+  return tiled_run(kind) ? (node_count(C.cnt) + 63) >> 6 : 1;
 }
 __device__ inline int tile_count(const NodeRec &C, bool range, int t) {
   return range ? min(64, node_count(C.cnt) - 64 * t) : node_count(C.cnt);
@@ -992,13 +995,15 @@ __device__ inline void smooth_batch(const LeafCtx &L, int nb, int lane, float4 *
 // Steps ta..tb-1 of near item c.
 template <bool MIXED, bool UNI>
 __device__ inline void near_steps(const FmmArgs &a, const LeafCtx &L, int c, const NodeRec &C,
-                                  bool range, bool ser, int ta, int tb, int lane,
+                                  int kind, bool ser, int ta, int tb, int lane,
                                   typename StageOf<MIXED>::T *st, double &ax,
                                   double &ay) {
-  if (range)
-    near_range<MIXED, UNI>(a, L, c, C, ser, ta, tb, lane, st, ax, ay);
-  else
+  if (tiled_run(kind)) {
+    const int s0 = kind == kItemRun ? rfl(C.first) : rfl(ldro(a.nfirst + c));
+    near_range<MIXED, UNI>(a, L, s0, node_count(C.cnt), ser, ta, tb, lane, st, ax, ay);
+  } else {
     near_leaf<MIXED, UNI>(a, L, C, ser, lane, st, ax, ay);
+  }
 }
 
 // Steps ta..tb-1 of near item c of any kind; smooth cells go to push(cell)
@@ -1037,7 +1042,7 @@ __device__ inline void item_steps(const FmmArgs &a, const LeafCtx &L, int c, con
       return;
     }
   }
-  near_steps<MIXED, UNI>(a, L, c, C, kind == kItemRange, ser, ta, tb, lane, st, ax, ay);
+  near_steps<MIXED, UNI>(a, L, c, C, kind, ser, ta, tb, lane, st, ax, ay);
 }
 
 
@@ -1066,6 +1071,7 @@ __device__ inline bool far_cell(double inv_theta, double RA, double rcut, const 
 // entries per level, which kStackMargin covers.
 constexpr int kStack = 384;
 constexpr int kStackMargin = 96;
+constexpr bool kMergeLeaves = true;  // consecutive sibling near leaves as one run
 
 // ---- local expansions inherited from the parent (L2L) ----------------------
 // A far cell of target leaf A that is also far from A's parent P is summed
@@ -1274,11 +1280,33 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
     ns += __builtin_popcountll(om);
     wave_sync();
     if (fm) far(fm, c);
+    // near leaves in one group of four children (lanes 4j..4j+3) with only
+    // empty children between them are one item (kItemRun)
+    const uint64_t lm = kMergeLeaves ? __builtin_amdgcn_ballot_w64(it && kind == kItemLeaf) : 0;
+    const uint64_t em =
+        kMergeLeaves ? __builtin_amdgcn_ballot_w64(c >= 0 && node_count(C.cnt) == 0) : 0;
     uint64_t m = im;
     while (m) {
       const int b = __builtin_ctzll(m);
       m &= m - 1;
-      if (!item(lane_of(c, b), node_of_lane(C, b), lane_of(kind, b), (sm >> b) & 1)) return;
+      NodeRec Cb = node_of_lane(C, b);
+      int kb = lane_of(kind, b);
+      bool ser = (sm >> b) & 1;
+      if (kMergeLeaves && kb == kItemLeaf) {
+        int tot = node_count(Cb.cnt);
+        for (int b2 = b + 1; b2 <= (b | 3); b2++) {
+          if ((lm >> b2) & 1) {
+            tot += node_count(lane_of(C.cnt, b2));
+            ser = ser || ((sm >> b2) & 1);
+            m &= ~(1ull << b2);
+            kb = kItemRun;
+          } else if (!((em >> b2) & 1)) {
+            break;
+          }
+        }
+        if (kb == kItemRun) Cb.cnt = tot;
+      }
+      if (!item(lane_of(c, b), Cb, kb, ser)) return;
     }
   }
 }
@@ -1321,7 +1349,7 @@ __device__ inline int64_t item_prefix(const FmmArgs &a, const LeafCtx &L, int c,
   if (t <= 0) return 0;
   if (kind == kItemSmooth) return L.su;
   if (kind == kItemLeaf) return near_units(node_count(C.cnt), L);
-  if (kind == kItemRange) {
+  if (tiled_run(kind)) {
     int64_t u = (int64_t)min(t, nt - 1) * near_units(64, L);
     if (t >= nt) u += near_units(node_count(C.cnt) - 64 * (nt - 1), L);
     return u;
