@@ -836,10 +836,10 @@ __device__ inline void smooth_eval(float zx, float zy, const float4 *row, float 
     }
     pr = fmaf(pr, zy, qr);
     pi = fmaf(pi, zy, qi);
-    // the next row's coefficient reads stay after this row's arithmetic (all
-    // rows' reads at once would hold ~90 registers): an empty asm statement
-    // that takes the running sum and orders memory
-    asm volatile("" : "+v"(pr), "+v"(pi) : : "memory");
+    // the reads of the rows after the next stay after this row's arithmetic
+    // (all rows' reads at once would hold ~90 registers): an empty asm
+    // statement every second row that takes the running sum and orders memory
+    if (b & 1) asm volatile("" : "+v"(pr), "+v"(pi) : : "memory");
   }
   gx += fmaf(E, pi, s0i);
   gy += fmaf(E, pr, s0r);
