@@ -506,13 +506,15 @@ def main():
         log(f"[bench] cpu baseline {cpu['value']:.4g} velocities/s ({cpu['sample']})")
 
     peak = PEAK[head["precision"]]
-    traffic = None
+    traffic, bound, util = None, "valu", None
     if args.traffic and os.path.exists(args.traffic) and head_mode != "treecode":
         with open(args.traffic) as f:
             tr = json.load(f)
         t = tr.get(head_mode) or tr.get(head["precision"])
         if t:
             traffic = t["bytes"]
+            # the bound is the unit the profiled counters show most utilised
+            bound, util = t.get("bound", bound), t.get("utilisation")
     pk = per_kernel(head, n, head["p"], head_mode != "treecode")
     out = {
         "metric": METRIC,
@@ -539,14 +541,15 @@ def main():
         "ablation": ablation,
         "modes": results,
         "layouts": layouts,
-        "roofline": {"kernel": head["kernel"], "bound": "valu",
+        "roofline": {"kernel": head["kernel"], "bound": bound, "utilisation": util,
                      "achieved": head["achieved_tflops"], "peak": peak, "unit": "TFLOP/s",
                      "frac": head["achieved_tflops"] / peak, "traffic": traffic,
                      "flops_per_launch": head["flops"], "avg_launch_ms": head["phases_ms"]["evaluate"],
                      "note": ("the evaluation (fused pass + continuation chunks + chunk sums); "
                               "near pairs and smooth cells run on the FP32 vector ALU, the far "
-                              "field on FP32 MFMA; the counters show the VALU busy, so the bound "
-                              "is the vector ALU; flops count each far cell at its leaf's own p"),
+                              "field on FP32 MFMA; bound and utilisation come from the rocprof "
+                              "counters of the profiles file (the most utilised unit); flops count "
+                              "each far cell at its leaf's own p"),
                      "per_kernel": pk},
         "cpu_baseline": cpu,
         "host_inclusive": host,
