@@ -283,8 +283,10 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
   __shared__ double2 mu[NT];
   __shared__ double2 cz[kSmoothC];   // c_ij lam^(i+j) in storage order
   constexpr int B = 32;               // blobs per chunk (LDS: 4 waves' worth per 64 KB)
-  __shared__ double2 pm[Q + 1][B];  // cir d^m
-  __shared__ double2 pn[Q + 1][B];  // conj(d)^n
+  // rows padded by one entry: the lanes of one step read rows m (and n) at
+  // the same column, which would otherwise fall into the same LDS banks
+  __shared__ double2 pm[Q + 1][B + 1];  // cir d^m
+  __shared__ double2 pn[Q + 1][B + 1];  // conj(d)^n
   const int node = node0 + blockIdx.x;
   if (node >= nnodes) return;
   const NodeRec C = nodes[node];
@@ -299,12 +301,19 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
   }
   const int f = nfirst[node];
   double ar = 0.0, ai = 0.0;
+  // this lane's blob of the chunk, loaded a chunk ahead (the loads of the
+  // next chunk overlap this chunk's sums)
+  double bx = 0.0, by = 0.0, bg = 0.0;
+  if (lane < min(B, cnt)) {
+    bx = sx[f + lane];
+    by = sy[f + lane];
+    bg = sg[f + lane];
+  }
   for (int j0 = 0; j0 < cnt; j0 += B) {
     const int nb = min(B, cnt - j0);
     if (lane < nb) {
-      const int j = f + j0 + lane;
-      const double dr = (C.cx - sx[j]) * inv_sigma, di = (C.cy - sy[j]) * inv_sigma;
-      double pr = sg[j], pi = 0.0, qr = 1.0, qi = 0.0;
+      const double dr = (C.cx - bx) * inv_sigma, di = (C.cy - by) * inv_sigma;
+      double pr = bg, pi = 0.0, qr = 1.0, qi = 0.0;
       for (int k = 0; k <= Q; k++) {
         pm[k][lane] = make_double2(pr, pi);
         pn[k][lane] = make_double2(qr, qi);
@@ -317,6 +326,12 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
       }
     }
     __syncthreads();
+    if (lane < min(B, cnt - j0 - B)) {
+      const int j = f + j0 + B + lane;
+      bx = sx[j];
+      by = sy[j];
+      bg = sg[j];
+    }
     if (lane < NT) {
       for (int j = 0; j < nb; j++) {
         const double2 u = pm[m][j], v = pn[n][j];
