@@ -189,15 +189,19 @@ __global__ __launch_bounds__(256) void parity_eval_kernel(TreeView tv, int nt,
 // popped leaf the lanes of the mask sum it directly.  Restricted to one lane
 // the events are that target's own walk (parity_eval_kernel) in the same
 // order with the same arithmetic, so the velocities are bit for bit the same;
-// here node records and coefficients are read at wave-uniform addresses and
-// the lanes of a leaf walk together.  The stack holds at most 3 entries per
-// level plus one.
+// here the lanes of a leaf walk together, and the four children's
+// coefficients (for M2S) and a popped leaf's sources (for the direct sums)
+// are staged in LDS, one vector load per lane, and read from there in the
+// same order as the per-target walk reads them from memory.  The stack holds
+// at most 3 entries per level plus one.
 constexpr int kParStack = 3 * kMaxDepth + 4;
 __global__ __launch_bounds__(64) void parity_leaf_kernel(TreeView tv, int nnodes,
                                                          const uint32_t *__restrict__ perm,
                                                          double2 *__restrict__ uv) {
   __shared__ int s_node[kParStack];
   __shared__ uint64_t s_mask[kParStack];
+  __shared__ double2 s_coef[4 * kMaxTerms];  // the four children's a_k
+  __shared__ double4 s_src[64];              // (x, y, cir, core radius) of a leaf's blobs
   const int leaf = blockIdx.x, lane = threadIdx.x;
   if (leaf >= nnodes) return;
   const NodeRec Lf = tv.nodes[leaf];
@@ -229,10 +233,46 @@ __global__ __launch_bounds__(64) void parity_leaf_kernel(TreeView tv, int nnodes
       const bool mine = (m >> lane) & 1;
       const NodeRec N = nodes[node];
       if (!node_internal(N.cnt)) {
-        if (mine) p2p_leaf(tv, N.first, node_count(N.cnt), x, y, ud, vd);
+        // p2p_leaf's arithmetic, sources from LDS in 64-blob chunks
+        const int lo = N.first, nc = node_count(N.cnt);
+        double ax = 0.0, ay = 0.0;
+        for (int j0 = 0; j0 < nc; j0 += 64) {
+          if (j0 + lane < nc) {
+            const int j = lo + j0 + lane;
+            s_src[lane] = make_double4(tv.sx[j], tv.sy[j], tv.sg[j], tv.ss[j]);
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          if (mine) {
+            const int jn = min(64, nc - j0);
+            for (int j = 0; j < jn; j++) {
+              const double4 sj = s_src[j];
+              const double rx = x - sj.x;
+              const double ry = y - sj.y;
+              const double r = sqrt(rx * rx + ry * ry);
+              if (r > 1.0e-10) {
+                const double rv2 = 1.0 / (r * r);
+                const double xai = r / sj.w;
+                const double rv = sj.z * (1.0 - exp(-xai * xai));
+                ax -= rv * ry * rv2;
+                ay += rv * rx * rv2;
+              }
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+        if (mine) {
+          ud += 0.5 * ax / M_PI;
+          vd += 0.5 * ay / M_PI;
+        }
         continue;
       }
       const int base = N.first;
+      // the children's coefficients: rows base .. base + 3, consecutive
+      for (int e = lane; e < 4 * p; e += 64) s_coef[e] = tv.coef[(size_t)base * p + e];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
       uint64_t om[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
@@ -245,7 +285,7 @@ __global__ __launch_bounds__(64) void parity_leaf_kernel(TreeView tv, int nnodes
           if (C.region > theta * r)
             op = true;
           else if (node_count(C.cnt) > 0)
-            m2s_horner(tv.coef + (size_t)c * p, p, dx, dy, phr, phi);
+            m2s_horner(s_coef + q * p, p, dx, dy, phr, phi);
         }
         om[q] = __builtin_amdgcn_ballot_w64(op);
       }
