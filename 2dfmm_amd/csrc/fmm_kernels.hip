@@ -415,9 +415,6 @@ __device__ inline uint64_t rfl64(uint64_t v) {
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
   return ((uint64_t)hi << 32) | lo;
 }
-__device__ inline double rfl64d(double v) {
-  return __builtin_bit_cast(double, rfl64(__builtin_bit_cast(uint64_t, v)));
-}
 
 // The value of lane b (wave-uniform) in every lane: v_readlane, no LDS
 // round trip.
@@ -625,10 +622,10 @@ __device__ inline bool needs_series(const FmmArgs &a, const LeafCtx &L, const No
 // are staged once in this wave's LDS slice `st` (one vector load per lane)
 // and read from there, so each source crosses the vector memory path once
 // per wave; with one replica every lane reads the same LDS word (a
-// broadcast).  This measured 5% faster for the mixed form than reading the
+// broadcast).  For the mixed form this measured 5% faster than reading the
 // sources at wave-uniform addresses through the scalar cache, four loads at
-// a time, whose latency the pair arithmetic did not cover (the same for the
-// plain form).
+// a time, whose latency the pair arithmetic did not cover; the plain form
+// measured the same either way.
 template <bool MIXED, bool UNI>
 __device__ inline void near_leaf(const FmmArgs &a, const LeafCtx &L, const NodeRec &C, bool ser,
                                  int lane, typename StageOf<MIXED>::T *st, double &ax,
@@ -741,9 +738,6 @@ __device__ inline int item_tiles(const FmmArgs &a, int c, const NodeRec &C, int 
     return (rfl(r.y) - rfl(r.x) + 63) >> 6;
   }
   return tiled_run(kind) ? (node_count(C.cnt) + 63) >> 6 : 1;
-}
-__device__ inline int tile_count(const NodeRec &C, bool range, int t) {
-  return range ? min(64, node_count(C.cnt) - 64 * t) : node_count(C.cnt);
 }
 
 // ---- smooth cells ----------------------------------------------------------
