@@ -40,41 +40,58 @@ struct Slots {
   }
 };
 
+// 1 / d for d > 0: the hardware reciprocal refined by two Newton steps
+// (within an ulp or so of the correctly rounded quotient; the velocities are
+// held to 1e-12 of the reference, the opening decisions keep the reference's
+// exact arithmetic)
+__device__ inline double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = r * fma(-d, r, 2.0);
+  return r * fma(-d, r, 2.0);
+}
+
 // phi += sum_{k=0}^{p-2} a_k (z - z_c)^{-(k+1)}, Horner in w = 1 / (z - z_c)
+// with fused multiply-adds
 __device__ inline void m2s_horner(const double2 *__restrict__ a, int p, double dx, double dy,
                                   double &phr, double &phi) {
   if (p < 2) return;
-  const double inv = 1.0 / (dx * dx + dy * dy);
+  const double inv = rcp_nr(fma(dx, dx, dy * dy));
   const double wr = dx * inv, wi = -dy * inv;
   double2 t = a[p - 2];
   double sr = t.x, si = t.y;
   for (int k = p - 3; k >= 0; k--) {
     const double2 ak = a[k];
-    const double nr = sr * wr - si * wi + ak.x;
-    const double ni = sr * wi + si * wr + ak.y;
+    const double nr = fma(sr, wr, fma(-si, wi, ak.x));
+    const double ni = fma(sr, wi, fma(si, wr, ak.y));
     sr = nr;
     si = ni;
   }
-  phr += sr * wr - si * wi;
-  phi += sr * wi + si * wr;
+  phr += fma(sr, wr, -si * wi);
+  phi += fma(sr, wi, si * wr);
+}
+
+// One pair of fmm.c:121-142 (regularised blob, cut-off r > 1e-10), source
+// (sx, sy, cir, core radius) = s; r from the correctly rounded sqrt as in
+// the reference, the divisions by reciprocals
+__device__ inline void p2p_pair(double x, double y, const double4 &s, double &ax, double &ay) {
+  const double rx = x - s.x;
+  const double ry = y - s.y;
+  const double r = sqrt(fma(rx, rx, ry * ry));
+  if (r > 1.0e-10) {
+    const double rv2 = rcp_nr(r * r);
+    const double xai = r * rcp_nr(s.w);
+    const double rv = s.z * (1.0 - exp(-xai * xai));
+    ax -= rv * ry * rv2;
+    ay += rv * rx * rv2;
+  }
 }
 
 // fmm.c:121-142 for one leaf
 __device__ inline void p2p_leaf(const TreeView &tv, int lo, int cnt, double x, double y,
                                 double &ud, double &vd) {
   double ax = 0.0, ay = 0.0;
-  for (int j = lo; j < lo + cnt; j++) {
-    const double rx = x - tv.sx[j];
-    const double ry = y - tv.sy[j];
-    const double r = sqrt(rx * rx + ry * ry);
-    if (r > 1.0e-10) {
-      const double rv2 = 1.0 / (r * r);
-      const double xai = r / tv.ss[j];
-      const double rv = tv.sg[j] * (1.0 - exp(-xai * xai));
-      ax -= rv * ry * rv2;
-      ay += rv * rx * rv2;
-    }
-  }
+  for (int j = lo; j < lo + cnt; j++)
+    p2p_pair(x, y, make_double4(tv.sx[j], tv.sy[j], tv.sg[j], tv.ss[j]), ax, ay);
   ud += 0.5 * ax / M_PI;
   vd += 0.5 * ay / M_PI;
 }
@@ -245,19 +262,7 @@ __global__ __launch_bounds__(64) void parity_leaf_kernel(TreeView tv, int nnodes
           __builtin_amdgcn_wave_barrier();
           if (mine) {
             const int jn = min(64, nc - j0);
-            for (int j = 0; j < jn; j++) {
-              const double4 sj = s_src[j];
-              const double rx = x - sj.x;
-              const double ry = y - sj.y;
-              const double r = sqrt(rx * rx + ry * ry);
-              if (r > 1.0e-10) {
-                const double rv2 = 1.0 / (r * r);
-                const double xai = r / sj.w;
-                const double rv = sj.z * (1.0 - exp(-xai * xai));
-                ax -= rv * ry * rv2;
-                ay += rv * rx * rv2;
-              }
-            }
+            for (int j = 0; j < jn; j++) p2p_pair(x, y, s_src[j], ax, ay);
           }
           __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
           __builtin_amdgcn_wave_barrier();
