@@ -336,7 +336,7 @@ def set_l2l(min_children) -> None:
 
 
 def set_treecode_walk(per_leaf: bool) -> None:
-    """Treecode at the blobs: a wave per leaf (True) or a lane per target."""
+    """Treecode at the blobs: a wave per 64 consecutive blobs (True) or a lane per target."""
     _check(lib.fmm_set_treecode_walk(1 if per_leaf else 0), "set_treecode_walk")
 
 

@@ -72,14 +72,17 @@ __device__ inline void m2s_horner(const double2 *__restrict__ a, int p, double d
 
 // One pair of fmm.c:121-142 (regularised blob, cut-off r > 1e-10), source
 // (sx, sy, cir, core radius) = s; r from the correctly rounded sqrt as in
-// the reference, the divisions by reciprocals
+// the reference, the divisions by reciprocals; a zero core radius keeps the
+// reference's infinite r / 0
 __device__ inline void p2p_pair(double x, double y, const double4 &s, double &ax, double &ay) {
   const double rx = x - s.x;
   const double ry = y - s.y;
   const double r = sqrt(fma(rx, rx, ry * ry));
   if (r > 1.0e-10) {
     const double rv2 = rcp_nr(r * r);
-    const double xai = r * rcp_nr(s.w);
+    // a zero core radius (mirror images) is a point vortex: r / 0 = inf in
+    // the reference, and rcp_nr(0) would turn 0 * inf into NaN
+    const double xai = s.w > 0.0 ? r * rcp_nr(s.w) : __builtin_inf();
     const double rv = s.z * (1.0 - exp(-xai * xai));
     ax -= rv * ry * rv2;
     ay += rv * rx * rv2;
@@ -197,12 +200,15 @@ __global__ __launch_bounds__(256) void parity_eval_kernel(TreeView tv, int nt,
   }
 }
 
-// Treecode at the blobs, one wave per leaf (its blobs are the targets, 64 at a
-// time): the wave walks the union of the nodes its targets open, depth first
-// in the reference's order -- children tested in quadrant order 0..3, opened
-// ones visited 3..0 -- each stack entry carrying the mask of lanes that opened
-// it.  At a popped internal node every lane of the mask tests the four
-// children itself and evaluates the ones it does not open by M2S at once; at a
+// Treecode at the blobs, one wave per 64 consecutive blobs in leaf order (the
+// targets: a leaf and its neighbours, which open nearly the same nodes; a
+// leaf holds fewer than 64 blobs, so one wave per leaf left half its lanes
+// without a target): the wave walks the union of the nodes its targets open,
+// depth first in the reference's order -- children tested in quadrant order
+// 0..3, opened ones visited 3..0 -- each stack entry carrying the mask of
+// lanes that opened it, one root entry per tree among the targets (forests).
+// At a popped internal node every lane of the mask tests the four children
+// itself and evaluates the ones it does not open by M2S at once; at a
 // popped leaf the lanes of the mask sum it directly.  Restricted to one lane
 // the events are that target's own walk (parity_eval_kernel) in the same
 // order with the same arithmetic, so the velocities are bit for bit the same;
@@ -210,34 +216,38 @@ __global__ __launch_bounds__(256) void parity_eval_kernel(TreeView tv, int nt,
 // coefficients (for M2S) and a popped leaf's sources (for the direct sums)
 // are staged in LDS, one vector load per lane, and read from there in the
 // same order as the per-target walk reads them from memory.  The stack holds
-// at most 3 entries per level plus one.
-constexpr int kParStack = 3 * kMaxDepth + 4;
-__global__ __launch_bounds__(64) void parity_leaf_kernel(TreeView tv, int nnodes,
+// at most 3 entries per level plus one per tree among the 64 targets.
+constexpr int kParStack = 3 * kMaxDepth + 1 + 64;
+__global__ __launch_bounds__(64) void parity_leaf_kernel(TreeView tv, int n,
                                                          const uint32_t *__restrict__ perm,
                                                          double2 *__restrict__ uv) {
   __shared__ int s_node[kParStack];
   __shared__ uint64_t s_mask[kParStack];
   __shared__ double2 s_coef[4 * kMaxTerms];  // the four children's a_k
   __shared__ double4 s_src[64];              // (x, y, cir, core radius) of a leaf's blobs
-  const int leaf = blockIdx.x, lane = threadIdx.x;
-  if (leaf >= nnodes) return;
-  const NodeRec Lf = tv.nodes[leaf];
-  const int cnt = node_count(Lf.cnt);
-  if (node_internal(Lf.cnt) || cnt == 0) return;
+  const int lane = threadIdx.x;
   const NodeRec *__restrict__ nodes = tv.nodes;
   const int p = tv.nterms;
   const double theta = tv.theta;
-  // root: 0 for a single tree; in a forest, the system of the leaf's blobs
-  const int root = tv.root_shift < 0 ? 0 : (int)(tv.key[Lf.first] >> tv.root_shift);
-  for (int b0 = 0; b0 < cnt; b0 += 64) {
-    const int t = Lf.first + b0 + lane;
-    const bool act = b0 + lane < cnt;
+  {
+    const int t = blockIdx.x * 64 + lane;
+    const bool act = t < n;
     const double x = act ? tv.sx[t] : 0.0, y = act ? tv.sy[t] : 0.0;
     double phr = 0.0, phi = 0.0, ud = 0.0, vd = 0.0;
-    int ns = 1;
-    if (lane == 0) {
-      s_node[0] = root;
-      s_mask[0] = cnt - b0 >= 64 ? ~0ull : ((1ull << (cnt - b0)) - 1ull);  // the active lanes
+    // root: 0 for a single tree; in a forest, the system of the blob; one
+    // stack entry per root among the targets (disjoint masks, any order)
+    const int root = tv.root_shift < 0 ? 0 : (act ? (int)(tv.key[t] >> tv.root_shift) : -1);
+    int ns = 0;
+    uint64_t left = __builtin_amdgcn_ballot_w64(act);
+    while (left) {
+      const int r = __builtin_amdgcn_readlane(root, __builtin_ctzll(left));
+      const uint64_t m = __builtin_amdgcn_ballot_w64(act && root == r);
+      if (lane == 0) {
+        s_node[ns] = r;
+        s_mask[ns] = m;
+      }
+      ns++;
+      left &= ~m;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -313,10 +323,10 @@ __global__ __launch_bounds__(64) void parity_leaf_kernel(TreeView tv, int nnodes
   }
 }
 
-int launch_parity_leaves(const TreeView &tv, int nnodes, const uint32_t *d_perm, double *d_uv,
+int launch_parity_leaves(const TreeView &tv, int n, const uint32_t *d_perm, double *d_uv,
                          hipStream_t s) {
-  if (nnodes <= 0) return 0;
-  parity_leaf_kernel<<<nnodes, 64, 0, s>>>(tv, nnodes, d_perm, reinterpret_cast<double2 *>(d_uv));
+  if (n <= 0) return 0;
+  parity_leaf_kernel<<<(n + 63) / 64, 64, 0, s>>>(tv, n, d_perm, reinterpret_cast<double2 *>(d_uv));
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -121,7 +121,7 @@ struct Context {
   // splitting of target leaves with much near work (0: off)
   int split_budget = 8192;
   int far_burst = 128;  // far cells per MFMA burst (64..128)
-  bool parity_leaves = true;  // treecode at the blobs: a wave per leaf (else a lane per target)
+  bool parity_leaves = true;  // treecode at the blobs: a wave per 64 blobs (else a lane per target)
   int l2l = 2;          // parent local expansions for parents of >= l2l target leaves (0: off)
   DevBuf<double2> plocal;
   DevBuf<int32_t> nodeli;
@@ -676,7 +676,7 @@ static int eval_sources_dev(double *d_uv) {
   const bool mixed = g.t_mode == FMM2D_MODE_MIXED;
   if (g.t_mode != FMM2D_MODE_TREECODE && ensure_split()) return -1;
   if (g.t_mode == FMM2D_MODE_TREECODE)
-    rc = g.parity_leaves ? launch_parity_leaves(view(), g.nnodes, g.perm.p, d_uv, g.stream)
+    rc = g.parity_leaves ? launch_parity_leaves(view(), g.n, g.perm.p, d_uv, g.stream)
                          : launch_parity_eval(view(), g.n, nullptr, g.perm.p, d_uv, g.stream);
   else if (g.persist_blocks > 0 && g.use_graph)
     rc = eval_fmm_graph(fmm_args(d_uv), mixed);

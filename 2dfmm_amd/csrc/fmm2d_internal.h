@@ -154,9 +154,10 @@ struct TreeView {
 };
 // Treecode velocities at targets (txy interleaved; or the sorted blobs
 // themselves when txy is null, then results are scattered through perm).
-// Treecode velocities at every blob, one wave per leaf walking the union of
-// its targets' opened nodes (bitwise those of launch_parity_eval at the blobs).
-int launch_parity_leaves(const TreeView &tv, int nnodes, const uint32_t *d_perm, double *d_uv,
+// Treecode velocities at all n blobs, one wave per 64 consecutive blobs in
+// leaf order walking the union of its targets' opened nodes (bitwise those of
+// launch_parity_eval at the blobs).
+int launch_parity_leaves(const TreeView &tv, int n, const uint32_t *d_perm, double *d_uv,
                          hipStream_t s);
 int launch_parity_eval(const TreeView &tv, int nt, const double *d_txy, const uint32_t *d_perm,
                        double *d_uv, hipStream_t s);

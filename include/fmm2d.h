@@ -88,10 +88,10 @@ int fmm_set_fmm_params(int p, int leaf, double theta, double cut_sigmas);
  * plain loop on the device; *max_abs_err gets the largest difference. */
 int fmm_selftest_mfma(int fp32, double *max_abs_err);
 
-/* Treecode mode at the blobs: 1 (default) one wave per leaf walking the union
- * of its targets' opened nodes, 0 one lane per target walking alone; the
- * results are the same bits (each target meets its own events in the
- * reference's order either way). */
+/* Treecode mode at the blobs: 1 (default) one wave per 64 consecutive blobs in
+ * leaf order walking the union of its targets' opened nodes, 0 one lane per
+ * target walking alone; the results are the same bits (each target meets its
+ * own events in the reference's order either way). */
 int fmm_set_treecode_walk(int per_leaf);
 
 /* 0: make_tree does not evaluate at the blobs; 1 (default): it does,
