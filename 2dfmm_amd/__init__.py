@@ -353,7 +353,7 @@ def get_split() -> int:
     return int(lib.fmm_get_split())
 
 
-def set_smooth(eta: float, min_blobs: int = 16) -> None:
+def set_smooth(eta: float, min_blobs: int = 48) -> None:
     """Smooth cells of the mixed mode (one core radius): near nodes of half side
     <= eta * sigma with >= min_blobs blobs are summed through a Taylor expansion
     of the regularised kernel (eta 0: off).  Takes effect at the next build."""

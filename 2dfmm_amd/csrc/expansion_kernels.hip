@@ -253,24 +253,30 @@ __device__ inline int smooth_slot(int i, int j) {
   return Q + 1 + (Q - 1 - j) * (Q - j) / 2 + (Q - 1 - j - i);
 }
 
-// k! for k < 2 kSmoothQ + 2 (exact in FP64: 17! < 2^53), a constant table
-// rather than a per-lane array (which would live in scratch memory)
+// k! for k <= kSmoothQ (exact in FP64: 12! < 2^53; no formula below needs
+// more), a constant table rather than a per-lane array (which would live in
+// scratch memory)
 struct SmoothFact {
-  double v[2 * kSmoothQ + 2];
+  double v[kSmoothQ + 1];
 };
 constexpr SmoothFact make_smooth_fact() {
   SmoothFact t{};
   t.v[0] = 1.0;
-  for (int k = 1; k < 2 * kSmoothQ + 2; k++) t.v[k] = t.v[k - 1] * (double)k;
+  for (int k = 1; k <= kSmoothQ; k++) t.v[k] = t.v[k - 1] * (double)k;
   return t;
 }
 __constant__ SmoothFact kSmoothFact = make_smooth_fact();
 
-// One wave per node.  The node's blobs go through in chunks of 64: lane j
-// forms cir_j d_j^m and conj(d_j)^n (m, n <= q) of blob j in LDS, then lane
-// t < (q+1)(q+2)/2 adds the products of term t = (m, n) over the chunk (FP64).
-// Lane k < kSmoothC then forms output coefficient k.
-__global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restrict__ nodes,
+// Two waves per node (one thread per moment: (q+1)(q+2)/2 = 91 of them).
+// The node's blobs go through in chunks of 32: thread j forms cir_j d_j^m and
+// conj(d_j)^n (m, n <= q) of blob j in LDS, then thread t < (q+1)(q+2)/2 adds
+// the products of term t = (m, n) over the chunk (FP64).  Thread k < kSmoothC
+// then forms output coefficient k.
+constexpr int kSmoothCoefThreads = 128;
+static_assert((kSmoothQ + 1) * (kSmoothQ + 2) / 2 <= kSmoothCoefThreads &&
+                  2 * kSmoothF4 <= kSmoothCoefThreads,
+              "one thread per moment and per output value");
+__global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const NodeRec *__restrict__ nodes,
                                                          int node0, int nnodes,
                                                          const int32_t *__restrict__ nfirst,
                                                          const double *__restrict__ sx,
@@ -282,7 +288,7 @@ __global__ __launch_bounds__(64) void smooth_coef_kernel(const NodeRec *__restri
   constexpr int NT = (Q + 1) * (Q + 2) / 2;
   __shared__ double2 mu[NT];
   __shared__ double2 cz[kSmoothC];   // c_ij lam^(i+j) in storage order
-  constexpr int B = 32;               // blobs per chunk (LDS: 4 waves' worth per 64 KB)
+  constexpr int B = 32;               // blobs per chunk
   // rows padded by one entry: the lanes of one step read rows m (and n) at
   // the same column, which would otherwise fall into the same LDS banks
   __shared__ double2 pm[Q + 1][B + 1];  // cir d^m
@@ -410,7 +416,7 @@ int launch_smooth_coef(const NodeRec *d_nodes, int node0, int node1, const int32
                        const double *sx, const double *sy, const double *sg, double sigma,
                        double region, int smin, float4 *d_out, hipStream_t s) {
   if (node1 <= node0) return 0;
-  smooth_coef_kernel<<<node1 - node0, 64, 0, s>>>(d_nodes, node0, node1, d_nfirst, sx, sy, sg,
+  smooth_coef_kernel<<<node1 - node0, kSmoothCoefThreads, 0, s>>>(d_nodes, node0, node1, d_nfirst, sx, sy, sg,
                                                   1.0 / sigma, region, smin, d_out);
   FMM_HIP_TRY(hipGetLastError());
   return 0;

@@ -134,8 +134,8 @@ struct Context {
   // smooth cells of the mixed form (one core radius): half side <= eta sigma,
   // at least smooth_min blobs; smooth_region_h = 2 eta sigma of the current
   // tree (0: none built)
-  double smooth_eta = 0.3;
-  int smooth_min = 16;
+  double smooth_eta = 0.5;
+  int smooth_min = 48;
   double smooth_region_h = 0.0;
   int t_smooth_min = 0;  // smooth_min the current tree's coefficients used
   DevBuf<float4> scoef;
@@ -1210,8 +1210,8 @@ int fmm_get_split(void) {
 
 int fmm_set_smooth(double eta, int min_blobs) {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (!(eta >= 0.0) || eta > 0.35 || min_blobs < 1) {
-    set_error("fmm_set_smooth: eta=%g (0 = off, at most 0.35), min_blobs=%d (>= 1)", eta,
+  if (!(eta >= 0.0) || eta > 0.6 || min_blobs < 1) {
+    set_error("fmm_set_smooth: eta=%g (0 = off, at most 0.6), min_blobs=%d (>= 1)", eta,
               min_blobs);
     return -1;
   }

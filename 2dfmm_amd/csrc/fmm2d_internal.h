@@ -121,7 +121,7 @@ int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int npare
 // then d_ab of the n >= 1 part in real monomials x^a y^b, in Horner order
 // (b = q-1 .. 0, a = q-1-b .. 0); see smooth_coef_kernel.  Coordinates of the evaluation are the FP32
 // leaf-relative ones scaled by sw = sqrt(log2 e) / sigma.
-constexpr int kSmoothQ = 8;
+constexpr int kSmoothQ = 12;
 constexpr int kSmoothC = (kSmoothQ + 1) + kSmoothQ * (kSmoothQ + 1) / 2;
 constexpr int kSmoothF4 = (kSmoothC + 1) / 2;  // float4 per cell
 // Coefficients of every node in [node0, node1) of region in (region/2,
@@ -257,7 +257,7 @@ int launch_l2l_setup(const int32_t *d_leaves, int nleaves, int32_t *d_nodeli,
 int launch_parent_local(const FmmArgs &args, bool mixed, int64_t *d_pcount, int nblocks,
                         hipStream_t s);
 // Work units of a smooth cell, and of a step's fixed cost (fmm_kernels.hip).
-constexpr int kSmoothUnits = 32;
+constexpr int kSmoothUnits = 60;
 constexpr int kStepUnits = 8;
 // Per-node work units of the smooth level summed into d_out[0..nnodes] (exclusive
 // prefix, total last); d_tmp: nnodes u32 of scratch.

@@ -448,9 +448,8 @@ struct StageOf<true> {
   typedef float4 T;
 };
 constexpr int kStageSlots = 128;
-// The mixed form's stage has room past kStageSlots for the values of one
-// smooth batch (5 cells x 64 targets, float2 each) and the 64 targets'
-// coordinates.
+// The mixed form's stage has room past kStageSlots for the coefficients of
+// one smooth batch (3 cells), their offsets and the 64 targets' coordinates.
 constexpr int kSmoothResSlots = 192;
 template <bool MIXED>
 constexpr int stage_slots() { return MIXED ? kStageSlots + kSmoothResSlots : kStageSlots; }
@@ -462,7 +461,7 @@ constexpr int stage_slots() { return MIXED ? kStageSlots + kSmoothResSlots : kSt
 // their coefficients).  A batch of nb cells is nb x cA evaluations, (target,
 // cell) pairs dealt to the 64 lanes in rounds; nb is chosen to fill the
 // rounds best, the larger on a tie.
-constexpr int kSmoothBatch = 5;
+constexpr int kSmoothBatch = 3;
 __device__ inline int smooth_batch_size(int cA) {
   int best = 1, bu = 0;  // bu: evaluations per 64 lane slots, x 1024
   for (int nb = 1; nb <= kSmoothBatch; nb++) {
@@ -843,16 +842,16 @@ __device__ inline void smooth_eval(float zx, float zy, const float4 *row, float 
 // steps).
 //
 // Stage layout of the mixed form (float4 slots of one wave's slice):
-//   [0, 160)    near-step sources (slots < kStageSlots) while a near item is
-//               summed; the values of one smooth batch (float2 per (target,
-//               cell) evaluation) while a batch is summed -- never both;
-//   [160, 275)  coefficients of the pending smooth cells, kSmoothF4 per cell,
+//   [0, 128)    near-step sources while a near item is summed; the values of
+//               one smooth batch (float2 per (target, cell) evaluation, at
+//               most 3 x 64) while a batch is summed -- never both;
+//   [128, 266)  coefficients of the pending smooth cells, kSmoothF4 per cell,
 //               written by direct global-to-LDS loads when a cell is found;
-//   [276, 279)  their offsets from the target leaf (float2 per cell);
-//   [288, 320)  the targets' coordinates (float2 per target).
-constexpr int kSmoothCoef = 160;
-constexpr int kSmoothOff = 276;
-constexpr int kSmoothTl = 288;
+//   [266, 268)  their offsets from the target leaf (float2 per cell);
+//   [268, 300)  the targets' coordinates (float2 per target).
+constexpr int kSmoothCoef = 128;
+constexpr int kSmoothOff = 266;
+constexpr int kSmoothTl = 268;
 static_assert(kStageSlots <= kSmoothCoef && 2 * kSmoothCoef >= kSmoothBatch * 64,
               "near stage or smooth values overlap the coefficients");
 static_assert(kSmoothCoef + kSmoothBatch * kSmoothF4 <= kSmoothOff &&

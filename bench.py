@@ -43,8 +43,8 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-# complex coefficients of a smooth cell's degree-8 expansion (9 + 36)
-SMOOTH_COEFS = 45
+# complex coefficients of a smooth cell's degree-12 expansion (13 + 78)
+SMOOTH_COEFS = 91
 sys.path.insert(0, ROOT)
 
 METRIC = "target velocities/s at N=4M, rel.err≤1e-6 vs FP64 direct: mixed+learned vs plain"
@@ -218,7 +218,7 @@ def per_kernel(res, n, p, fmm_mode):
                  "8 p^2 flops per far cell of a target leaf (its own p) or of a parent"),
                 ("l2l", 8 * p * p * w["l2l_shifts"], "valu", "8 p^2 flops per shift to a child"),
                 ("smooth", 8 * SMOOTH_COEFS * w["smooth"], "valu",
-                 "8 x 45 flops per (target, smooth cell)"),
+                 f"8 x {SMOOTH_COEFS} flops per (target, smooth cell)"),
                 ("l2p", 8 * w["l2p_terms"], "valu", "8 p_A flops per target")):
             tf = fl / t / 1e12
             out.append({"kernel": f"{name} in {res['kernel']}", "phase": "evaluate", "bound": bound,
@@ -261,9 +261,9 @@ def main():
                          "local expansion down (0: off)")
     ap.add_argument("--learned-floor", type=float, default=2.5,
                     help="learned mode: floor of the near-field radius in core radii")
-    ap.add_argument("--smooth", type=float, default=0.3,
+    ap.add_argument("--smooth", type=float, default=0.5,
                     help="mixed modes: smooth cells of half side <= SMOOTH core radii (0: off)")
-    ap.add_argument("--smooth-min", type=int, default=16,
+    ap.add_argument("--smooth-min", type=int, default=48,
                     help="mixed modes: fewest blobs of a smooth cell")
     ap.add_argument("--traffic",
                     default=os.path.join(ROOT, "profiles", "fmm_eval_gaussian_mixture_4M_traffic.txt"),

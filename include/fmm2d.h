@@ -146,10 +146,10 @@ int fmm_set_far_burst(int cells);
 int fmm_set_l2l(int min_children);
 /* Smooth cells (no reference counterpart; mixed mode with one core radius
  * sigma).  A near node whose half side is at most eta sigma and that holds at
- * least min_blobs blobs is summed for every target through a degree-8 Taylor
+ * least min_blobs blobs is summed for every target through a degree-12 Taylor
  * expansion of the regularised kernel about its centre (coefficients computed
- * at make_tree) instead of pair by pair.  eta in [0, 0.35]; 0 turns it off.
- * Takes effect at the next make_tree.  Default eta 0.3, min_blobs 16. */
+ * at make_tree) instead of pair by pair.  eta in [0, 0.6]; 0 turns it off.
+ * Takes effect at the next make_tree.  Default eta 0.5, min_blobs 48. */
 int fmm_set_smooth(double eta, int min_blobs);
 int fmm_get_smooth(double *eta, int *min_blobs);
 /* Compute units of the current device (for sizing persistent grids). */

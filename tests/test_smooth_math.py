@@ -14,7 +14,7 @@ import math
 import numpy as np
 import pytest
 
-Q = 8
+Q = 12  # the degree the library uses (kSmoothQ)
 LAM = math.sqrt(math.log(2.0))
 
 
@@ -85,7 +85,7 @@ def _cell(seed, s, n=64):
     return src, g
 
 
-@pytest.mark.parametrize("s", [0.05, 0.15, 0.22])
+@pytest.mark.parametrize("s", [0.1, 0.3, 0.49, 0.6])
 def test_expansion_matches_exact_sum(s):
     src, g = _cell(1, s)
     nu, c = coefficients(moments(-src, g))
@@ -96,8 +96,9 @@ def test_expansion_matches_exact_sum(s):
             Z = r * complex(math.cos(ang), math.sin(ang))
             exact = np.sum(g * f_exact(Z - src))
             worst = max(worst, abs(evaluate(Z, nu, c) - exact) / np.sum(np.abs(g)))
-    # truncation after total degree 8: |d|^9 / 9!-like terms
-    assert worst <= {0.05: 1e-13, 0.15: 1e-9, 0.22: 2e-8}[s], worst
+    # truncation after total degree 12: |d|^13 / 13!-like terms; half sides up
+    # to 0.6 core radii (the largest eta the library accepts)
+    assert worst <= {0.1: 1e-14, 0.3: 1e-10, 0.49: 5e-8, 0.6: 6e-7}[s], worst
 
 
 def test_phi_branches_agree_with_quadrature():
