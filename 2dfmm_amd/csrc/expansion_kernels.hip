@@ -266,6 +266,26 @@ constexpr SmoothFact make_smooth_fact() {
   return t;
 }
 __constant__ SmoothFact kSmoothFact = make_smooth_fact();
+// C(n, k) and the falling products (i + l)! / i! for n, i + l <= kSmoothQ:
+// integers below 2^29, exact in FP64, so the coefficient formulas need no
+// division
+struct SmoothInt {
+  double binom[kSmoothQ + 1][kSmoothQ + 1];
+  double rise[kSmoothQ + 1][kSmoothQ + 1];  // rise[i][l] = (i + l)! / i!
+};
+constexpr SmoothInt make_smooth_int() {
+  SmoothInt t{};
+  for (int n = 0; n <= kSmoothQ; n++) {
+    t.binom[n][0] = 1.0;
+    for (int k = 1; k <= n; k++) t.binom[n][k] = t.binom[n][k - 1] * (double)(n - k + 1) / (double)k;
+  }
+  for (int i = 0; i <= kSmoothQ; i++) {
+    t.rise[i][0] = 1.0;
+    for (int l = 1; i + l <= kSmoothQ; l++) t.rise[i][l] = t.rise[i][l - 1] * (double)(i + l);
+  }
+  return t;
+}
+__constant__ SmoothInt kSmoothInt = make_smooth_int();
 
 // Two waves per node (one thread per moment: (q+1)(q+2)/2 = 91 of them).
 // The node's blobs go through in chunks of 32: thread j forms cir_j d_j^m and
@@ -371,7 +391,7 @@ __global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const N
       const int i = Q - 1 - j - k;
       for (int l = 0; i + j + 2 * l + 1 <= Q; l++) {
         const double2 v = mu[smooth_tri(j + l) + i + l + 1];
-        double c = fact[j + l] / (fact[l] * fact[j]) * fact[i + l] / fact[i];
+        double c = kSmoothInt.binom[j + l][l] * kSmoothInt.rise[i][l];  // C(j+l, l) (i+l)!/i!
         if ((i + j + l) & 1) c = -c;
         vr += v.x * c;
         vi += v.y * c;
@@ -393,7 +413,7 @@ __global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const N
       const int j = n - i;
       double w = 0.0;
       for (int l = max(0, b - i); l <= min(j, b); l++) {
-        const double t = fact[i] / (fact[b - l] * fact[i - b + l]) * (fact[j] / (fact[l] * fact[j - l]));
+        const double t = kSmoothInt.binom[i][b - l] * kSmoothInt.binom[j][l];  // C(i, b-l) C(j, l)
         w += (l & 1) ? -t : t;
       }
       const double2 v = cz[smooth_slot(i, j)];

@@ -77,6 +77,17 @@ struct Context {
   int mode = FMM2D_MODE_TREECODE;
   bool prefetch = true;
   hipStream_t stream = nullptr;
+  // the evaluation's second stream (default priority) and its fork/join
+  // events: the leaves after the front ones run there (launch_fmm_eval)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int nfront = 0;  // front leaves of the launch order of the current tree
+  // the build's last kernels that only the evaluation reads (leaf-relative
+  // sources, smooth coefficients) run on the side stream after ev_sfork;
+  // ev_side marks their end, and side_pending says the main stream has not
+  // waited for it yet (join_side)
+  hipEvent_t ev_sfork = nullptr, ev_side = nullptr;
+  bool side_pending = false;
   bool init = false;
   // tree
   bool valid = false;
@@ -94,6 +105,9 @@ struct Context {
   DevBuf<double4> loc64;
   DevBuf<int32_t> leaves, lat;
   DevBuf<uint32_t> lflag, lpos;
+  // launch order of the target leaves (sort keys, then the order itself)
+  DevBuf<uint64_t> lkey, lkey_alt;
+  DevBuf<uint32_t> lord, lord_alt;
   DevBuf<double2> direct_ws;  // partial sums of the direct judge
   double smax_h = 0.0;        // largest core radius of the tree's blobs
   double wmax_h = 0.0;        // log2(e) / (smallest core radius)^2, capped
@@ -128,6 +142,7 @@ struct Context {
   DevBuf<SplitCk> ck;
   DevBuf<int32_t> nfirst;
   DevBuf<int2> leaf_ck;
+  DevBuf<int32_t> split_list;  // the leaves with chunk records of the last evaluation
   DevBuf<double2> part;
   DevBuf<uint32_t> split_ctr;
   int ncu = 0;
@@ -192,6 +207,11 @@ static int ctx_init() {
     return -1;
   }
   FMM_HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  FMM_HIP_TRY(hipStreamCreateWithFlags(&g.side, hipStreamNonBlocking));
+  FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
+  FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
+  FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_sfork, hipEventDisableTiming));
+  FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_side, hipEventDisableTiming));
   FMM_HIP_TRY(hipHostMalloc((void **)&g.h_total, 64, hipHostMallocDefault));
   // the level table, then the root square (read back together, pinned)
   FMM_HIP_TRY(hipHostMalloc((void **)&g.h_lt, sizeof(int32_t) * kLtWords + 64 + sizeof(RootBox),
@@ -202,6 +222,15 @@ static int ctx_init() {
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_uv, hipEventDisableTiming));
   for (auto &e : g.tev) FMM_HIP_TRY(hipEventCreate(&e));
   g.init = true;
+  return 0;
+}
+
+// The main stream waits for the side stream's part of the last build.
+static int join_side() {
+  if (g.side_pending) {
+    FMM_HIP_TRY(hipStreamWaitEvent(g.stream, g.ev_side, 0));
+    g.side_pending = false;
+  }
   return 0;
 }
 
@@ -266,6 +295,7 @@ constexpr int kSortLevels = 16;
 static int build(int n, const double *dxy, const double *dcir, const double *dcor, int nsys = 1,
                  const int32_t *h_off = nullptr) {
   hipStream_t s = g.stream;
+  if (join_side()) return -1;  // the last build's side work reads buffers rebuilt here
   g.valid = false;
   g.uv_pending = g.uv_ready = false;
   g.hash.clear();
@@ -434,23 +464,35 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     if (g.leaves.ensure(n, s) || g.lat.ensure(n, s) || g.lflag.ensure(n, s) || g.lpos.ensure(n, s) ||
         g.loc.ensure(n, s) || g.loc64.ensure(n, s) || g.smax.ensure(2, s))
       return -1;
-    if (g.ntl_dev.ensure(2, s) || g.counter.ensure(1, s)) return -1;
+    // ntl_dev: [0] target leaves, [1] sibling groups (L2L), [2] front leaves
+    if (g.ntl_dev.ensure(4, s) || g.counter.ensure(1, s)) return -1;
     if (launch_leaf_list(g.nodes.p, g.nnodes, n, g.lat.p, g.lflag.p, g.lpos.p, g.leaves.p,
                          g.ntl_dev.p, s))
       return -1;
     if (launch_minmax_reduce(dcor, n, g.smax.p, s)) return -1;
+    // launch order keys (boxes large against the near radius first) and the
+    // number of such front leaves, read back with the leaf count
+    const int lbound = std::min(n, g.nnodes);
+    if (g.lkey.ensure(lbound, s) || g.lkey_alt.ensure(lbound, s) || g.lord.ensure(lbound, s) ||
+        g.lord_alt.ensure(lbound, s) ||
+        launch_leaf_launch_keys(g.nodes.p, g.leaves.p, lbound, g.ntl_dev.p, g.smax.p, g.fmm_cut,
+                                g.lkey.p, g.lord.p, g.ntl_dev.p + 2, s))
+      return -1;
     double smm[2] = {0.0, 0.0};
-    FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.ntl_dev.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.ntl_dev.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               s));
     FMM_HIP_TRY(hipMemcpyAsync(smm, g.smax.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     FMM_HIP_TRY(hipStreamSynchronize(s));
     g.ntleaves = (int)g.h_total[0];
+    g.nfront = (int)g.h_total[2];
     g.smax_h = smm[0];
     g.rcut = g.fmm_cut * smm[0];
     // the FP32 pair kernel's w = log2(e) / sigma^2 is capped at 7.2e19
     g.wmax_h = smm[1] > 0.0 ? std::min(1.4426950408889634 / (smm[1] * smm[1]), 7.2e19) : 7.2e19;
     g.sw_h = smm[0] == smm[1] ? std::sqrt(g.wmax_h) : 0.0;
-    if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
-                          g.loc.p, g.loc64.p, g.sw_h, s))
+    // the launch order: one stable 8-bit radix pass over the keys
+    if (radix_sort_pairs(&g.lkey.p, &g.lord.p, &g.lkey_alt.p, &g.lord_alt.p, g.ntleaves, 0, 8,
+                         g.hist.p, g.hist_scan.p, s))
       return -1;
     if (g.nfirst.ensure(g.nnodes, s) || launch_node_first_blob(g.nodes.p, g.nnodes, g.nfirst.p, s))
       return -1;
@@ -459,27 +501,38 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
         launch_l2l_setup(g.leaves.p, g.ntleaves, g.nodeli.p, g.ntl_dev.p + 1, g.nnodes / 4, s))
       return -1;
     // smooth cells: mixed form with one core radius only
-    if (g.t_mode == FMM2D_MODE_MIXED && g.sw_h > 0.0 && g.smooth_eta > 0.0) {
-      const double region = 2.0 * g.smooth_eta * smm[0];
-      // every internal node (>= num_p blobs) of the smooth level is a smooth cell
-      const int smin = std::min(g.smooth_min, g.t_num_p);
-      // a single tree's smooth level is one contiguous range of nodes (every
-      // level halves the root's side exactly); a forest's roots differ
-      int n0 = 0, n1 = g.nnodes;
-      if (nsys == 1) {
-        double rl = g.h_root->region;
-        int l = 0;
-        while (l < g.depth && rl > region) {
-          rl *= 0.5;
-          l++;
-        }
-        n0 = l == 0 ? 0 : g.h_lt[kLtCb + l];
-        n1 = l == 0 ? nroot_slots : g.h_lt[kLtCb + l + 1];
+    const bool smooth = g.t_mode == FMM2D_MODE_MIXED && g.sw_h > 0.0 && g.smooth_eta > 0.0;
+    const double region = 2.0 * g.smooth_eta * smm[0];
+    // every internal node (>= num_p blobs) of the smooth level is a smooth cell
+    const int smin = std::min(g.smooth_min, g.t_num_p);
+    // a single tree's smooth level is one contiguous range of nodes (every
+    // level halves the root's side exactly); a forest's roots differ
+    int n0 = 0, n1 = g.nnodes;
+    if (smooth && nsys == 1) {
+      double rl = g.h_root->region;
+      int l = 0;
+      while (l < g.depth && rl > region) {
+        rl *= 0.5;
+        l++;
       }
-      if (g.scoef.ensure((size_t)g.nnodes * kSmoothF4, s) ||
-          launch_smooth_coef(g.nodes.p, n0, n1, g.nfirst.p, g.sx.p, g.sy.p, g.sg.p, smm[0],
-                             region, smin, g.scoef.p, s))
-        return -1;
+      n0 = l == 0 ? 0 : g.h_lt[kLtCb + l];
+      n1 = l == 0 ? nroot_slots : g.h_lt[kLtCb + l + 1];
+    }
+    if (smooth && g.scoef.ensure((size_t)g.nnodes * kSmoothF4, s)) return -1;
+    // Leaf-relative sources and smooth coefficients feed the evaluation only:
+    // they run on the side stream while the main one forms the smooth ranges,
+    // the learned parameters and (in the evaluation) the parent expansions.
+    FMM_HIP_TRY(hipEventRecord(g.ev_sfork, s));
+    FMM_HIP_TRY(hipStreamWaitEvent(g.side, g.ev_sfork, 0));
+    if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
+                          g.loc.p, g.loc64.p, g.sw_h, g.side))
+      return -1;
+    if (smooth && launch_smooth_coef(g.nodes.p, n0, n1, g.nfirst.p, g.sx.p, g.sy.p, g.sg.p,
+                                     smm[0], region, smin, g.scoef.p, g.side))
+      return -1;
+    FMM_HIP_TRY(hipEventRecord(g.ev_side, g.side));
+    g.side_pending = true;
+    if (smooth) {
       if (g.sflag.ensure(2 * (size_t)g.nnodes, s) || g.srange.ensure(g.nnodes, s) ||
           launch_smooth_ranges(g.nodes.p, g.gparent.p, g.nnodes, nroot_slots, region, smin,
                                g.sflag.p, g.srange.p, s))
@@ -518,12 +571,14 @@ static TreeView view() {
   return tv;
 }
 
-static int need_tree() {
+// join = false only for the evaluation, which waits for the side stream
+// itself, after its parent expansions.
+static int need_tree(bool join = true) {
   if (!g.init || !g.valid) {
     set_error("no valid tree: call make_tree first");
     return -1;
   }
-  return 0;
+  return join ? join_side() : 0;
 }
 
 static FmmArgs fmm_args(double *d_uv) {
@@ -539,6 +594,9 @@ static FmmArgs fmm_args(double *d_uv) {
   a.loc = g.loc.p;
   a.leaves = g.leaves.p;
   a.nleaves = g.ntleaves;
+  a.lorder = g.t_mode != FMM2D_MODE_TREECODE ? g.lord.p : nullptr;
+  a.pos0 = 0;
+  a.npos = g.ntleaves;
   a.perm = g.perm.p;
   a.uv = reinterpret_cast<double2 *>(d_uv);
   a.p = g.t_nterms;
@@ -552,10 +610,12 @@ static FmmArgs fmm_args(double *d_uv) {
   a.sw = g.sw_h;
   a.nfirst = g.nfirst.p;
   a.split_budget = g.split_budget;
-  a.ck_cap = (int)g.ck.cap;
+  a.ck_cap = (int)(g.ck.cap / 2);  // two record regions (launch_fmm_eval)
   a.ck = g.ck.p;
   a.ck_count = g.split_ctr.p;
   a.leaf_ck = g.leaf_ck.p;
+  a.split_list = g.split_list.p;
+  a.split_count = g.split_ctr.p ? g.split_ctr.p + 4 : nullptr;
   a.part = g.part.p;
   a.smooth_region = g.smooth_region_h;
   a.smooth_min = g.t_smooth_min;
@@ -581,12 +641,13 @@ static int ensure_split() {
   hipStream_t s = g.stream;
   // chunks taken by the last evaluation (pinned copy, read without waiting:
   // a hint only)
-  const size_t taken = g.h_total[8];
+  const size_t taken = std::max(g.h_total[8], g.h_total[10]);
   g.ck_need = std::max(g.ck_need, taken + taken / 4);
   const size_t cap = std::min(std::max(2 * (size_t)g.ntleaves + 4096, g.ck_need),
                               16 * (size_t)g.ntleaves + 4096);
-  if (g.ck.ensure(cap, s) || g.leaf_ck.ensure((size_t)g.ntleaves + 1, s) ||
-      g.part.ensure(64 * g.ck.cap, s) || g.split_ctr.ensure(2, s))
+  if (g.ck.ensure(2 * cap, s) || g.leaf_ck.ensure((size_t)g.ntleaves + 1, s) ||
+      g.part.ensure(64 * g.ck.cap, s) || g.split_ctr.ensure(5, s) ||
+      g.split_list.ensure((size_t)g.ntleaves + 1, s))
     return -1;
   if (g.ncu <= 0) {
     int dev = 0;
@@ -597,6 +658,12 @@ static int ensure_split() {
 }
 
 static int split_blocks() { return 4 * (g.ncu > 0 ? g.ncu : 256); }
+
+// Region 1 of the records (leaves after the front ones) rarely holds any in
+// the mixed form, and a full persistent grid that finds none costs ~50 us;
+// the full grid is used when the last evaluation wrote more than 64 there
+// (h_total[10], read without waiting: a hint only).
+static int side_blocks() { return g.h_total[10] > 64 ? split_blocks() : split_blocks() / 4; }
 
 // Learned per-leaf (p, cut) for the current tree: features -> MLP -> params.
 static int apply_learned() {
@@ -656,7 +723,7 @@ static int eval_fmm_graph(const FmmArgs &a, bool mixed) {
 }
 
 static int eval_sources_dev(double *d_uv) {
-  if (need_tree()) return -1;
+  if (need_tree(false)) return -1;
   if (g.n == 0) return 0;
   if (g.n_real >= 0) {
     // mirror images ride along as sources: evaluate all, return the real blobs
@@ -675,6 +742,11 @@ static int eval_sources_dev(double *d_uv) {
   int rc;
   const bool mixed = g.t_mode == FMM2D_MODE_MIXED;
   if (g.t_mode != FMM2D_MODE_TREECODE && ensure_split()) return -1;
+  if ((g.t_mode == FMM2D_MODE_TREECODE || g.persist_blocks > 0) && join_side()) return -1;
+  // the default path waits for the build's side work after the parent
+  // expansions (which do not read it)
+  hipEvent_t ready = g.side_pending ? g.ev_side : nullptr;
+  g.side_pending = false;
   if (g.t_mode == FMM2D_MODE_TREECODE)
     rc = g.parity_leaves ? launch_parity_leaves(view(), g.n, g.perm.p, d_uv, g.stream)
                          : launch_parity_eval(view(), g.n, nullptr, g.perm.p, d_uv, g.stream);
@@ -685,12 +757,13 @@ static int eval_sources_dev(double *d_uv) {
                                     g.persist_blocks, g.split_ctr.p, split_blocks(), g.stream);
   else
     rc = launch_fmm_eval(fmm_args(d_uv), mixed, g.ntl_dev.p, g.split_ctr.p, split_blocks(),
-                         g.stream, g.timing ? g.tev[9] : nullptr);
+                         g.stream, g.timing ? g.tev[9] : nullptr, g.side, g.ev_fork, g.ev_join,
+                         g.nfront, ready, side_blocks());
   if (!rc && g.timing && g.t_mode != FMM2D_MODE_TREECODE && g.persist_blocks == 0)
     g.tev_rec[9] = true;
   mark(6);
   if (!rc && g.t_mode != FMM2D_MODE_TREECODE && g.split_budget > 0)
-    FMM_HIP_TRY(hipMemcpyAsync(g.h_total + 8, g.split_ctr.p, sizeof(uint32_t),
+    FMM_HIP_TRY(hipMemcpyAsync(g.h_total + 8, g.split_ctr.p, 3 * sizeof(uint32_t),
                                hipMemcpyDeviceToHost, g.stream));
   return rc;
 }
@@ -1842,6 +1915,7 @@ int fmm_count_work_detail(long long *out, int n) {
 int fmm_synchronize(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g.init) return 0;
+  if (join_side()) return -1;
   FMM_HIP_TRY(hipStreamSynchronize(g.stream));
   return 0;
 }
@@ -1849,7 +1923,9 @@ int fmm_synchronize(void) {
 void fmm_release(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g.init) return;
+  (void)hipStreamSynchronize(g.side);
   (void)hipStreamSynchronize(g.stream);
+  g.side_pending = false;
   g.xy.release(); g.cir.release(); g.cor.release();
   g.key.release(); g.key_alt.release(); g.idx.release(); g.idx_alt.release();
   g.hist.release(); g.hist_scan.release(); g.perm.release(); g.lt.release(); g.lstat.release(); g.cflag.release();
@@ -1857,9 +1933,10 @@ void fmm_release(void) {
   g.bbox_part.release(); g.uv_dev.release(); g.tmp_in.release(); g.tmp_out.release();
   g.nodes.release(); g.gparent.release(); g.split.release(); g.coef.release(); g.root.release();
   g.loc.release(); g.loc64.release(); g.smax.release(); g.leaves.release(); g.lat.release();
-  g.lflag.release(); g.lpos.release(); g.direct_ws.release();
+  g.lflag.release(); g.lpos.release(); g.lkey.release(); g.lkey_alt.release(); g.lord.release();
+  g.lord_alt.release(); g.direct_ws.release();
   g.doff.release(); g.ntl_dev.release(); g.counter.release();
-  g.ck.release(); g.leaf_ck.release(); g.part.release(); g.split_ctr.release();
+  g.ck.release(); g.leaf_ck.release(); g.split_list.release(); g.part.release(); g.split_ctr.release();
   g.nfirst.release();
   g.plocal.release();
   g.nodeli.release();

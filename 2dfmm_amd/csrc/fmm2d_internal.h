@@ -194,6 +194,10 @@ struct FmmArgs {
   const double4 *loc64; // (x, y, cir, 1/sigma^2) in FP64
   const int32_t *leaves;
   int nleaves;
+  // launch order of the target leaves (null: leaf-list order); a fused-pass
+  // launch takes positions [pos0, pos0 + npos) of it
+  const uint32_t *lorder;
+  int pos0, npos;
   const uint32_t *perm;
   double2 *uv;  // caller order
   int p;
@@ -217,10 +221,14 @@ struct FmmArgs {
   // (first record, records), 64 partial sums per record
   int split_budget;
   int ck_cap;
+  int ck_base;  // index of record ck[0] among all records (part, leaf_ck)
   SplitCk *ck;
   uint32_t *ck_count;
   int2 *leaf_ck;
   double2 *part;
+  // the split leaves (those with records) and their number
+  int32_t *split_list;
+  uint32_t *split_count;
   // smooth cells (smooth_region 0: off): a near node of region in
   // (smooth_region/2, smooth_region] with at least smooth_min blobs is summed
   // through its coefficients scoef[node * kSmoothF4 ...]
@@ -273,16 +281,35 @@ int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleav
 // Non-empty leaves in blob (Morton) order; their number lands in *d_count.
 int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, uint32_t *d_flag,
                      uint32_t *d_pos, int32_t *d_leaves, uint32_t *d_count, hipStream_t s);
+// Sort keys of the launch order of the target leaves (front leaves: a half
+// diagonal above the near radius cut * d_smax[0], the largest first; all
+// others after them in leaf-list order), d_val[li] = li, for li < *d_count
+// (at most n_bound); sorted by their low 8 bits, stably, they give the order
+// in which the evaluation takes the leaves.  *d_front = the front leaves.
+int launch_leaf_launch_keys(const NodeRec *d_nodes, const int32_t *d_leaves, int n_bound,
+                            const uint32_t *d_count, const double *d_smax, double cut,
+                            uint64_t *d_key, uint32_t *d_val, uint32_t *d_front, hipStream_t s);
 // d_out[0] = max, d_out[1] = min of n values >= 0.
 int launch_minmax_reduce(const double *d_v, int n, double *d_out, hipStream_t s);
 // First blob of every node's subtree: the first blob of its leftmost leaf.
 int launch_node_first_blob(const NodeRec *d_nodes, int nnodes, int32_t *d_out, hipStream_t s);
-// One wave per target leaf; with splitting on, d_split_ctr holds two counters
-// and the continuation passes run on split_blocks x 256 threads.
-// mid (optional): an event recorded between the fused pass and the split passes.
+// One wave per target leaf; with splitting on, d_split_ctr holds five
+// counters (records taken and records summed, of the two record regions, and
+// the split leaves listed in args.split_list) and
+// the continuation passes run on split_blocks x 256 threads.  With a side
+// stream and 0 < nfront < nleaves, the first nfront leaves of the launch
+// order and their continuation pass run on s (record region 0) while the
+// other leaves run on `side` (region 1, records from args.ck_cap on); s then
+// waits for side (events fork and join), sums region 1's records and adds the
+// chunk sums.  mid (optional): an event recorded before the last continuation
+// pass.  ready (optional): an event s waits for after the parent expansions,
+// before the fused pass (the build's work on the side stream).  side_blocks:
+// the grid (x 256 threads) of region 1's continuation pass.
 int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
                     uint32_t *d_split_ctr, int split_blocks, hipStream_t s,
-                    hipEvent_t mid = nullptr);
+                    hipEvent_t mid = nullptr, hipStream_t side = nullptr,
+                    hipEvent_t fork = nullptr, hipEvent_t join = nullptr, int nfront = 0,
+                    hipEvent_t ready = nullptr, int side_blocks = 0);
 
 // ---- time stepping around the evaluation (step_kernels.hip) ---------------
 int launch_drift_ab3(int n, double *d_xy, const double *d_hist, double dt, int mirror, double dm,

@@ -240,6 +240,44 @@ int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, u
   return 0;
 }
 
+// A leaf whose box is large against the near radius has a near region of
+// about 1/theta times its box (a sparse box of a few blobs next to a dense
+// cluster sums pairs with much of the cluster), and its walk takes up to ten
+// times a typical leaf's.  Such leaves are started first, the largest first:
+// key = 9 + floor(log2(rcut / R)) for R > rcut (half diagonal R), 9 for the
+// others, which keep their Morton order (neighbours in space run together).
+__global__ __launch_bounds__(256) void leaf_launch_keys_kernel(const NodeRec *__restrict__ nodes,
+                                                               const int32_t *__restrict__ leaves,
+                                                               const uint32_t *__restrict__ count,
+                                                               const double *__restrict__ smax,
+                                                               double cut,
+                                                               uint64_t *__restrict__ key,
+                                                               uint32_t *__restrict__ val,
+                                                               uint32_t *__restrict__ front) {
+  const int li = blockIdx.x * 256 + threadIdx.x;
+  if (li >= (int)*count) return;
+  const double rcut = cut * smax[0];
+  const double R = nodes[leaves[li]].region * 0.70710678118654752;
+  int k = 9;
+  if (R > rcut) k = min(8, max(0, 9 + ilogb(rcut / R)));
+  key[li] = (uint64_t)k;
+  val[li] = (uint32_t)li;
+  const uint64_t fm = __builtin_amdgcn_ballot_w64(k < 9);
+  if (fm && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(fm))
+    atomicAdd(front, (uint32_t)__builtin_popcountll(fm));
+}
+
+int launch_leaf_launch_keys(const NodeRec *d_nodes, const int32_t *d_leaves, int n_bound,
+                            const uint32_t *d_count, const double *d_smax, double cut,
+                            uint64_t *d_key, uint32_t *d_val, uint32_t *d_front, hipStream_t s) {
+  FMM_HIP_TRY(hipMemsetAsync(d_front, 0, sizeof(uint32_t), s));
+  if (n_bound <= 0) return 0;
+  leaf_launch_keys_kernel<<<(n_bound + 255) / 256, 256, 0, s>>>(d_nodes, d_leaves, d_count, d_smax,
+                                                                cut, d_key, d_val, d_front);
+  FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 // out[0] = max, out[1] = min of n values >= 0 (their bit patterns order like
 // unsigned integers, so integer atomics combine the blocks).
 __global__ __launch_bounds__(256) void minmax_reduce_kernel(const double *__restrict__ v, int n,
@@ -1624,7 +1662,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
         }
       }
       if (fits) {
-        rec = make_int2((int)base, nck);
+        rec = make_int2(a.ck_base + (int)base, nck);
       } else {
         inline_chunks = true;
         for (int k = 0; k < nck; k++) {
@@ -1638,7 +1676,12 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
         }
       }
     }
-    if (lane == 0) a.leaf_ck[li] = rec;
+    if (lane == 0) {
+      a.leaf_ck[li] = rec;
+      // the leaves with records, for the chunk sums (their order there does
+      // not matter: each leaf's sums are added in its own chunk order)
+      if (rec.y > 0) a.split_list[atomicAdd(a.split_count, 1u)] = li;
+    }
   }
   if (L.tact && L.rep == 0) {
     double u = 0.5 * phi / M_PI + 0.5 * S.x / M_PI;
@@ -1663,7 +1706,9 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
 // 8); XCD x is given blocks x, x + 8, x + 16, ... of kXcdBlock consecutive
 // leaves (neighbours in space), so the tree nodes and sources of one block's
 // walks meet in one L2 while the blocks still spread the clustered work
-// evenly over the XCDs.  Grid: a multiple of 8 kXcdBlock workgroups.
+// evenly over the XCDs.  Grid: a multiple of 8 kXcdBlock workgroups.  The
+// positions are those of the launch order (lorder: boxes large against the
+// near radius first, so the longest walks do not start last).
 constexpr int kXcds = 8;
 constexpr int kXcdBlock = 64;
 __device__ inline int xcd_leaf(int b, int nleaves) {
@@ -1682,8 +1727,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fm
   __shared__ int s_far[1][kFarList];
   __shared__ int32_t s_cku[1][kMaxCk];
   __shared__ ParentBox s_pbox[1];
-  const int li = xcd_leaf(blockIdx.x, a.nleaves);
-  if (li >= a.nleaves) return;
+  const int q = xcd_leaf(blockIdx.x, a.npos);
+  if (q >= a.npos) return;
+  const int pos = a.pos0 + q;
+  const int li = a.lorder ? (int)a.lorder[pos] : pos;
   fmm_process_leaf<MIXED, UNI>(a, li, 0, threadIdx.x, s_loc, s_red, s_stage, s_ck, s_cku, s_stack,
                                s_far, s_pbox);
 }
@@ -1866,6 +1913,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
     if (lane == 0) li = atomicAdd(counter, 1u);
     li = __shfl(li, 0, 64);
     if (li >= total) break;
+    if (a.lorder) li = a.lorder[li];
     fmm_process_leaf<MIXED, UNI>(a, (int)li, w, lane, s_loc, s_red, s_stage, s_ck, s_cku,
                                  s_stack, s_far, s_pbox);
   }
@@ -1894,24 +1942,38 @@ __global__ __launch_bounds__(256) void fmm_chunk_kernel(FmmArgs a, uint32_t *__r
     run_chunk<MIXED, UNI>(a, L, (int64_t)rfl64((uint64_t)r.cum), (int64_t)rfl64((uint64_t)r.end),
                           lane, s_stage[w], s_stack[w], ax, ay);
     const double2 P = reduce_replicas(L, lane, s_red[w], ax, ay);
-    if (L.tact && L.rep == 0) a.part[(size_t)j * 64 + L.ti] = P;
+    if (L.tact && L.rep == 0) a.part[((size_t)a.ck_base + j) * 64 + L.ti] = P;
   }
 }
 
-// Adds the chunk sums of every split leaf to its targets, in chunk order.
-__global__ __launch_bounds__(256) void fmm_split_sum_kernel(FmmArgs a,
-                                                            const uint32_t *__restrict__ nleaves) {
+// Adds the chunk sums of every split leaf (split_list) to its targets, in
+// chunk order.
+__global__ __launch_bounds__(256) void fmm_split_sum_kernel(FmmArgs a) {
   const int lane = threadIdx.x & 63;
-  const uint32_t total = *nleaves;
+  const uint32_t total = *a.split_count;
   const uint32_t nw = gridDim.x * 4;
-  for (uint32_t li = blockIdx.x * 4 + (threadIdx.x >> 6); li < total; li += nw) {
+  for (uint32_t j = blockIdx.x * 4 + (threadIdx.x >> 6); j < total; j += nw) {
+    const int li = a.split_list[j];
     const int2 k = a.leaf_ck[li];
     if (k.y == 0) continue;
     const NodeRec NA = a.nodes[a.leaves[li]];
     if (lane < node_count(NA.cnt)) {
       double sx = 0.0, sy = 0.0;
-      for (int i = 0; i < k.y; i++) {
-        const double2 P = a.part[(size_t)(k.x + i) * 64 + lane];
+      // eight loads in flight, then added in chunk order
+      const double2 *src = a.part + (size_t)k.x * 64 + lane;
+      int i = 0;
+      for (; i + 8 <= k.y; i += 8) {
+        double2 P[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) P[j] = src[(size_t)(i + j) * 64];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          sx += P[j].x;
+          sy += P[j].y;
+        }
+      }
+      for (; i < k.y; i++) {
+        const double2 P = src[(size_t)i * 64];
         sx += P.x;
         sy += P.y;
       }
@@ -1928,8 +1990,9 @@ __global__ void zero_counters_kernel(uint32_t *c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
-static int launch_split_passes(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
-                               uint32_t *d_chunk_counter, int nblocks, hipStream_t s) {
+// The continuation pass over the records of args' region.
+static int launch_chunk_pass(const FmmArgs &args, bool mixed, uint32_t *d_chunk_counter,
+                             int nblocks, hipStream_t s) {
   if (mixed && args.sw > 0.f)
     fmm_chunk_kernel<true, true><<<nblocks, 256, 0, s>>>(args, d_chunk_counter);
   else if (mixed)
@@ -1937,10 +2000,14 @@ static int launch_split_passes(const FmmArgs &args, bool mixed, const uint32_t *
   else
     fmm_chunk_kernel<false, false><<<nblocks, 256, 0, s>>>(args, d_chunk_counter);
   FMM_HIP_TRY(hipGetLastError());
-  // four target leaves per workgroup (grid-stride past the host count, which a
-  // replayed graph may hold from an older tree)
-  const int sb = std::max(nblocks, (args.nleaves + 3) / 4);
-  fmm_split_sum_kernel<<<sb, 256, 0, s>>>(args, d_nleaves);
+  return 0;
+}
+
+static int launch_split_passes(const FmmArgs &args, bool mixed, uint32_t *d_chunk_counter,
+                               int nblocks, hipStream_t s) {
+  if (launch_chunk_pass(args, mixed, d_chunk_counter, nblocks, s)) return -1;
+  // four split leaves per workgroup, grid-stride over the device-side count
+  fmm_split_sum_kernel<<<std::max(1, nblocks / 4), 256, 0, s>>>(args);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1955,7 +2022,7 @@ int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *
   FMM_HIP_TRY(hipGetLastError());
   const bool split = args.split_budget > 0;
   if (split) {
-    zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 2);
+    zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 5);
     FMM_HIP_TRY(hipGetLastError());
   }
   if (mixed && args.sw > 0.f)
@@ -1965,7 +2032,7 @@ int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *
   else
     fmm_leaf_persistent_kernel<false, false><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
   FMM_HIP_TRY(hipGetLastError());
-  if (split) return launch_split_passes(args, mixed, d_nleaves, d_split_ctr + 1, split_blocks, s);
+  if (split) return launch_split_passes(args, mixed, d_split_ctr + 1, split_blocks, s);
   return 0;
 }
 
@@ -2182,16 +2249,10 @@ int launch_fmm_count(const FmmArgs &args, int64_t *d_out, hipStream_t s) {
   return 0;
 }
 
-int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
-                    uint32_t *d_split_ctr, int split_blocks, hipStream_t s, hipEvent_t mid) {
-  if (args.nleaves <= 0) return 0;
-  if (launch_parent_local(args, mixed, nullptr, split_blocks, s)) return -1;
-  const bool split = args.split_budget > 0;
-  if (split) {
-    zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 2);
-    FMM_HIP_TRY(hipGetLastError());
-  }
-  const int grid = kXcds * kXcdBlock * ((args.nleaves + kXcds * kXcdBlock - 1) / (kXcds * kXcdBlock));
+// The fused pass over launch positions [args.pos0, args.pos0 + args.npos).
+static int launch_leaf_grid(const FmmArgs &args, bool mixed, hipStream_t s) {
+  if (args.npos <= 0) return 0;
+  const int grid = kXcds * kXcdBlock * ((args.npos + kXcds * kXcdBlock - 1) / (kXcds * kXcdBlock));
   if (mixed && args.sw > 0.f)
     fmm_leaf_kernel<true, true><<<grid, 64, 0, s>>>(args);
   else if (mixed)
@@ -2199,8 +2260,52 @@ int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
   else
     fmm_leaf_kernel<false, false><<<grid, 64, 0, s>>>(args);
   FMM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
+                    uint32_t *d_split_ctr, int split_blocks, hipStream_t s, hipEvent_t mid,
+                    hipStream_t side, hipEvent_t fork, hipEvent_t join, int nfront,
+                    hipEvent_t ready, int side_blocks) {
+  if (args.nleaves <= 0) {
+    if (ready) FMM_HIP_TRY(hipStreamWaitEvent(s, ready, 0));
+    return 0;
+  }
+  if (launch_parent_local(args, mixed, nullptr, split_blocks, s)) return -1;
+  if (ready) FMM_HIP_TRY(hipStreamWaitEvent(s, ready, 0));
+  const bool split = args.split_budget > 0;
+  if (split) {
+    zero_counters_kernel<<<1, 64, 0, s>>>(d_split_ctr, 5);
+    FMM_HIP_TRY(hipGetLastError());
+  }
+  FmmArgs a = args;
+  a.pos0 = 0;
+  a.npos = args.nleaves;
+  a.ck_base = 0;
+  if (split && side && fork && join && nfront > 0 && nfront < args.nleaves) {
+    // the front leaves (the longest walks; they hold nearly all the split
+    // leaves) and their continuation pass on s, concurrently with the other
+    // leaves on the side stream: the continuation pass overlaps the fused
+    // pass instead of following it on an idle device
+    FmmArgs b = a;
+    a.npos = nfront;
+    b.pos0 = nfront;
+    b.npos = args.nleaves - nfront;
+    b.ck = args.ck + args.ck_cap;
+    b.ck_count = d_split_ctr + 2;
+    b.ck_base = args.ck_cap;
+    FMM_HIP_TRY(hipEventRecord(fork, s));
+    FMM_HIP_TRY(hipStreamWaitEvent(side, fork, 0));
+    if (launch_leaf_grid(a, mixed, s) || launch_leaf_grid(b, mixed, side)) return -1;
+    if (launch_chunk_pass(a, mixed, d_split_ctr + 1, split_blocks, s)) return -1;
+    FMM_HIP_TRY(hipEventRecord(join, side));
+    FMM_HIP_TRY(hipStreamWaitEvent(s, join, 0));
+    if (mid) FMM_HIP_TRY(hipEventRecord(mid, s));
+    return launch_split_passes(b, mixed, d_split_ctr + 3, std::max(1, side_blocks), s);
+  }
+  if (launch_leaf_grid(a, mixed, s)) return -1;
   if (mid) FMM_HIP_TRY(hipEventRecord(mid, s));
-  if (split) return launch_split_passes(args, mixed, d_nleaves, d_split_ctr + 1, split_blocks, s);
+  if (split) return launch_split_passes(a, mixed, d_split_ctr + 1, split_blocks, s);
   return 0;
 }
 
