@@ -288,7 +288,11 @@ int fmm_count_work_detail(long long *out, int n);
  * leaf i (nleaves = fmm_num_target_leaves()). */
 int fmm_count_work_leaves(long long *out, int nleaves);
 
-/* Wait for all work of the library; release all device memory. */
+/* Wait for all work of the library; release all device memory.  The library
+ * works on two non-blocking streams of its own, at the default priority: the
+ * evaluation runs its front leaves on one and the other leaves on the other
+ * and joins them before it ends; make_tree leaves some set-up on the second,
+ * which every later call waits for.  fmm_synchronize waits for both. */
 int fmm_synchronize(void);
 void fmm_release(void);
 
