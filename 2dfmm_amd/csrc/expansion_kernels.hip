@@ -38,12 +38,16 @@ __device__ inline void halve(double (&v)[NV], int lane, int o) {
 // NV = 2 x (terms rounded up to 16 or 32) values per lane.
 // Nodes that get no expansion otherwise -- empty leaves and the roots (never
 // shifted into, fmm.c:41) -- get zeros here, so no clearing pass is needed.
+// leaf_abs (optional): sum of |cir| over each leaf's blobs (zero for empty
+// leaves and roots), summed per lane and then across the wave in a fixed
+// order.
 template <int NV>
 __global__ __launch_bounds__(256) void s2m_kernel(const NodeRec *__restrict__ nodes, int nnodes,
                                                   int nroot_slots, const double *__restrict__ sx,
                                                   const double *__restrict__ sy,
                                                   const double *__restrict__ sg, int nterms,
-                                                  double2 *__restrict__ coef) {
+                                                  double2 *__restrict__ coef,
+                                                  double *__restrict__ leaf_abs) {
   const int node = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (node >= nnodes) return;
@@ -51,9 +55,11 @@ __global__ __launch_bounds__(256) void s2m_kernel(const NodeRec *__restrict__ no
   const int c = node_count(r.cnt);
   if (node < nroot_slots || (!node_internal(r.cnt) && c == 0)) {
     if (lane < nterms) coef[(size_t)node * nterms + lane] = make_double2(0.0, 0.0);
+    if (leaf_abs && lane == 0) leaf_abs[node] = 0.0;
     return;
   }
   if (node_internal(r.cnt)) return;
+  double gabs = 0.0;
   // v[2k], v[2k+1] = cir (z - z_c)^k (real, imaginary) of this lane's blob;
   // after the reduce-scatter lane l holds the wave's sum of v[l * NV / 64]
   double acc = 0.0;
@@ -63,6 +69,7 @@ __global__ __launch_bounds__(256) void s2m_kernel(const NodeRec *__restrict__ no
     const double tr = ok ? sx[r.first + e] - r.cx : 0.0;
     const double ti = ok ? sy[r.first + e] - r.cy : 0.0;
     const double g = ok ? sg[r.first + e] : 0.0;
+    gabs += fabs(g);
     double v[NV];
     double pr = 1.0, pi = 0.0;
 #pragma unroll
@@ -89,16 +96,22 @@ __global__ __launch_bounds__(256) void s2m_kernel(const NodeRec *__restrict__ no
   const int j = NV == 64 ? lane : lane >> 1, k = j >> 1;
   if ((NV == 64 || (lane & 1) == 0) && k < nterms)
     reinterpret_cast<double *>(coef)[2 * ((size_t)node * nterms + k) + (j & 1)] = acc;
+  if (leaf_abs) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) gabs += __shfl_xor(gabs, o, 64);
+    if (lane == 0) leaf_abs[node] = gabs;
+  }
 }
 
 int launch_s2m(const NodeRec *d_nodes, int nnodes, int nroot_slots, const double *sx,
-               const double *sy, const double *sg, int nterms, double2 *d_coef, hipStream_t s) {
+               const double *sy, const double *sg, int nterms, double2 *d_coef, double *d_leaf_abs,
+               hipStream_t s) {
   if (nterms <= 16)
     s2m_kernel<32><<<(nnodes + 3) / 4, 256, 0, s>>>(d_nodes, nnodes, nroot_slots, sx, sy, sg,
-                                                    nterms, d_coef);
+                                                    nterms, d_coef, d_leaf_abs);
   else
     s2m_kernel<64><<<(nnodes + 3) / 4, 256, 0, s>>>(d_nodes, nnodes, nroot_slots, sx, sy, sg,
-                                                    nterms, d_coef);
+                                                    nterms, d_coef, d_leaf_abs);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -86,7 +86,7 @@ struct Context {
   // sources, smooth coefficients) run on the side stream after ev_sfork;
   // ev_side marks their end, and side_pending says the main stream has not
   // waited for it yet (join_side)
-  hipEvent_t ev_sfork = nullptr, ev_side = nullptr;
+  hipEvent_t ev_sfork = nullptr, ev_side1 = nullptr, ev_side = nullptr;
   bool side_pending = false;
   bool init = false;
   // tree
@@ -120,6 +120,7 @@ struct Context {
   float cut_min = 2.5f;
   DevBuf<float> mlp_w, feat, mlp_out;
   DevBuf<double> mean_abs;
+  DevBuf<double> leaf_abs;  // per node: sum of |cir| over a leaf's blobs (learned features)
   DevBuf<float2> leaf_par;   // learned (or explicitly set) per-leaf (p, cut)
   bool leaf_par_on = false;
   // mirror images: the tree holds n_real blobs and their images after them
@@ -211,6 +212,7 @@ static int ctx_init() {
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_sfork, hipEventDisableTiming));
+  FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_side1, hipEventDisableTiming));
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_side, hipEventDisableTiming));
   FMM_HIP_TRY(hipHostMalloc((void **)&g.h_total, 64, hipHostMallocDefault));
   // the level table, then the root square (read back together, pinned)
@@ -451,8 +453,10 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   mark(3);
   const size_t ncoef = (size_t)g.nnodes * g.t_nterms;
   if (g.coef.ensure(ncoef, s)) return -1;
+  const bool accurate = g.t_mode != FMM2D_MODE_TREECODE;
+  if (accurate && g.leaf_abs.ensure(g.nnodes, s)) return -1;
   if (launch_s2m(g.nodes.p, g.nnodes, nroot_slots, g.sx.p, g.sy.p, g.sg.p, g.t_nterms, g.coef.p,
-                 s))
+                 accurate ? g.leaf_abs.p : nullptr, s))
     return -1;
   for (int l = g.depth - 1; l >= 1; l--) {
     const int a = g.split_off[l], b = g.split_off[l + 1];
@@ -490,16 +494,6 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     // the FP32 pair kernel's w = log2(e) / sigma^2 is capped at 7.2e19
     g.wmax_h = smm[1] > 0.0 ? std::min(1.4426950408889634 / (smm[1] * smm[1]), 7.2e19) : 7.2e19;
     g.sw_h = smm[0] == smm[1] ? std::sqrt(g.wmax_h) : 0.0;
-    // the launch order: one stable 8-bit radix pass over the keys
-    if (radix_sort_pairs(&g.lkey.p, &g.lord.p, &g.lkey_alt.p, &g.lord_alt.p, g.ntleaves, 0, 8,
-                         g.hist.p, g.hist_scan.p, s))
-      return -1;
-    if (g.nfirst.ensure(g.nnodes, s) || launch_node_first_blob(g.nodes.p, g.nnodes, g.nfirst.p, s))
-      return -1;
-    // L2L: node -> target-leaf index, sibling groups, parent expansion slots
-    if (g.nodeli.ensure(g.nnodes, s) || g.plocal.ensure((size_t)(g.nnodes / 4 + 1) * 16, s) ||
-        launch_l2l_setup(g.leaves.p, g.ntleaves, g.nodeli.p, g.ntl_dev.p + 1, g.nnodes / 4, s))
-      return -1;
     // smooth cells: mixed form with one core radius only
     const bool smooth = g.t_mode == FMM2D_MODE_MIXED && g.sw_h > 0.0 && g.smooth_eta > 0.0;
     const double region = 2.0 * g.smooth_eta * smm[0];
@@ -518,37 +512,57 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
       n0 = l == 0 ? 0 : g.h_lt[kLtCb + l];
       n1 = l == 0 ? nroot_slots : g.h_lt[kLtCb + l + 1];
     }
-    if (smooth && g.scoef.ensure((size_t)g.nnodes * kSmoothF4, s)) return -1;
-    // Leaf-relative sources and smooth coefficients feed the evaluation only:
-    // they run on the side stream while the main one forms the smooth ranges,
-    // the learned parameters and (in the evaluation) the parent expansions.
+    if (g.nfirst.ensure(g.nnodes, s) || g.nodeli.ensure(g.nnodes, s) ||
+        g.plocal.ensure((size_t)(g.nnodes / 4 + 1) * 16, s))
+      return -1;
+    if (smooth && (g.scoef.ensure((size_t)g.nnodes * kSmoothF4, s) ||
+                   g.sflag.ensure(2 * (size_t)g.nnodes, s) || g.srange.ensure(g.nnodes, s) ||
+                   g.sunits.ensure((size_t)g.nnodes + 1, s) || g.sutmp.ensure(g.nnodes, s)))
+      return -1;
+    // The rest of the set-up runs on the side stream while the main one forms
+    // the learned parameters (which read only the tree, the multipoles and the
+    // per-leaf |cir| sums): first what the parent expansions read (node ->
+    // target leaf, smooth ranges; event ev_side1), then what only the fused
+    // pass reads (launch order, leaf-relative sources, smooth coefficients and
+    // work units; event ev_side).  The evaluation waits for each just before
+    // it needs it; every other later call waits for both (join_side).
+    hipStream_t t = g.side;
     FMM_HIP_TRY(hipEventRecord(g.ev_sfork, s));
-    FMM_HIP_TRY(hipStreamWaitEvent(g.side, g.ev_sfork, 0));
-    if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
-                          g.loc.p, g.loc64.p, g.sw_h, g.side))
-      return -1;
-    if (smooth && launch_smooth_coef(g.nodes.p, n0, n1, g.nfirst.p, g.sx.p, g.sy.p, g.sg.p,
-                                     smm[0], region, smin, g.scoef.p, g.side))
-      return -1;
-    FMM_HIP_TRY(hipEventRecord(g.ev_side, g.side));
-    g.side_pending = true;
+    FMM_HIP_TRY(hipStreamWaitEvent(t, g.ev_sfork, 0));
     if (smooth) {
-      if (g.sflag.ensure(2 * (size_t)g.nnodes, s) || g.srange.ensure(g.nnodes, s) ||
-          launch_smooth_ranges(g.nodes.p, g.gparent.p, g.nnodes, nroot_slots, region, smin,
-                               g.sflag.p, g.srange.p, s))
-        return -1;
-      if (g.sunits.ensure((size_t)g.nnodes + 1, s) || g.sutmp.ensure(g.nnodes, s) ||
-          launch_smooth_units(g.nodes.p, g.nnodes, region, smin, g.sutmp.p, g.sunits.p, s))
-        return -1;
       g.smooth_region_h = region;
       g.t_smooth_min = smin;
     }
-    // valid already for the fused kernel's arguments used by the learned step
+    // the learned parameters first: they lead to the parent expansions, so
+    // the host enqueues them before the side stream's work
     g.valid = true;
     if (g.learned && apply_learned()) {
       g.valid = false;
       return -1;
     }
+    // L2L: node -> target-leaf index, sibling groups, parent expansion slots
+    if (launch_l2l_setup(g.leaves.p, g.ntleaves, g.nodeli.p, g.ntl_dev.p + 1, g.nnodes / 4, t))
+      return -1;
+    if (smooth && launch_smooth_ranges(g.nodes.p, g.gparent.p, g.nnodes, nroot_slots, region, smin,
+                                       g.sflag.p, g.srange.p, t))
+      return -1;
+    FMM_HIP_TRY(hipEventRecord(g.ev_side1, t));
+    // the second part now: enqueued after the evaluation's parent expansions,
+    // its kernels found no free wave slots until those ended
+    if (radix_sort_pairs(&g.lkey.p, &g.lord.p, &g.lkey_alt.p, &g.lord_alt.p, g.ntleaves, 0, 8,
+                         g.hist.p, g.hist_scan.p, t))
+      return -1;  // the launch order: one stable 8-bit radix pass over the keys
+    if (launch_node_first_blob(g.nodes.p, g.nnodes, g.nfirst.p, t)) return -1;
+    if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
+                          g.loc.p, g.loc64.p, g.sw_h, t))
+      return -1;
+    if (smooth && (launch_smooth_coef(g.nodes.p, n0, n1, g.nfirst.p, g.sx.p, g.sy.p, g.sg.p,
+                                      smm[0], region, smin, g.scoef.p, t) ||
+                   launch_smooth_units(g.nodes.p, g.nnodes, region, smin, g.sutmp.p, g.sunits.p,
+                                       t)))
+      return -1;
+    FMM_HIP_TRY(hipEventRecord(g.ev_side, t));
+    g.side_pending = true;
   }
   mark(4);
   g.valid = true;
@@ -674,7 +688,8 @@ static int apply_learned() {
       g.leaf_par.ensure(nl, s) || g.mean_abs.ensure(kMeanAbsWords, s))
     return -1;
   FmmArgs a = fmm_args(nullptr);
-  if (launch_leaf_features(a, g.t_num_p, g.sg.p, g.n, g.mean_abs.p, g.feat.p, s)) return -1;
+  if (launch_leaf_features(a, g.t_num_p, g.sg.p, g.n, g.leaf_abs.p, g.mean_abs.p, g.feat.p, s))
+    return -1;
   if (launch_mlp_forward(g.mlp_w.p, nl, g.feat.p, g.mlp_out.p, s)) return -1;
   if (launch_leaf_params(g.mlp_out.p, nl, g.p_margin, g.cut_margin, g.cut_min, g.t_nterms,
                          (float)g.fmm_cut,
@@ -743,10 +758,11 @@ static int eval_sources_dev(double *d_uv) {
   const bool mixed = g.t_mode == FMM2D_MODE_MIXED;
   if (g.t_mode != FMM2D_MODE_TREECODE && ensure_split()) return -1;
   if ((g.t_mode == FMM2D_MODE_TREECODE || g.persist_blocks > 0) && join_side()) return -1;
-  // the default path waits for the build's side work after the parent
-  // expansions (which do not read it)
+  // the default path waits for the build's side work in two parts: before
+  // the parent expansions for what they read, before the fused pass for the
+  // rest
+  hipEvent_t ready1 = g.side_pending ? g.ev_side1 : nullptr;
   hipEvent_t ready = g.side_pending ? g.ev_side : nullptr;
-  g.side_pending = false;
   if (g.t_mode == FMM2D_MODE_TREECODE)
     rc = g.parity_leaves ? launch_parity_leaves(view(), g.n, g.perm.p, d_uv, g.stream)
                          : launch_parity_eval(view(), g.n, nullptr, g.perm.p, d_uv, g.stream);
@@ -758,7 +774,8 @@ static int eval_sources_dev(double *d_uv) {
   else
     rc = launch_fmm_eval(fmm_args(d_uv), mixed, g.ntl_dev.p, g.split_ctr.p, split_blocks(),
                          g.stream, g.timing ? g.tev[9] : nullptr, g.side, g.ev_fork, g.ev_join,
-                         g.nfront, ready, side_blocks());
+                         g.nfront, ready, side_blocks(), ready1);
+  g.side_pending = false;
   if (!rc && g.timing && g.t_mode != FMM2D_MODE_TREECODE && g.persist_blocks == 0)
     g.tev_rec[9] = true;
   mark(6);
@@ -1350,7 +1367,8 @@ int fmm_leaf_features(float *feat) {
   if (nl <= 0) return 0;
   hipStream_t s = g.stream;
   if (g.feat.ensure((size_t)nl * 8, s) || g.mean_abs.ensure(kMeanAbsWords, s)) return -1;
-  if (launch_leaf_features(fmm_args(nullptr), g.t_num_p, g.sg.p, g.n, g.mean_abs.p, g.feat.p, s))
+  if (launch_leaf_features(fmm_args(nullptr), g.t_num_p, g.sg.p, g.n, g.leaf_abs.p, g.mean_abs.p,
+                           g.feat.p, s))
     return -1;
   FMM_HIP_TRY(hipMemcpyAsync(feat, g.feat.p, sizeof(float) * nl * 8, hipMemcpyDeviceToHost, s));
   FMM_HIP_TRY(hipStreamSynchronize(s));

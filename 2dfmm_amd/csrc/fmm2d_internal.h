@@ -107,9 +107,11 @@ int launch_gather(const uint32_t *d_perm, int n, const double *d_xy, const doubl
 
 // ---- expansions (expansion_kernels.hip) ----------------------------------
 // S2M of every non-empty leaf; zeros for empty leaves and the first
-// nroot_slots nodes (roots); internal nodes are left to M2M.
+// nroot_slots nodes (roots); internal nodes are left to M2M.  d_leaf_abs
+// (optional, per node): the sum of |cir| over each leaf's blobs.
 int launch_s2m(const NodeRec *d_nodes, int nnodes, int nroot_slots, const double *sx,
-               const double *sy, const double *sg, int nterms, double2 *d_coef, hipStream_t s);
+               const double *sy, const double *sg, int nterms, double2 *d_coef, double *d_leaf_abs,
+               hipStream_t s);
 int launch_m2m_level(const NodeRec *d_nodes, const int32_t *d_parents, int nparents, int nterms,
                      double2 *d_coef, hipStream_t s);
 
@@ -302,14 +304,14 @@ int launch_node_first_blob(const NodeRec *d_nodes, int nnodes, int32_t *d_out, h
 // other leaves run on `side` (region 1, records from args.ck_cap on); s then
 // waits for side (events fork and join), sums region 1's records and adds the
 // chunk sums.  mid (optional): an event recorded before the last continuation
-// pass.  ready (optional): an event s waits for after the parent expansions,
-// before the fused pass (the build's work on the side stream).  side_blocks:
-// the grid (x 256 threads) of region 1's continuation pass.
+// pass.  ready1, ready (optional): events s waits for before the parent
+// expansions and before the fused pass (the build's work on the side stream).
+// side_blocks: the grid (x 256 threads) of region 1's continuation pass.
 int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
                     uint32_t *d_split_ctr, int split_blocks, hipStream_t s,
                     hipEvent_t mid = nullptr, hipStream_t side = nullptr,
                     hipEvent_t fork = nullptr, hipEvent_t join = nullptr, int nfront = 0,
-                    hipEvent_t ready = nullptr, int side_blocks = 0);
+                    hipEvent_t ready = nullptr, int side_blocks = 0, hipEvent_t ready1 = nullptr);
 
 // ---- time stepping around the evaluation (step_kernels.hip) ---------------
 int launch_drift_ab3(int n, double *d_xy, const double *d_hist, double dt, int mirror, double dm,
@@ -323,12 +325,14 @@ int launch_mirror_images(int n, const double *d_xy, const double *d_cir, double 
 
 // ---- learned per-leaf parameters (learned_kernels.hip) --------------------
 constexpr int kMlpWeights = 4866;  // 8 -> 64 -> 64 -> 2
-// d_cir: the n circulations in leaf order; d_mean_abs: kMeanAbsWords
-// doubles, [0] gets mean |cir| (computed in a fixed order), the rest is
-// scratch for block partials.
+// d_cir: the n circulations in leaf order; d_leaf_abs: per node, the sum of
+// |cir| over a leaf's blobs (launch_s2m); d_mean_abs: kMeanAbsWords doubles,
+// [0] gets mean |cir| (computed in a fixed order), the rest is scratch for
+// block partials.
 constexpr int kMeanAbsWords = 1 + 1024;
 int launch_leaf_features(const FmmArgs &a, int leaf_cap, const double *d_cir, int n,
-                         double *d_mean_abs, float *d_feat, hipStream_t s);
+                         const double *d_leaf_abs, double *d_mean_abs, float *d_feat,
+                         hipStream_t s);
 int launch_mlp_forward(const float *d_w, int n, const float *d_x, float *d_out, hipStream_t s);
 size_t mlp_scratch_floats(int n);
 int launch_mlp_grad(const float *d_w, int n, const float *d_x, const float *d_y, float *d_scratch,

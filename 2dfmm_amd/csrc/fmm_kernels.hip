@@ -2266,11 +2266,12 @@ static int launch_leaf_grid(const FmmArgs &args, bool mixed, hipStream_t s) {
 int launch_fmm_eval(const FmmArgs &args, bool mixed, const uint32_t *d_nleaves,
                     uint32_t *d_split_ctr, int split_blocks, hipStream_t s, hipEvent_t mid,
                     hipStream_t side, hipEvent_t fork, hipEvent_t join, int nfront,
-                    hipEvent_t ready, int side_blocks) {
+                    hipEvent_t ready, int side_blocks, hipEvent_t ready1) {
   if (args.nleaves <= 0) {
     if (ready) FMM_HIP_TRY(hipStreamWaitEvent(s, ready, 0));
     return 0;
   }
+  if (ready1) FMM_HIP_TRY(hipStreamWaitEvent(s, ready1, 0));
   if (launch_parent_local(args, mixed, nullptr, split_blocks, s)) return -1;
   if (ready) FMM_HIP_TRY(hipStreamWaitEvent(s, ready, 0));
   const bool split = args.split_budget > 0;

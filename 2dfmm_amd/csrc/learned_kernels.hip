@@ -28,7 +28,7 @@ static_assert(kNW == kMlpWeights, "weight count");
 // One thread per target leaf (the features' logarithms and roots run on all
 // lanes; the leaf's |cir| are summed in blob order).
 __global__ __launch_bounds__(256) void leaf_features_kernel(const FmmArgs a, int leaf_cap,
-                                                            const double *__restrict__ cir,
+                                                            const double *__restrict__ leaf_abs,
                                                             const double *__restrict__ mean_abs,
                                                             float *__restrict__ feat) {
   const int li = blockIdx.x * 256 + threadIdx.x;
@@ -38,9 +38,8 @@ __global__ __launch_bounds__(256) void leaf_features_kernel(const FmmArgs a, int
   const int cA = node_count(NA.cnt);
   const int root = a.root_shift < 0 ? 0 : (int)(a.key[NA.first] >> a.root_shift);
   const double side = NA.region, rside = a.nodes[root].region;
-  double sabs = 0.0;
-  // circulations in leaf order (8 B per blob, one contiguous run per leaf)
-  for (int j = 0; j < cA; j++) sabs += fabs(cir[NA.first + j]);
+  // sum of |cir| over the leaf's blobs (formed by the S2M pass)
+  const double sabs = leaf_abs[A];
   const double2 a0 = a.coef[(size_t)A * a.p], a1 = a.coef[(size_t)A * a.p + 1];
   const int par = a.gparent[A >> 2];
   const int cpar = par >= 0 ? node_count(a.nodes[par].cnt) : cA;
@@ -89,13 +88,14 @@ __global__ __launch_bounds__(64) void abs_mean_kernel(double *__restrict__ out, 
 }
 
 int launch_leaf_features(const FmmArgs &a, int leaf_cap, const double *d_cir, int n,
-                         double *d_mean_abs, float *d_feat, hipStream_t s) {
+                         const double *d_leaf_abs, double *d_mean_abs, float *d_feat,
+                         hipStream_t s) {
   if (a.nleaves <= 0) return 0;
   int nb = (n + 255) / 256;
   nb = nb > kAbsBlocks ? kAbsBlocks : (nb < 1 ? 1 : nb);
   abs_sum_kernel<<<nb, 256, 0, s>>>(d_cir, n, d_mean_abs + 1);
   abs_mean_kernel<<<1, 64, 0, s>>>(d_mean_abs, nb, n);
-  leaf_features_kernel<<<(a.nleaves + 255) / 256, 256, 0, s>>>(a, leaf_cap, d_cir, d_mean_abs,
+  leaf_features_kernel<<<(a.nleaves + 255) / 256, 256, 0, s>>>(a, leaf_cap, d_leaf_abs, d_mean_abs,
                                                                 d_feat);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
