@@ -7,8 +7,8 @@ the previous ones):
   1. leaf     leaf capacity: the target tile of the near-field sums (a wave's
               targets: up to `leaf` targets x 64 / leaf source replicas) and,
               through the tree depth, how many far cells each leaf contracts;
-     blocks   persistent grid size in workgroups (0 = one wave per target
-              leaf over a sized grid, no work queue);
+     blocks   persistent grid size in units of four waves (0 = one wave per
+              target leaf over a sized grid, no work queue);
      graph    replay the persistent evaluation from a captured HIP graph;
   2. split    near-work units per chunk of a heavy target leaf: how the P2P
               tiles of one leaf are dealt to waves (fmm_set_split);

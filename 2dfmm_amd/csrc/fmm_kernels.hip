@@ -1893,20 +1893,22 @@ int launch_parent_local(const FmmArgs &args, bool mixed, int64_t *d_pcount, int 
 // counter until the device-side leaf count is exhausted.  Every wave exits
 // once the counter passes the count, so the grid always drains.  Nothing in
 // the launch depends on host-side counts, so it can be captured in a graph
-// and replayed on a rebuilt tree.
+// and replayed on a rebuilt tree.  One wave per workgroup at four waves per
+// SIMD, like the grid form (four waves per workgroup at three per SIMD ran
+// 1.1-1.4x slower).
 template <bool MIXED, bool UNI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fmm_leaf_persistent_kernel(FmmArgs a,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fmm_leaf_persistent_kernel(FmmArgs a,
                                                                   const uint32_t *__restrict__ nleaves,
                                                                   uint32_t *__restrict__ counter) {
-  __shared__ double2 s_loc[4][kFmmMaxP];
-  __shared__ double2 s_red[4][64];
-  __shared__ typename StageOf<MIXED>::T s_stage[4][stage_slots<MIXED>()];
-  __shared__ int64_t s_ck[4][kMaxCk];
-  __shared__ int s_stack[4][kStack];
-  __shared__ int s_far[4][kFarList];
-  __shared__ int32_t s_cku[4][kMaxCk];
-  __shared__ ParentBox s_pbox[4];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ double2 s_loc[1][kFmmMaxP];
+  __shared__ double2 s_red[1][64];
+  __shared__ typename StageOf<MIXED>::T s_stage[1][stage_slots<MIXED>()];
+  __shared__ int64_t s_ck[1][kMaxCk];
+  __shared__ int s_stack[1][kStack];
+  __shared__ int s_far[1][kFarList];
+  __shared__ int32_t s_cku[1][kMaxCk];
+  __shared__ ParentBox s_pbox[1];
+  const int w = 0, lane = threadIdx.x;
   const uint32_t total = *nleaves;
   for (;;) {
     uint32_t li = 0;
@@ -2026,11 +2028,11 @@ int launch_fmm_eval_persistent(const FmmArgs &args, bool mixed, const uint32_t *
     FMM_HIP_TRY(hipGetLastError());
   }
   if (mixed && args.sw > 0.f)
-    fmm_leaf_persistent_kernel<true, true><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
+    fmm_leaf_persistent_kernel<true, true><<<4 * nblocks, 64, 0, s>>>(args, d_nleaves, d_counter);
   else if (mixed)
-    fmm_leaf_persistent_kernel<true, false><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
+    fmm_leaf_persistent_kernel<true, false><<<4 * nblocks, 64, 0, s>>>(args, d_nleaves, d_counter);
   else
-    fmm_leaf_persistent_kernel<false, false><<<nblocks, 256, 0, s>>>(args, d_nleaves, d_counter);
+    fmm_leaf_persistent_kernel<false, false><<<4 * nblocks, 64, 0, s>>>(args, d_nleaves, d_counter);
   FMM_HIP_TRY(hipGetLastError());
   if (split) return launch_split_passes(args, mixed, d_split_ctr + 1, split_blocks, s);
   return 0;

@@ -120,9 +120,10 @@ int fmm_make_forest(int nsys, const int32_t *offsets, const double *xy, const do
 int fmm_make_forest_device(int nsys, const int32_t *offsets, const double *d_xy,
                            const double *d_cir, const double *d_cor);
 
-/* Accurate modes: evaluate with a persistent grid of `blocks` workgroups that
- * take target leaves from a work counter (0 = one workgroup per four leaves),
- * optionally replayed from a captured HIP graph (use_graph = 1). */
+/* Accurate modes: evaluate with a persistent grid of `blocks` x 4 waves (one
+ * wave per workgroup) that take target leaves from a work counter (0 = one
+ * wave per leaf over a sized grid), optionally replayed from a captured HIP
+ * graph (use_graph = 1). */
 int fmm_set_persistent(int blocks, int use_graph);
 /* Accurate modes: target leaves with more than `budget` units of near work
  * (a unit = one wave step over 64 target-source pairs, plus 2 per near
