@@ -140,6 +140,8 @@ struct Context {
   int l2l = 2;          // parent local expansions for parents of >= l2l target leaves (0: off)
   DevBuf<double2> plocal;
   DevBuf<int32_t> nodeli;
+  DevBuf<uint32_t> pfront;  // start lists of the children's walks (kFrontCap per group)
+  DevBuf<int32_t> pfcnt;
   DevBuf<SplitCk> ck;
   DevBuf<int32_t> nfirst;
   DevBuf<int2> leaf_ck;
@@ -513,7 +515,9 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
       n1 = l == 0 ? nroot_slots : g.h_lt[kLtCb + l + 1];
     }
     if (g.nfirst.ensure(g.nnodes, s) || g.nodeli.ensure(g.nnodes, s) ||
-        g.plocal.ensure((size_t)(g.nnodes / 4 + 1) * 16, s))
+        g.plocal.ensure((size_t)(g.nnodes / 4 + 1) * 16, s) ||
+        g.pfront.ensure((size_t)(g.nnodes / 4 + 1) * kFrontCap, s) ||
+        g.pfcnt.ensure((size_t)(g.nnodes / 4 + 1), s))
       return -1;
     if (smooth && (g.scoef.ensure((size_t)g.nnodes * kSmoothF4, s) ||
                    g.sflag.ensure(2 * (size_t)g.nnodes, s) || g.srange.ensure(g.nnodes, s) ||
@@ -642,6 +646,9 @@ static FmmArgs fmm_args(double *d_uv) {
   a.nodeli = g.nodeli.p;
   a.ngroups = g.ntl_dev.p ? g.ntl_dev.p + 1 : nullptr;
   a.ngroups_h = g.nnodes / 4;
+  // start lists hold node indices below 2^27 (the rest of the word is a mask)
+  a.pfront = (a.l2l && g.nnodes < (1 << kSkipBits)) ? g.pfront.p : nullptr;
+  a.pfcnt = g.pfcnt.p;
   return a;
 }
 
@@ -1958,6 +1965,8 @@ void fmm_release(void) {
   g.nfirst.release();
   g.plocal.release();
   g.nodeli.release();
+  g.pfront.release();
+  g.pfcnt.release();
   g.scoef.release();
   g.sflag.release();
   g.srange.release();

@@ -256,7 +256,16 @@ struct FmmArgs {
   const int32_t *nodeli;
   const uint32_t *ngroups;
   int ngroups_h;  // the same count on the host (sizes a grid; 0: read it on the device only)
+  // where the walks of a parent's target-leaf children start (written by the
+  // parent's walk): per sibling group g, pfcnt[g] entries pfront[kFrontCap g
+  // ...], each four siblings (first-child index | skipped-children mask << 27);
+  // pfcnt[g] < 0: no list (no parent expansion, or more than kFrontCap
+  // entries), the children walk from the root.  pfront null: off.
+  uint32_t *pfront;
+  int32_t *pfcnt;
 };
+constexpr int kFrontCap = 128;
+constexpr int kSkipBits = 27;  // node index bits of a start-list entry
 // Node -> leaf-list index of the target leaves, and the group count on the
 // device (the inputs of the parent expansions).
 int launch_l2l_setup(const int32_t *d_leaves, int nleaves, int32_t *d_nodeli,

@@ -1060,7 +1060,19 @@ constexpr bool kMergeLeaves = true;  // consecutive sibling near leaves as one r
 // ("covered") instead.  Stack entries carry the flag "the parent opened this
 // node" in bit 31.  Only parents with at least two target-leaf children take
 // part.
+//
+// The parent's walk also lists where its children's walks start (pfront):
+// every group of four siblings below a node all of them open that holds a
+// child neither covered nor opened by all of them (a "boundary" child), with
+// a mask of the children to skip (covered or opened by all; each one's part
+// is the parent's, or lies in later entries).  A child's walk then starts
+// from these entries, not from the root: it meets exactly the nodes the
+// walk from the root meets below the nodes all children open, classified by
+// the same rule, and no covered cell.  Entries: first-child index (below
+// 2^27) | skip mask << 27.
 constexpr int kCovBit = int(0x80000000u);
+constexpr int kSkipShift = kSkipBits;
+constexpr int kIndexMask = (1 << kSkipShift) - 1;
 // The parent's box and the target-leaf siblings of A, kept in the wave's LDS
 // (read only where a walk step meets a node the parent opened).
 struct ParentBox {
@@ -1073,7 +1085,20 @@ struct ParentCtx {
   int pP = 0;              // terms of the parent's expansion
   uint32_t sib = 0;        // A's other target-leaf siblings (mask over q)
   const ParentBox *box = nullptr;
+  const uint32_t *front = nullptr;  // where the walk starts (null: the root)
+  int nfront = 0;
 };
+
+// The start list of target leaf li's walk, or null (walk from the root).
+__device__ inline const uint32_t *leaf_front(const FmmArgs &a, int li, int &nfront) {
+  nfront = 0;
+  if (!a.l2l || !a.pfront) return nullptr;
+  const int g = rfl(ldro(a.leaves + li)) >> 2;
+  const int n = rfl(ldro(a.pfcnt + g));
+  if (n < 0) return nullptr;
+  nfront = n;
+  return a.pfront + (size_t)g * kFrontCap;
+}
 
 // Near radius and terms of target leaf li.
 __device__ inline void leaf_rcut_p(const FmmArgs &a, int li, double &rcut, int &p) {
@@ -1120,6 +1145,13 @@ __device__ inline ParentCtx parent_ctx(const FmmArgs &a, int li, int lane, Paren
   pc.g = g;
   pc.sib = m & ~(1u << (iA & 3));
   pc.box = box;
+  if (a.pfront) {
+    const int n = rfl(ldro(a.pfcnt + g));
+    if (n >= 0) {
+      pc.front = a.pfront + (size_t)g * kFrontCap;
+      pc.nfront = n;
+    }
+  }
   const NodeRec NP = ldro(a.nodes + par);
   double rcutP = 0.0;
 #pragma unroll
@@ -1187,10 +1219,15 @@ template <bool SMOOTH, bool L2L, typename FarFn, typename ItemFn>
 __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, int *stack,
                                  FarFn &&far, ItemFn &&item, const ParentCtx &pc) {
   const bool smooth_on = SMOOTH && a.smooth_region > 0.0;
-  const bool cov_on = L2L && pc.on;
-  const int root = rfl(root_of(a, L.NA.first));
+  // from the parent's start list the walk never meets a covered cell
+  const bool cov_on = L2L && pc.on && !pc.front;
   int ns = 1;
-  if (lane == 0) stack[0] = ldro(a.nodes + root).first | (cov_on ? kCovBit : 0);
+  if (pc.front) {
+    ns = pc.nfront;
+    for (int i = lane; i < ns; i += 64) stack[i] = (int)pc.front[i];
+  } else if (lane == 0) {
+    stack[0] = ldro(a.nodes + rfl(root_of(a, L.NA.first))).first | (cov_on ? kCovBit : 0);
+  }
   wave_sync();
   while (ns > 0) {
     const int k = min(min(16, ns), max(1, (kStack - kStackMargin - ns) / 3));
@@ -1199,7 +1236,8 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
     if ((lane >> 2) < k) {
       const int e = stack[ns - k + (lane >> 2)];
       cov = e < 0;
-      c = (e & 0x7fffffff) + (lane & 3);
+      c = (e & kIndexMask) + (lane & 3);
+      if ((e >> (kSkipShift + (lane & 3))) & 1) c = -1;  // skipped child of a start entry
     }
     wave_sync();
     ns -= k;
@@ -1374,10 +1412,13 @@ __device__ inline int64_t work_pos(int k, int t) { return ((int64_t)k << 32) | (
 // the fused pass's; items before the chunk are only counted); smooth cells in
 // batches, the last batch summed at the end.
 template <bool MIXED, bool UNI>
-__device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t start, int64_t end,
-                                 int lane, typename StageOf<MIXED>::T *st, int *stack, double &ax,
-                                 double &ay) {
+__device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int li, int64_t start,
+                                 int64_t end, int lane, typename StageOf<MIXED>::T *st, int *stack,
+                                 double &ax, double &ay) {
   constexpr bool SM = MIXED && UNI;
+  // the fused pass's walk order: from the parent's start list if it had one
+  ParentCtx pc;
+  pc.front = leaf_front(a, li, pc.nfront);
   const int k0 = (int)(start >> 32), t0 = (int)(uint32_t)start;
   const int k1 = (int)(end >> 32), t1 = (int)(uint32_t)end;
   int k = 0;
@@ -1402,7 +1443,7 @@ __device__ inline void run_chunk(const FmmArgs &a, const LeafCtx &L, int64_t sta
     const int tb = ki == k1 ? min(t1, nt) : nt;
     if (ta < tb) item_steps<MIXED, UNI>(a, L, c, C, kind, ser, ta, tb, lane, st, ax, ay, push);
     return ki < k1;
-  }, ParentCtx{});
+  }, pc);
   if constexpr (SM) {
     if (snb > 0) smooth_batch(L, snb, lane, st, ax, ay);
   }
@@ -1669,7 +1710,7 @@ __device__ __attribute__((always_inline)) void fmm_process_leaf(const FmmArgs &a
           const int64_t start = (int64_t)rfl64((uint64_t)s_ck[w][k]);
           const int64_t end = k + 1 < nck ? (int64_t)rfl64((uint64_t)s_ck[w][k + 1]) : INT64_MAX;
           double cx = 0.0, cy = 0.0;
-          run_chunk<MIXED, UNI>(a, L, start, end, lane, s_stage[w], stack, cx, cy);
+          run_chunk<MIXED, UNI>(a, L, li, start, end, lane, s_stage[w], stack, cx, cy);
           const double2 P = reduce_replicas(L, lane, s_red[w], cx, cy);
           extra.x += P.x;
           extra.y += P.y;
@@ -1763,6 +1804,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
     const uint32_t m = par >= 0 ? group_targets(a, g, C4, rc, pq) : 0u;
     if (__builtin_popcount(m) < a.l2l) {
       if (pcount && lane == 0) pcount[3 * (size_t)g] = pcount[3 * (size_t)g + 1] = pcount[3 * (size_t)g + 2] = 0;
+      if (a.pfront && lane == 0) a.pfcnt[g] = -1;
       continue;
     }
     const NodeRec NP = ldro(a.nodes + par);
@@ -1786,7 +1828,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (lane < kFmmMaxP) s_loc[w][lane] = make_double2(0.0, 0.0);
     int *stack = s_stack[w];
     const int root = rfl(root_of(a, first0));
-    int ns = 1, nfar = 0;
+    int ns = 1, nfar = 0, nfr = 0;
     int64_t total = 0;
     if (lane == 0) stack[0] = ldro(a.nodes + root).first;
     wave_sync();
@@ -1820,6 +1862,24 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
         if (ns + below < kStack) stack[ns + below] = C.first;
       }
       ns += __builtin_popcountll(om);
+      // the children's start list: each group of four (lanes 4j..4j+3) with
+      // a boundary child, the covered and opened ones marked to be skipped
+      if (a.pfront) {
+        const uint64_t bm = __builtin_amdgcn_ballot_w64(node_count(C.cnt) > 0 && !f && !op);
+        const int j4 = lane & ~3;
+        const bool head = (lane & 3) == 0 && ((bm >> j4) & 0xFull) != 0;
+        const uint64_t hm = __builtin_amdgcn_ballot_w64(head);
+        const int nh = __builtin_popcountll(hm);
+        if (nfr >= 0 && nfr + nh > kFrontCap) nfr = -1;  // too many: the children walk from the root
+        if (nfr >= 0) {
+          if (head)
+            a.pfront[(size_t)g * kFrontCap + nfr +
+                     __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0))] =
+                (uint32_t)c | ((uint32_t)(((fm | om) >> j4) & 0xFull) << kSkipShift);
+          nfr += nh;
+        }
+      }
       wave_sync();
       if (fm) {
         const int nf = __builtin_popcountll(fm);
@@ -1837,6 +1897,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
     if (nfar > 0) far_contract<MIXED>(a, NP, NP.region, pP, sPpow, lane, s_far[w], nfar, s_loc[w]);
     if (lane < kFmmMaxP) a.plocal[(size_t)g * kFmmMaxP + lane] = s_loc[w][lane];
+    if (a.pfront && lane == 0) a.pfcnt[g] = nfr;
     if (pcount && lane == 0) {
       pcount[3 * (size_t)g] = total;
       pcount[3 * (size_t)g + 1] = pP;
@@ -1941,8 +2002,8 @@ __global__ __launch_bounds__(256) void fmm_chunk_kernel(FmmArgs a, uint32_t *__r
     if (li < 0) continue;
     const LeafCtx L = leaf_ctx<MIXED, UNI>(a, li, lane);
     double ax = 0.0, ay = 0.0;
-    run_chunk<MIXED, UNI>(a, L, (int64_t)rfl64((uint64_t)r.cum), (int64_t)rfl64((uint64_t)r.end),
-                          lane, s_stage[w], s_stack[w], ax, ay);
+    run_chunk<MIXED, UNI>(a, L, li, (int64_t)rfl64((uint64_t)r.cum),
+                          (int64_t)rfl64((uint64_t)r.end), lane, s_stage[w], s_stack[w], ax, ay);
     const double2 P = reduce_replicas(L, lane, s_red[w], ax, ay);
     if (L.tact && L.rep == 0) a.part[((size_t)a.ck_base + j) * 64 + L.ti] = P;
   }

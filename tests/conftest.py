@@ -7,7 +7,15 @@ import sys
 import numpy as np
 import pytest
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# torch (device buffers and streams of some GPU tests) first, whichever tests
+# run: the HIP runtime it brings is then the one the library binds, as in
+# bench.py; loaded after the library, torch found no device
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
+ROOT =os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
