@@ -300,6 +300,45 @@ constexpr SmoothInt make_smooth_int() {
 }
 __constant__ SmoothInt kSmoothInt = make_smooth_int();
 
+// Per-cell constants of the coefficient transform, so that no thread forms
+// them in a loop: lam^k (k <= kSmoothQ + 1, lam = sqrt(ln 2), by repeated
+// products) and, for every term (a, b) of the n >= 1 part in storage order,
+// the integer weights w_i = sum_l (-1)^l C(i, b-l) C(a+b-i, l) (i <= a+b) of
+// d_ab = i^b sum_i w_i c_{i, a+b-i}.
+struct SmoothConst {
+  double lam[kSmoothQ + 2];
+  double w[kSmoothC - kSmoothQ - 1][kSmoothQ];
+  int8_t ta[kSmoothC - kSmoothQ - 1], tb[kSmoothC - kSmoothQ - 1];
+};
+constexpr SmoothConst make_smooth_const() {
+  constexpr int Q = kSmoothQ;
+  SmoothConst t{};
+  const SmoothInt bi = make_smooth_int();
+  t.lam[0] = 1.0;
+  for (int k = 1; k <= Q + 1; k++) t.lam[k] = t.lam[k - 1] * 0.83255461115769775635;
+  for (int o = 0; o < kSmoothC - Q - 1; o++) {
+    int k = o, j = Q - 1;
+    while (k >= Q - j) {
+      k -= Q - j;
+      j--;
+    }
+    const int a = Q - 1 - j - k, b = j, n = a + b;
+    t.ta[o] = (int8_t)a;
+    t.tb[o] = (int8_t)b;
+    for (int i = 0; i <= n; i++) {
+      const int jj = n - i;
+      double w = 0.0;
+      for (int l = (b - i > 0 ? b - i : 0); l <= (jj < b ? jj : b); l++) {
+        const double v = bi.binom[i][b - l] * bi.binom[jj][l];
+        w += (l & 1) ? -v : v;
+      }
+      t.w[o][i] = w;
+    }
+  }
+  return t;
+}
+__constant__ SmoothConst kSmoothConst = make_smooth_const();
+
 // Two waves per node (one thread per moment: (q+1)(q+2)/2 = 91 of them).
 // The node's blobs go through in chunks of 32: thread j forms cir_j d_j^m and
 // conj(d_j)^n (m, n <= q) of blob j in LDS, then thread t < (q+1)(q+2)/2 adds
@@ -388,20 +427,14 @@ __global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const N
   double vr = 0.0, vi = 0.0;
   int ti = 0, tj = 0;  // (i, j) of the n >= 1 term of this lane
   {
-    const double lam = 0.83255461115769775635;  // sqrt(ln 2)
     if (lane <= Q) {
       const int mm = Q - lane;
       const double2 v = mu[smooth_tri(mm)];
-      const double sc = pow(lam, (double)(mm + 1));
+      const double sc = kSmoothConst.lam[mm + 1];
       vr = v.x * sc;
       vi = v.y * sc;
     } else if (lane < kSmoothC) {
-      int k = lane - (Q + 1), j = Q - 1;
-      while (k >= Q - j) {
-        k -= Q - j;
-        j--;
-      }
-      const int i = Q - 1 - j - k;
+      const int i = kSmoothConst.ta[lane - (Q + 1)], j = kSmoothConst.tb[lane - (Q + 1)];
       for (int l = 0; i + j + 2 * l + 1 <= Q; l++) {
         const double2 v = mu[smooth_tri(j + l) + i + l + 1];
         double c = kSmoothInt.binom[j + l][l] * kSmoothInt.rise[i][l];  // C(j+l, l) (i+l)!/i!
@@ -409,7 +442,7 @@ __global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const N
         vr += v.x * c;
         vi += v.y * c;
       }
-      const double sc = pow(lam, (double)(i + j));
+      const double sc = kSmoothConst.lam[i + j];
       vr *= sc;
       vi *= sc;
       ti = i;
@@ -421,14 +454,11 @@ __global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const N
   if (lane > Q && lane < kSmoothC) {
     // d_ab (a = ti, b = tj): the terms c_ij with i + j = a + b
     const int a = ti, b = tj, n = a + b;
+    const double *wt = kSmoothConst.w[lane - (Q + 1)];
     double sr = 0.0, si = 0.0;
     for (int i = 0; i <= n; i++) {
       const int j = n - i;
-      double w = 0.0;
-      for (int l = max(0, b - i); l <= min(j, b); l++) {
-        const double t = kSmoothInt.binom[i][b - l] * kSmoothInt.binom[j][l];  // C(i, b-l) C(j, l)
-        w += (l & 1) ? -t : t;
-      }
+      const double w = wt[i];  // sum_l (-1)^l C(i, b-l) C(j, l)
       const double2 v = cz[smooth_slot(i, j)];
       sr += w * v.x;
       si += w * v.y;
