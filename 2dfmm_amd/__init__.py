@@ -71,6 +71,7 @@ lib.fmm_set_split.argtypes = [C.c_int]
 lib.fmm_set_far_burst.argtypes = [C.c_int]
 lib.fmm_set_treecode_walk.argtypes = [C.c_int]
 lib.fmm_set_l2l.argtypes = [C.c_int]
+lib.fmm_set_start_lists.argtypes = [C.c_int]
 lib.fmm_get_split.argtypes = []
 lib.fmm_set_smooth.argtypes = [C.c_double, C.c_int]
 lib.fmm_get_smooth.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -333,6 +334,12 @@ def set_l2l(min_children) -> None:
     if min_children is True:
         min_children = 2
     _check(lib.fmm_set_l2l(int(min_children)), "set_l2l")
+
+
+def set_start_lists(on: bool) -> None:
+    """Children's walks from the parent's start lists (True, default) or from
+    the root (False); the same interactions either way."""
+    _check(lib.fmm_set_start_lists(1 if on else 0), "set_start_lists")
 
 
 def set_treecode_walk(per_leaf: bool) -> None:

@@ -138,6 +138,7 @@ struct Context {
   int far_burst = 128;  // far cells per MFMA burst (64..128)
   bool parity_leaves = true;  // treecode at the blobs: a wave per 64 blobs (else a lane per target)
   int l2l = 2;          // parent local expansions for parents of >= l2l target leaves (0: off)
+  bool start_lists = true;  // children's walks from the parent's start lists
   DevBuf<double2> plocal;
   DevBuf<int32_t> nodeli;
   DevBuf<uint32_t> pfront;  // start lists of the children's walks (kFrontCap per group)
@@ -647,7 +648,7 @@ static FmmArgs fmm_args(double *d_uv) {
   a.ngroups = g.ntl_dev.p ? g.ntl_dev.p + 1 : nullptr;
   a.ngroups_h = g.nnodes / 4;
   // start lists hold node indices below 2^27 (the rest of the word is a mask)
-  a.pfront = (a.l2l && g.nnodes < (1 << kSkipBits)) ? g.pfront.p : nullptr;
+  a.pfront = (a.l2l && g.start_lists && g.nnodes < (1 << kSkipBits)) ? g.pfront.p : nullptr;
   a.pfcnt = g.pfcnt.p;
   return a;
 }
@@ -1297,6 +1298,16 @@ int fmm_set_l2l(int min_children) {
     return -1;
   }
   g.l2l = min_children == 1 ? 2 : min_children;
+  return 0;
+}
+
+int fmm_set_start_lists(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (on != 0 && on != 1) {
+    set_error("fmm_set_start_lists: %d (0 or 1)", on);
+    return -1;
+  }
+  g.start_lists = on == 1;
   return 0;
 }
 

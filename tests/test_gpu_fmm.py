@@ -250,3 +250,48 @@ def test_l2l_parent_expansions(fmm, oracle, mode):
         assert np.array_equal(got.view(np.uint64), u1.view(np.uint64)), (blocks, graph)
     fmm.set_persistent(0, False)
     fmm.set_mode(fmm.MODE_TREECODE)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_start_lists_same_interactions(fmm, oracle, mode):
+    """The children of a parent expansion walk from the parent's start list
+    (default) or from the root: both must meet the same far cells and near
+    items, so the velocities agree to the rounding of a different summation
+    order (FP64: 1e-12; mixed: FP32 pair sums grouped differently), with
+    heavy leaves split into continuation chunks (small split budget) and on
+    the persistent path.  A missing or repeated item would move some
+    targets by far more."""
+    wl = workloads()
+    n = 60000
+    xy, cir, cor = wl.gaussian_mixture(n, seed=82)
+    idx = wl.sample_targets(n, 1500, seed=82)
+    d = oracle.direct(xy, cir, cor, xy[idx], style=1)
+    tol = 1e-12 if mode == 1 else 1e-6
+    try:
+        fmm.set_split(256)
+        out = {}
+        for on in (False, True):
+            fmm.set_start_lists(on)
+            out[on] = _run(fmm, mode, xy, cir, cor)
+            e = rel_err(out[on][idx], d)
+            _say(f"[start lists] mode {mode} on={on}: rel_L2={e[0]:.2e} rel_max={e[1]:.2e}")
+            assert e[0] <= 5e-7
+        scale = np.abs(out[False]).max()
+        diff = np.abs(out[True] - out[False]).max() / scale
+        _say(f"[start lists] mode {mode}: max |on - off| / max |u| = {diff:.2e}")
+        assert diff <= tol
+        fmm.set_split(0)
+        whole = _run(fmm, mode, xy, cir, cor)
+        assert np.abs(whole - out[True]).max() / scale <= tol
+        fmm.set_split(256)
+        fmm.set_persistent(256, False)
+        fmm.make_tree(xy, cir, cor)
+        got = fmm.eval_sources()
+        assert np.array_equal(got.view(np.uint64), out[True].view(np.uint64))
+    finally:
+        fmm.set_start_lists(True)
+        fmm.set_persistent(0, False)
+        fmm.set_split(8192)
+        fmm.set_mode(fmm.MODE_TREECODE)
+
+
