@@ -1223,8 +1223,12 @@ __device__ inline void walk_wide(const FmmArgs &a, const LeafCtx &L, int lane, i
   const bool cov_on = L2L && pc.on && !pc.front;
   int ns = 1;
   if (pc.front) {
+    // reversed: the walk pops the top first, so it takes the groups in the
+    // parent's walk order (measured: the continuation chunks' walks reach
+    // their items sooner this way; 4M mixed form without smooth cells 35.4 ->
+    // 31.5 ms per step)
     ns = pc.nfront;
-    for (int i = lane; i < ns; i += 64) stack[i] = (int)pc.front[i];
+    for (int i = lane; i < ns; i += 64) stack[i] = (int)pc.front[ns - 1 - i];
   } else if (lane == 0) {
     stack[0] = ldro(a.nodes + rfl(root_of(a, L.NA.first))).first | (cov_on ? kCovBit : 0);
   }
