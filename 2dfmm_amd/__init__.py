@@ -336,10 +336,13 @@ def set_l2l(min_children) -> None:
     _check(lib.fmm_set_l2l(int(min_children)), "set_l2l")
 
 
-def set_start_lists(on: bool) -> None:
-    """Children's walks from the parent's start lists (True, default) or from
-    the root (False); the same interactions either way."""
-    _check(lib.fmm_set_start_lists(1 if on else 0), "set_start_lists")
+def set_start_lists(cap=True) -> None:
+    """Children's walks from the parent's start lists of at most `cap` entries
+    (True = 128, the default and most) or from the root (False / 0, and
+    parents whose list would be longer); the same interactions either way."""
+    if cap is True:
+        cap = 128
+    _check(lib.fmm_set_start_lists(int(cap)), "set_start_lists")
 
 
 def set_treecode_walk(per_leaf: bool) -> None:

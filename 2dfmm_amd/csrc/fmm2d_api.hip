@@ -138,7 +138,7 @@ struct Context {
   int far_burst = 128;  // far cells per MFMA burst (64..128)
   bool parity_leaves = true;  // treecode at the blobs: a wave per 64 blobs (else a lane per target)
   int l2l = 2;          // parent local expansions for parents of >= l2l target leaves (0: off)
-  bool start_lists = true;  // children's walks from the parent's start lists
+  int start_cap = kFrontCap;  // entries per start list (0: children walk from the root)
   DevBuf<double2> plocal;
   DevBuf<int32_t> nodeli;
   DevBuf<uint32_t> pfront;  // start lists of the children's walks (kFrontCap per group)
@@ -648,8 +648,9 @@ static FmmArgs fmm_args(double *d_uv) {
   a.ngroups = g.ntl_dev.p ? g.ntl_dev.p + 1 : nullptr;
   a.ngroups_h = g.nnodes / 4;
   // start lists hold node indices below 2^27 (the rest of the word is a mask)
-  a.pfront = (a.l2l && g.start_lists && g.nnodes < (1 << kSkipBits)) ? g.pfront.p : nullptr;
+  a.pfront = (a.l2l && g.start_cap > 0 && g.nnodes < (1 << kSkipBits)) ? g.pfront.p : nullptr;
   a.pfcnt = g.pfcnt.p;
+  a.front_cap = g.start_cap;
   return a;
 }
 
@@ -1301,13 +1302,13 @@ int fmm_set_l2l(int min_children) {
   return 0;
 }
 
-int fmm_set_start_lists(int on) {
+int fmm_set_start_lists(int cap) {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (on != 0 && on != 1) {
-    set_error("fmm_set_start_lists: %d (0 or 1)", on);
+  if (cap < 0 || cap > kFrontCap) {
+    set_error("fmm_set_start_lists: %d (0 = off, at most %d entries)", cap, kFrontCap);
     return -1;
   }
-  g.start_lists = on == 1;
+  g.start_cap = cap;
   return 0;
 }
 

@@ -259,10 +259,11 @@ struct FmmArgs {
   // where the walks of a parent's target-leaf children start (written by the
   // parent's walk): per sibling group g, pfcnt[g] entries pfront[kFrontCap g
   // ...], each four siblings (first-child index | skipped-children mask << 27);
-  // pfcnt[g] < 0: no list (no parent expansion, or more than kFrontCap
-  // entries), the children walk from the root.  pfront null: off.
+  // pfcnt[g] < 0: no list (no parent expansion, or more than front_cap <=
+  // kFrontCap entries), the children walk from the root.  pfront null: off.
   uint32_t *pfront;
   int32_t *pfcnt;
+  int front_cap;
 };
 constexpr int kFrontCap = 128;
 constexpr int kSkipBits = 27;  // node index bits of a start-list entry

@@ -1874,7 +1874,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
         const bool head = (lane & 3) == 0 && ((bm >> j4) & 0xFull) != 0;
         const uint64_t hm = __builtin_amdgcn_ballot_w64(head);
         const int nh = __builtin_popcountll(hm);
-        if (nfr >= 0 && nfr + nh > kFrontCap) nfr = -1;  // too many: the children walk from the root
+        if (nfr >= 0 && nfr + nh > a.front_cap) nfr = -1;  // too many: the children walk from the root
         if (nfr >= 0) {
           if (head)
             a.pfront[(size_t)g * kFrontCap + nfr +

@@ -146,13 +146,15 @@ int fmm_set_far_burst(int cells);
  * default 2); 0: every target leaf contracts all of its far cells itself. */
 int fmm_set_l2l(int min_children);
 /* Where the walks of the parent-expansion children start (no reference
- * counterpart).  on = 1 (default): the parent's walk lists the sibling groups
- * below the nodes all its target-leaf children open that hold a child neither
- * covered by its expansion nor opened by all of them, and each child's walk
- * starts there; 0: each child walks from the root and skips the covered cells
- * on the way.  Both meet the same far cells and near items (in another
- * order, so the sums agree to rounding, not bitwise); 1 takes fewer steps. */
-int fmm_set_start_lists(int on);
+ * counterpart).  cap > 0 (default 128, the most): the parent's walk lists the
+ * sibling groups below the nodes all its target-leaf children open that hold
+ * a child neither covered by its expansion nor opened by all of them, and
+ * each child's walk starts there; a parent with more than cap such groups,
+ * and every parent with cap = 0, leaves its children to walk from the root,
+ * skipping the covered cells on the way.  Both meet the same far cells and
+ * near items (in another order, so the sums agree to rounding, not
+ * bitwise); the lists take fewer steps. */
+int fmm_set_start_lists(int cap);
 /* Smooth cells (no reference counterpart; mixed mode with one core radius
  * sigma).  A near node whose half side is at most eta sigma and that holds at
  * least min_blobs blobs is summed for every target through a degree-12 Taylor

@@ -270,16 +270,20 @@ def test_start_lists_same_interactions(fmm, oracle, mode):
     try:
         fmm.set_split(256)
         out = {}
-        for on in (False, True):
-            fmm.set_start_lists(on)
-            out[on] = _run(fmm, mode, xy, cir, cor)
-            e = rel_err(out[on][idx], d)
-            _say(f"[start lists] mode {mode} on={on}: rel_L2={e[0]:.2e} rel_max={e[1]:.2e}")
+        # off, lists of at most 12 entries (most parents fall back to the
+        # root walk, the rest keep their lists), and the full lists
+        for cap in (0, 12, 128):
+            fmm.set_start_lists(cap)
+            out[cap] = _run(fmm, mode, xy, cir, cor)
+            e = rel_err(out[cap][idx], d)
+            _say(f"[start lists] mode {mode} cap={cap}: rel_L2={e[0]:.2e} rel_max={e[1]:.2e}")
             assert e[0] <= 5e-7
-        scale = np.abs(out[False]).max()
-        diff = np.abs(out[True] - out[False]).max() / scale
-        _say(f"[start lists] mode {mode}: max |on - off| / max |u| = {diff:.2e}")
-        assert diff <= tol
+        scale = np.abs(out[0]).max()
+        for cap in (12, 128):
+            diff = np.abs(out[cap] - out[0]).max() / scale
+            _say(f"[start lists] mode {mode}: max |cap {cap} - off| / max |u| = {diff:.2e}")
+            assert diff <= tol
+        out[True] = out[128]
         fmm.set_split(0)
         whole = _run(fmm, mode, xy, cir, cor)
         assert np.abs(whole - out[True]).max() / scale <= tol
