@@ -104,9 +104,11 @@ int launch_leaf_features(const FmmArgs &a, int leaf_cap, const double *d_cir, in
 // ---------------------------------------------------------------------------
 // MLP
 // ---------------------------------------------------------------------------
-// One thread per sample; weights staged in LDS.  With y != null also the
+// One thread per sample; weights staged in LDS.  With TRAIN also the
 // backward pass: per-sample activations and deltas for the weight gradients
-// of mean((out - y)^2) over the n samples.
+// of mean((out - y)^2) over the n samples (training; inference runs
+// mlp_infer_kernel below, whose multiply-adds are fused, so its outputs
+// differ from this forward pass by rounding only).
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void mlp_kernel(const float *__restrict__ wts, int n,
                                                   const float *__restrict__ x,
@@ -221,10 +223,49 @@ __global__ __launch_bounds__(256) void adam_kernel(float *__restrict__ w, const 
   w[k] -= lr * mh / (sqrtf(vh) + 1e-8f);
 }
 
+// Inference: one thread per row, the weights read at wave-uniform addresses
+// (scalar loads; no copy into LDS per block) and every multiply-add fused.
+__global__ __launch_bounds__(256) void mlp_infer_kernel(const float *__restrict__ wts, int n,
+                                                        const float *__restrict__ x,
+                                                        float *__restrict__ out) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= n) return;
+  float xin[kIn];
+#pragma unroll
+  for (int k = 0; k < kIn; k++) xin[k] = x[(size_t)b * kIn + k];
+  float h1[kH];
+#pragma unroll
+  for (int i = 0; i < kH; i++) {
+    float s = wts[oB1 + i];
+#pragma unroll
+    for (int k = 0; k < kIn; k++) s = fmaf(wts[oW1 + i * kIn + k], xin[k], s);
+    h1[i] = s > 0.f ? s : 0.f;
+  }
+  float o0 = wts[oB3], o1 = wts[oB3 + 1];
+  // four hidden units at a time: four independent multiply-add chains
+  for (int j = 0; j < kH; j += 4) {
+    float s[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) s[u] = wts[oB2 + j + u];
+#pragma unroll
+    for (int i = 0; i < kH; i++) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) s[u] = fmaf(wts[oW2 + (j + u) * kH + i], h1[i], s[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const float h = s[u] > 0.f ? s[u] : 0.f;
+      o0 = fmaf(wts[oW3 + j + u], h, o0);
+      o1 = fmaf(wts[oW3 + kH + j + u], h, o1);
+    }
+  }
+  out[(size_t)b * kOut] = o0;
+  out[(size_t)b * kOut + 1] = o1;
+}
+
 int launch_mlp_forward(const float *d_w, int n, const float *d_x, float *d_out, hipStream_t s) {
   if (n <= 0) return 0;
-  mlp_kernel<false><<<(n + 255) / 256, 256, 0, s>>>(d_w, n, d_x, nullptr, d_out, nullptr, nullptr,
-                                                    nullptr, nullptr, nullptr, nullptr);
+  mlp_infer_kernel<<<(n + 255) / 256, 256, 0, s>>>(d_w, n, d_x, d_out);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
