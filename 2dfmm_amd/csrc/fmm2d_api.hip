@@ -87,6 +87,8 @@ struct Context {
   // ev_side marks their end, and side_pending says the main stream has not
   // waited for it yet (join_side)
   hipEvent_t ev_sfork = nullptr, ev_side1 = nullptr, ev_side = nullptr;
+  // the core radii's largest and smallest, formed on the side stream during the tree build
+  hipEvent_t ev_mm = nullptr;
   bool side_pending = false;
   bool init = false;
   // tree
@@ -216,6 +218,7 @@ static int ctx_init() {
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_sfork, hipEventDisableTiming));
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_side1, hipEventDisableTiming));
+  FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_mm, hipEventDisableTiming));
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_side, hipEventDisableTiming));
   FMM_HIP_TRY(hipHostMalloc((void **)&g.h_total, 64, hipHostMallocDefault));
   // the level table, then the root square (read back together, pinned)
@@ -325,6 +328,16 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   if (n == 0) {
     g.valid = true;
     return 0;
+  }
+  // accurate modes: the core radii's largest and smallest (near radius, pair
+  // kernel scale, smooth cells) on the side stream, beside the sort and the
+  // tree levels; the main stream waits for them before the launch keys
+  if (g.t_mode != FMM2D_MODE_TREECODE) {
+    if (g.smax.ensure(2, s)) return -1;
+    FMM_HIP_TRY(hipEventRecord(g.ev_sfork, s));
+    FMM_HIP_TRY(hipStreamWaitEvent(g.side, g.ev_sfork, 0));
+    if (launch_minmax_reduce(dcor, n, g.smax.p, g.side)) return -1;
+    FMM_HIP_TRY(hipEventRecord(g.ev_mm, g.side));
   }
   int sysbits = 0;
   while ((1 << sysbits) < nsys) sysbits++;
@@ -476,7 +489,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
     if (launch_leaf_list(g.nodes.p, g.nnodes, n, g.lat.p, g.lflag.p, g.lpos.p, g.leaves.p,
                          g.ntl_dev.p, s))
       return -1;
-    if (launch_minmax_reduce(dcor, n, g.smax.p, s)) return -1;
+    FMM_HIP_TRY(hipStreamWaitEvent(s, g.ev_mm, 0));
     // launch order keys (boxes large against the near radius first) and the
     // number of such front leaves, read back with the leaf count
     const int lbound = std::min(n, g.nnodes);
