@@ -103,12 +103,70 @@ __global__ __launch_bounds__(256) void s2m_kernel(const NodeRec *__restrict__ no
   }
 }
 
+// nterms <= 16: one node per half-wave (a leaf holds 29 blobs on average at
+// leaf size 64, so a whole wave per leaf idled half of it).  The 32 lanes
+// go through the leaf's blobs 32 at a time; the reduce-scatter halves stay
+// inside the 32 lanes and leave lane j with the sum of value j (terms k =
+// j / 2, real or imaginary part j % 2).
+__global__ __launch_bounds__(256) void s2m_half_kernel(const NodeRec *__restrict__ nodes, int nnodes,
+                                                       int nroot_slots, const double *__restrict__ sx,
+                                                       const double *__restrict__ sy,
+                                                       const double *__restrict__ sg, int nterms,
+                                                       double2 *__restrict__ coef,
+                                                       double *__restrict__ leaf_abs) {
+  constexpr int NV = 32;
+  const int node = blockIdx.x * 8 + (threadIdx.x >> 5);
+  const int lane = threadIdx.x & 31;
+  if (node >= nnodes) return;
+  const NodeRec r = nodes[node];
+  const int c = node_count(r.cnt);
+  if (node < nroot_slots || (!node_internal(r.cnt) && c == 0)) {
+    if (lane < nterms) coef[(size_t)node * nterms + lane] = make_double2(0.0, 0.0);
+    if (leaf_abs && lane == 0) leaf_abs[node] = 0.0;
+    return;
+  }
+  if (node_internal(r.cnt)) return;
+  double gabs = 0.0, acc = 0.0;
+  for (int base = 0; base < c; base += 32) {
+    const int e = base + lane;
+    const bool ok = e < c;
+    const double tr = ok ? sx[r.first + e] - r.cx : 0.0;
+    const double ti = ok ? sy[r.first + e] - r.cy : 0.0;
+    const double g = ok ? sg[r.first + e] : 0.0;
+    gabs += fabs(g);
+    double v[NV];
+    double pr = 1.0, pi = 0.0;
+#pragma unroll
+    for (int k = 0; k < NV / 2; k++) {
+      v[2 * k] = k < nterms ? g * pr : 0.0;
+      v[2 * k + 1] = k < nterms ? g * pi : 0.0;
+      const double nr = pr * tr - pi * ti;
+      const double ni = pr * ti + pi * tr;
+      pr = nr;
+      pi = ni;
+    }
+    halve<NV / 2>(v, lane, 16);
+    halve<NV / 4>(v, lane, 8);
+    halve<NV / 8>(v, lane, 4);
+    halve<NV / 16>(v, lane, 2);
+    halve<NV / 32>(v, lane, 1);
+    acc += v[0];
+  }
+  const int k = lane >> 1;
+  if (k < nterms) reinterpret_cast<double *>(coef)[2 * ((size_t)node * nterms + k) + (lane & 1)] = acc;
+  if (leaf_abs) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) gabs += __shfl_xor(gabs, o, 64);
+    if (lane == 0) leaf_abs[node] = gabs;
+  }
+}
+
 int launch_s2m(const NodeRec *d_nodes, int nnodes, int nroot_slots, const double *sx,
                const double *sy, const double *sg, int nterms, double2 *d_coef, double *d_leaf_abs,
                hipStream_t s) {
   if (nterms <= 16)
-    s2m_kernel<32><<<(nnodes + 3) / 4, 256, 0, s>>>(d_nodes, nnodes, nroot_slots, sx, sy, sg,
-                                                    nterms, d_coef, d_leaf_abs);
+    s2m_half_kernel<<<(nnodes + 7) / 8, 256, 0, s>>>(d_nodes, nnodes, nroot_slots, sx, sy, sg,
+                                                     nterms, d_coef, d_leaf_abs);
   else
     s2m_kernel<64><<<(nnodes + 3) / 4, 256, 0, s>>>(d_nodes, nnodes, nroot_slots, sx, sy, sg,
                                                     nterms, d_coef, d_leaf_abs);
