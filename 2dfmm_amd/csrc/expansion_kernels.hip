@@ -448,16 +448,14 @@ __global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const N
   for (int j0 = 0; j0 < cnt; j0 += B) {
     const int nb = min(B, cnt - j0);
     if (lane < nb) {
+      // d^k by repeated products; cir d^k and conj(d)^k = conj(d^k) from it
       const double dr = (C.cx - bx) * inv_sigma, di = (C.cy - by) * inv_sigma;
-      double pr = bg, pi = 0.0, qr = 1.0, qi = 0.0;
+      double qr = 1.0, qi = 0.0;
       for (int k = 0; k <= Q; k++) {
-        pm[k][lane] = make_double2(pr, pi);
-        pn[k][lane] = make_double2(qr, qi);
-        double t = pr * dr - pi * di;  // times d
-        pi = pr * di + pi * dr;
-        pr = t;
-        t = qr * dr + qi * di;  // times conj(d)
-        qi = qi * dr - qr * di;
+        pm[k][lane] = make_double2(bg * qr, bg * qi);
+        pn[k][lane] = make_double2(qr, -qi);
+        const double t = fma(qr, dr, -(qi * di));  // times d
+        qi = fma(qr, di, qi * dr);
         qr = t;
       }
     }
@@ -471,8 +469,8 @@ __global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const N
     if (lane < NT) {
       for (int j = 0; j < nb; j++) {
         const double2 u = pm[m][j], v = pn[n][j];
-        ar += u.x * v.x - u.y * v.y;
-        ai += u.x * v.y + u.y * v.x;
+        ar = fma(u.x, v.x, fma(-u.y, v.y, ar));
+        ai = fma(u.x, v.y, fma(u.y, v.x, ai));
       }
     }
     __syncthreads();
