@@ -140,8 +140,8 @@ __global__ __launch_bounds__(256) void s2m_half_kernel(const NodeRec *__restrict
     for (int k = 0; k < NV / 2; k++) {
       v[2 * k] = k < nterms ? g * pr : 0.0;
       v[2 * k + 1] = k < nterms ? g * pi : 0.0;
-      const double nr = pr * tr - pi * ti;
-      const double ni = pr * ti + pi * tr;
+      const double nr = fma(pr, tr, -(pi * ti));
+      const double ni = fma(pr, ti, pi * tr);
       pr = nr;
       pi = ni;
     }
