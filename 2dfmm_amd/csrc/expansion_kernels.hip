@@ -412,8 +412,9 @@ __global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const N
                                                          const double *__restrict__ sx,
                                                          const double *__restrict__ sy,
                                                          const double *__restrict__ sg,
-                                                         double inv_sigma, double region,
-                                                         int smin, float4 *__restrict__ out) {
+                                                         const double *__restrict__ smax,
+                                                         double eta, int smin,
+                                                         float4 *__restrict__ out) {
   constexpr int Q = kSmoothQ;
   constexpr int NT = (Q + 1) * (Q + 2) / 2;
   __shared__ double2 mu[NT];
@@ -425,6 +426,9 @@ __global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const N
   __shared__ double2 pn[Q + 1][B + 1];  // conj(d)^n
   const int node = node0 + blockIdx.x;
   if (node >= nnodes) return;
+  // one core radius only (smax[0] = largest, smax[1] = smallest)
+  if (smax[0] != smax[1]) return;
+  const double region = 2.0 * eta * smax[0], inv_sigma = 1.0 / smax[0];
   const NodeRec C = nodes[node];
   const int cnt = node_count(C.cnt);
   if (!(C.region <= region && 2.0 * C.region > region && cnt >= smin)) return;
@@ -532,11 +536,11 @@ __global__ __launch_bounds__(kSmoothCoefThreads) void smooth_coef_kernel(const N
 }
 
 int launch_smooth_coef(const NodeRec *d_nodes, int node0, int node1, const int32_t *d_nfirst,
-                       const double *sx, const double *sy, const double *sg, double sigma,
-                       double region, int smin, float4 *d_out, hipStream_t s) {
+                       const double *sx, const double *sy, const double *sg,
+                       const double *d_smax, double eta, int smin, float4 *d_out, hipStream_t s) {
   if (node1 <= node0) return 0;
   smooth_coef_kernel<<<node1 - node0, kSmoothCoefThreads, 0, s>>>(d_nodes, node0, node1, d_nfirst, sx, sy, sg,
-                                                  1.0 / sigma, region, smin, d_out);
+                                                  d_smax, eta, smin, d_out);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -621,10 +625,13 @@ int launch_smooth_ranges(const NodeRec *d_nodes, const int32_t *d_gparent, int n
 // Work units of the nodes of the smooth level (the others 0): a smooth cell
 // kSmoothUnits, a small leaf its blobs + kStepUnits; then their prefix sums.
 __global__ __launch_bounds__(256) void smooth_units_kernel(const NodeRec *__restrict__ nodes,
-                                                           int nnodes, double region, int smin,
+                                                           int nnodes,
+                                                           const double *__restrict__ smax,
+                                                           double eta, int smin,
                                                            uint32_t *__restrict__ u) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nnodes) return;
+  const double region = 2.0 * eta * smax[0];
   const NodeRec C = nodes[i];
   const int cnt = node_count(C.cnt);
   uint32_t v = 0;
@@ -633,10 +640,11 @@ __global__ __launch_bounds__(256) void smooth_units_kernel(const NodeRec *__rest
   u[i] = v;
 }
 
-int launch_smooth_units(const NodeRec *d_nodes, int nnodes, double region, int smin,
-                        uint32_t *d_tmp, uint32_t *d_out, hipStream_t s) {
+int launch_smooth_units(const NodeRec *d_nodes, int nnodes, const double *d_smax, double eta,
+                        int smin, uint32_t *d_tmp, uint32_t *d_out, hipStream_t s) {
   if (nnodes <= 0) return 0;
-  smooth_units_kernel<<<(nnodes + 255) / 256, 256, 0, s>>>(d_nodes, nnodes, region, smin, d_tmp);
+  smooth_units_kernel<<<(nnodes + 255) / 256, 256, 0, s>>>(d_nodes, nnodes, d_smax, eta, smin,
+                                                           d_tmp);
   FMM_HIP_TRY(hipGetLastError());
   return scan_u32(d_tmp, d_out, nnodes, d_out + nnodes, s);
 }

@@ -86,7 +86,7 @@ struct Context {
   // sources, smooth coefficients) run on the side stream after ev_sfork;
   // ev_side marks their end, and side_pending says the main stream has not
   // waited for it yet (join_side)
-  hipEvent_t ev_sfork = nullptr, ev_side1 = nullptr, ev_side = nullptr;
+  hipEvent_t ev_sfork = nullptr, ev_side = nullptr;
   // the core radii's largest and smallest, formed on the side stream during the tree build
   hipEvent_t ev_mm = nullptr;
   bool side_pending = false;
@@ -217,7 +217,6 @@ static int ctx_init() {
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_sfork, hipEventDisableTiming));
-  FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_side1, hipEventDisableTiming));
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_mm, hipEventDisableTiming));
   FMM_HIP_TRY(hipEventCreateWithFlags(&g.ev_side, hipEventDisableTiming));
   FMM_HIP_TRY(hipHostMalloc((void **)&g.h_total, 64, hipHostMallocDefault));
@@ -333,7 +332,7 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   // kernel scale, smooth cells) on the side stream, beside the sort and the
   // tree levels; the main stream waits for them before the launch keys
   if (g.t_mode != FMM2D_MODE_TREECODE) {
-    if (g.smax.ensure(2, s)) return -1;
+    if (g.smax.ensure(4, s)) return -1;
     FMM_HIP_TRY(hipEventRecord(g.ev_sfork, s));
     FMM_HIP_TRY(hipStreamWaitEvent(g.side, g.ev_sfork, 0));
     if (launch_minmax_reduce(dcor, n, g.smax.p, g.side)) return -1;
@@ -482,12 +481,47 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
   if (g.t_mode != FMM2D_MODE_TREECODE) {
     // target leaves, leaf-relative FP32 sources, near-field radius
     if (g.leaves.ensure(n, s) || g.lat.ensure(n, s) || g.lflag.ensure(n, s) || g.lpos.ensure(n, s) ||
-        g.loc.ensure(n, s) || g.loc64.ensure(n, s) || g.smax.ensure(2, s))
+        g.loc.ensure(n, s) || g.loc64.ensure(n, s) || g.smax.ensure(4, s))
       return -1;
     // ntl_dev: [0] target leaves, [1] sibling groups (L2L), [2] front leaves
     if (g.ntl_dev.ensure(4, s) || g.counter.ensure(1, s)) return -1;
     if (launch_leaf_list(g.nodes.p, g.nnodes, n, g.lat.p, g.lflag.p, g.lpos.p, g.leaves.p,
                          g.ntl_dev.p, s))
+      return -1;
+    if (g.nfirst.ensure(g.nnodes, s) || g.nodeli.ensure(g.nnodes, s) ||
+        g.plocal.ensure((size_t)(g.nnodes / 4 + 1) * 16, s) ||
+        g.pfront.ensure((size_t)(g.nnodes / 4 + 1) * kFrontCap, s) ||
+        g.pfcnt.ensure((size_t)(g.nnodes / 4 + 1), s))
+      return -1;
+    // smooth cells: mixed form with one core radius only (the radii are
+    // checked on the device: the kernels below skip every node otherwise);
+    // every internal node (>= num_p blobs) of the smooth level is a smooth cell
+    const bool smooth_try = g.t_mode == FMM2D_MODE_MIXED && g.smooth_eta > 0.0;
+    const int smin = std::min(g.smooth_min, g.t_num_p);
+    if (smooth_try && (g.scoef.ensure((size_t)g.nnodes * kSmoothF4, s) ||
+                       g.sflag.ensure(2 * (size_t)g.nnodes, s) || g.srange.ensure(g.nnodes, s) ||
+                       g.sunits.ensure((size_t)g.nnodes + 1, s) || g.sutmp.ensure(g.nnodes, s)))
+      return -1;
+    // What only the fused pass reads runs on the side stream from here on,
+    // beside the launch keys, the readback and the learned parameters: the
+    // leaf-relative sources, the smooth coefficients and work units (their
+    // scales come from the core radii's min/max, already formed on that
+    // stream), then the launch order (event ev_side).  The evaluation waits
+    // for it just before the fused pass; every other later call waits for it
+    // at once (join_side).  The scans of the two streams share one workspace
+    // (scan_u32): the main stream's last scan (the leaf list) is ordered
+    // before the side stream's first.
+    hipStream_t t = g.side;
+    FMM_HIP_TRY(hipEventRecord(g.ev_sfork, s));
+    FMM_HIP_TRY(hipStreamWaitEvent(t, g.ev_sfork, 0));
+    if (launch_node_first_blob(g.nodes.p, g.nnodes, g.nfirst.p, t)) return -1;
+    if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntl_dev.p, g.nleaves, g.sx.p, g.sy.p, g.sg.p,
+                          g.ss.p, g.loc.p, g.loc64.p, g.smax.p, t))
+      return -1;
+    if (smooth_try && (launch_smooth_coef(g.nodes.p, 0, g.nnodes, g.nfirst.p, g.sx.p, g.sy.p,
+                                          g.sg.p, g.smax.p, g.smooth_eta, smin, g.scoef.p, t) ||
+                       launch_smooth_units(g.nodes.p, g.nnodes, g.smax.p, g.smooth_eta, smin,
+                                           g.sutmp.p, g.sunits.p, t)))
       return -1;
     FMM_HIP_TRY(hipStreamWaitEvent(s, g.ev_mm, 0));
     // launch order keys (boxes large against the near radius first) and the
@@ -498,86 +532,41 @@ static int build(int n, const double *dxy, const double *dcir, const double *dco
         launch_leaf_launch_keys(g.nodes.p, g.leaves.p, lbound, g.ntl_dev.p, g.smax.p, g.fmm_cut,
                                 g.lkey.p, g.lord.p, g.ntl_dev.p + 2, s))
       return -1;
-    double smm[2] = {0.0, 0.0};
+    double smm[4] = {0.0, 0.0, 0.0, 0.0};
     FMM_HIP_TRY(hipMemcpyAsync(g.h_total, g.ntl_dev.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                s));
-    FMM_HIP_TRY(hipMemcpyAsync(smm, g.smax.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    FMM_HIP_TRY(hipMemcpyAsync(smm, g.smax.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
     FMM_HIP_TRY(hipStreamSynchronize(s));
     g.ntleaves = (int)g.h_total[0];
     g.nfront = (int)g.h_total[2];
     g.smax_h = smm[0];
     g.rcut = g.fmm_cut * smm[0];
-    // the FP32 pair kernel's w = log2(e) / sigma^2 is capped at 7.2e19
-    g.wmax_h = smm[1] > 0.0 ? std::min(1.4426950408889634 / (smm[1] * smm[1]), 7.2e19) : 7.2e19;
-    g.sw_h = smm[0] == smm[1] ? std::sqrt(g.wmax_h) : 0.0;
-    // smooth cells: mixed form with one core radius only
-    const bool smooth = g.t_mode == FMM2D_MODE_MIXED && g.sw_h > 0.0 && g.smooth_eta > 0.0;
+    // the FP32 pair kernel's w = log2(e) / sigma^2 (capped at 7.2e19) and, with
+    // one core radius, its square root (formed on the device)
+    g.wmax_h = smm[2];
+    g.sw_h = smm[3];
+    const bool smooth = smooth_try && g.sw_h > 0.0;
     const double region = 2.0 * g.smooth_eta * smm[0];
-    // every internal node (>= num_p blobs) of the smooth level is a smooth cell
-    const int smin = std::min(g.smooth_min, g.t_num_p);
-    // a single tree's smooth level is one contiguous range of nodes (every
-    // level halves the root's side exactly); a forest's roots differ
-    int n0 = 0, n1 = g.nnodes;
-    if (smooth && nsys == 1) {
-      double rl = g.h_root->region;
-      int l = 0;
-      while (l < g.depth && rl > region) {
-        rl *= 0.5;
-        l++;
-      }
-      n0 = l == 0 ? 0 : g.h_lt[kLtCb + l];
-      n1 = l == 0 ? nroot_slots : g.h_lt[kLtCb + l + 1];
-    }
-    if (g.nfirst.ensure(g.nnodes, s) || g.nodeli.ensure(g.nnodes, s) ||
-        g.plocal.ensure((size_t)(g.nnodes / 4 + 1) * 16, s) ||
-        g.pfront.ensure((size_t)(g.nnodes / 4 + 1) * kFrontCap, s) ||
-        g.pfcnt.ensure((size_t)(g.nnodes / 4 + 1), s))
-      return -1;
-    if (smooth && (g.scoef.ensure((size_t)g.nnodes * kSmoothF4, s) ||
-                   g.sflag.ensure(2 * (size_t)g.nnodes, s) || g.srange.ensure(g.nnodes, s) ||
-                   g.sunits.ensure((size_t)g.nnodes + 1, s) || g.sutmp.ensure(g.nnodes, s)))
-      return -1;
-    // The rest of the set-up runs on the side stream while the main one forms
-    // the learned parameters (which read only the tree, the multipoles and the
-    // per-leaf |cir| sums): first what the parent expansions read (node ->
-    // target leaf, smooth ranges; event ev_side1), then what only the fused
-    // pass reads (launch order, leaf-relative sources, smooth coefficients and
-    // work units; event ev_side).  The evaluation waits for each just before
-    // it needs it; every other later call waits for both (join_side).
-    hipStream_t t = g.side;
-    FMM_HIP_TRY(hipEventRecord(g.ev_sfork, s));
-    FMM_HIP_TRY(hipStreamWaitEvent(t, g.ev_sfork, 0));
     if (smooth) {
       g.smooth_region_h = region;
       g.t_smooth_min = smin;
     }
-    // the learned parameters first: they lead to the parent expansions, so
-    // the host enqueues them before the side stream's work
+    // the learned parameters, then what the parent expansions read (node ->
+    // target leaf, smooth runs) on the main stream, which the evaluation
+    // continues
     g.valid = true;
     if (g.learned && apply_learned()) {
       g.valid = false;
       return -1;
     }
-    // L2L: node -> target-leaf index, sibling groups, parent expansion slots
-    if (launch_l2l_setup(g.leaves.p, g.ntleaves, g.nodeli.p, g.ntl_dev.p + 1, g.nnodes / 4, t))
+    if (launch_l2l_setup(g.leaves.p, g.ntleaves, g.nodeli.p, g.ntl_dev.p + 1, g.nnodes / 4, s))
       return -1;
     if (smooth && launch_smooth_ranges(g.nodes.p, g.gparent.p, g.nnodes, nroot_slots, region, smin,
-                                       g.sflag.p, g.srange.p, t))
+                                       g.sflag.p, g.srange.p, s))
       return -1;
-    FMM_HIP_TRY(hipEventRecord(g.ev_side1, t));
-    // the second part now: enqueued after the evaluation's parent expansions,
-    // its kernels found no free wave slots until those ended
+    // the launch order: one stable 8-bit radix pass over the keys
     if (radix_sort_pairs(&g.lkey.p, &g.lord.p, &g.lkey_alt.p, &g.lord_alt.p, g.ntleaves, 0, 8,
                          g.hist.p, g.hist_scan.p, t))
-      return -1;  // the launch order: one stable 8-bit radix pass over the keys
-    if (launch_node_first_blob(g.nodes.p, g.nnodes, g.nfirst.p, t)) return -1;
-    if (launch_leaf_local(g.nodes.p, g.leaves.p, g.ntleaves, g.sx.p, g.sy.p, g.sg.p, g.ss.p,
-                          g.loc.p, g.loc64.p, g.sw_h, t))
-      return -1;
-    if (smooth && (launch_smooth_coef(g.nodes.p, n0, n1, g.nfirst.p, g.sx.p, g.sy.p, g.sg.p,
-                                      smm[0], region, smin, g.scoef.p, t) ||
-                   launch_smooth_units(g.nodes.p, g.nnodes, region, smin, g.sutmp.p, g.sunits.p,
-                                       t)))
       return -1;
     FMM_HIP_TRY(hipEventRecord(g.ev_side, t));
     g.side_pending = true;
@@ -783,7 +772,7 @@ static int eval_sources_dev(double *d_uv) {
   // the default path waits for the build's side work in two parts: before
   // the parent expansions for what they read, before the fused pass for the
   // rest
-  hipEvent_t ready1 = g.side_pending ? g.ev_side1 : nullptr;
+  hipEvent_t ready1 = nullptr;  // what the parent expansions read is on the main stream
   hipEvent_t ready = g.side_pending ? g.ev_side : nullptr;
   if (g.t_mode == FMM2D_MODE_TREECODE)
     rc = g.parity_leaves ? launch_parity_leaves(view(), g.n, g.perm.p, d_uv, g.stream)

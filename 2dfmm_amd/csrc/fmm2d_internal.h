@@ -127,11 +127,13 @@ constexpr int kSmoothQ = 12;
 constexpr int kSmoothC = (kSmoothQ + 1) + kSmoothQ * (kSmoothQ + 1) / 2;
 constexpr int kSmoothF4 = (kSmoothC + 1) / 2;  // float4 per cell
 // Coefficients of every node in [node0, node1) of region in (region/2,
-// region] with at least smin blobs (those of node k are sx..sg[nfirst[k] ...]),
-// at d_out[node * kSmoothF4]; other nodes are left as they are.
+// region], region = 2 eta sigma, with at least smin blobs (those of node k are
+// sx..sg[nfirst[k] ...]), at d_out[node * kSmoothF4]; other nodes are left as
+// they are, and so is every node without one core radius (d_smax as written
+// by launch_minmax_reduce: sigma = d_smax[0] = d_smax[1]).
 int launch_smooth_coef(const NodeRec *d_nodes, int node0, int node1, const int32_t *d_nfirst,
-                       const double *sx, const double *sy, const double *sg, double sigma,
-                       double region, int smin, float4 *d_out, hipStream_t s);
+                       const double *sx, const double *sy, const double *sg,
+                       const double *d_smax, double eta, int smin, float4 *d_out, hipStream_t s);
 // Smooth runs: for every internal node above the smooth level (region >
 // `region`) that holds a smooth cell (smooth level, >= smin blobs): the index
 // range [x, y) of its smooth-level descendants (consecutive: each level is
@@ -279,17 +281,21 @@ int launch_parent_local(const FmmArgs &args, bool mixed, int64_t *d_pcount, int 
 // Work units of a smooth cell, and of a step's fixed cost (fmm_kernels.hip).
 constexpr int kSmoothUnits = 60;
 constexpr int kStepUnits = 8;
-// Per-node work units of the smooth level summed into d_out[0..nnodes] (exclusive
-// prefix, total last); d_tmp: nnodes u32 of scratch.
-int launch_smooth_units(const NodeRec *d_nodes, int nnodes, double region, int smin,
-                        uint32_t *d_tmp, uint32_t *d_out, hipStream_t s);
+// Per-node work units of the smooth level (region = 2 eta d_smax[0]) summed
+// into d_out[0..nnodes] (exclusive prefix, total last); d_tmp: nnodes u32 of
+// scratch.
+int launch_smooth_units(const NodeRec *d_nodes, int nnodes, const double *d_smax, double eta,
+                        int smin, uint32_t *d_tmp, uint32_t *d_out, hipStream_t s);
 int launch_mfma_selftest(const double *a, const double *b, double *out, double *ref, int fp32,
                          hipStream_t s);
-// sw > 0 (all core radii equal, sw = sqrt(log2(e) / sigma^2)): FP32 sources as
-// (x', y', cir sw, 1) with coordinates relative to the leaf centre times sw.
-int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleaves,
-                      const double *sx, const double *sy, const double *sg, const double *ss,
-                      float4 *loc, double4 *loc64, double sw, hipStream_t s);
+// Leaf-relative sources of the *d_count (at most bound) target leaves.  sw =
+// d_smax[3] > 0 (all core radii equal, sw = sqrt(log2(e) / sigma^2)): FP32
+// sources as (x', y', cir sw, 1) with coordinates relative to the leaf centre
+// times sw.
+int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, const uint32_t *d_count,
+                      int bound, const double *sx, const double *sy, const double *sg,
+                      const double *ss, float4 *loc, double4 *loc64, const double *d_smax,
+                      hipStream_t s);
 // Non-empty leaves in blob (Morton) order; their number lands in *d_count.
 int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, uint32_t *d_flag,
                      uint32_t *d_pos, int32_t *d_leaves, uint32_t *d_count, hipStream_t s);
@@ -301,7 +307,9 @@ int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, u
 int launch_leaf_launch_keys(const NodeRec *d_nodes, const int32_t *d_leaves, int n_bound,
                             const uint32_t *d_count, const double *d_smax, double cut,
                             uint64_t *d_key, uint32_t *d_val, uint32_t *d_front, hipStream_t s);
-// d_out[0] = max, d_out[1] = min of n values >= 0.
+// d_out[0] = max, d_out[1] = min of n values >= 0 (core radii); d_out[2] =
+// w_max = log2(e) / min^2 (capped at 7.2e19), d_out[3] = sqrt(w_max) if max ==
+// min, else 0.  Four doubles.
 int launch_minmax_reduce(const double *d_v, int n, double *d_out, hipStream_t s);
 // First blob of every node's subtree: the first blob of its leftmost leaf.
 int launch_node_first_blob(const NodeRec *d_nodes, int nnodes, int32_t *d_out, hipStream_t s);

@@ -173,14 +173,17 @@ int launch_mfma_selftest(const double *a, const double *b, double *out, double *
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void leaf_local_kernel(const NodeRec *__restrict__ nodes,
                                                          const int32_t *__restrict__ leaves,
-                                                         int nleaves, const double *__restrict__ sx,
+                                                         const uint32_t *__restrict__ count,
+                                                         const double *__restrict__ sx,
                                                          const double *__restrict__ sy,
                                                          const double *__restrict__ sg,
                                                          const double *__restrict__ ss,
                                                          float4 *__restrict__ loc,
-                                                         double4 *__restrict__ loc64, double sw) {
+                                                         double4 *__restrict__ loc64,
+                                                         const double *__restrict__ smax) {
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (w >= nleaves) return;
+  if (w >= (int)*count) return;
+  const double sw = smax[3];
   const NodeRec r = nodes[leaves[w]];
   const int c = node_count(r.cnt);
   for (int e = lane; e < c; e += 64) {
@@ -199,12 +202,13 @@ __global__ __launch_bounds__(256) void leaf_local_kernel(const NodeRec *__restri
   }
 }
 
-int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, int nleaves,
-                      const double *sx, const double *sy, const double *sg, const double *ss,
-                      float4 *loc, double4 *loc64, double sw, hipStream_t s) {
-  if (nleaves <= 0) return 0;
-  leaf_local_kernel<<<(nleaves + 3) / 4, 256, 0, s>>>(d_nodes, d_leaves, nleaves, sx, sy, sg, ss,
-                                                      loc, loc64, sw);
+int launch_leaf_local(const NodeRec *d_nodes, const int32_t *d_leaves, const uint32_t *d_count,
+                      int bound, const double *sx, const double *sy, const double *sg,
+                      const double *ss, float4 *loc, double4 *loc64, const double *d_smax,
+                      hipStream_t s) {
+  if (bound <= 0) return 0;
+  leaf_local_kernel<<<(bound + 3) / 4, 256, 0, s>>>(d_nodes, d_leaves, d_count, sx, sy, sg, ss,
+                                                    loc, loc64, d_smax);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -248,6 +252,7 @@ int launch_leaf_list(const NodeRec *d_nodes, int nnodes, int n, int32_t *d_at, u
 // others, which keep their Morton order (neighbours in space run together).
 __global__ __launch_bounds__(256) void leaf_launch_keys_kernel(const NodeRec *__restrict__ nodes,
                                                                const int32_t *__restrict__ leaves,
+                                                               int n_bound,
                                                                const uint32_t *__restrict__ count,
                                                                const double *__restrict__ smax,
                                                                double cut,
@@ -255,7 +260,7 @@ __global__ __launch_bounds__(256) void leaf_launch_keys_kernel(const NodeRec *__
                                                                uint32_t *__restrict__ val,
                                                                uint32_t *__restrict__ front) {
   const int li = blockIdx.x * 256 + threadIdx.x;
-  if (li >= (int)*count) return;
+  if (li >= (int)*count || li >= n_bound) return;
   const double rcut = cut * smax[0];
   const double R = nodes[leaves[li]].region * 0.70710678118654752;
   int k = 9;
@@ -272,8 +277,8 @@ int launch_leaf_launch_keys(const NodeRec *d_nodes, const int32_t *d_leaves, int
                             uint64_t *d_key, uint32_t *d_val, uint32_t *d_front, hipStream_t s) {
   FMM_HIP_TRY(hipMemsetAsync(d_front, 0, sizeof(uint32_t), s));
   if (n_bound <= 0) return 0;
-  leaf_launch_keys_kernel<<<(n_bound + 255) / 256, 256, 0, s>>>(d_nodes, d_leaves, d_count, d_smax,
-                                                                cut, d_key, d_val, d_front);
+  leaf_launch_keys_kernel<<<(n_bound + 255) / 256, 256, 0, s>>>(d_nodes, d_leaves, n_bound, d_count,
+                                                                d_smax, cut, d_key, d_val, d_front);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -312,6 +317,15 @@ __global__ void minmax_init_kernel(double *out) {
   out[1] = 1.0e300;
 }
 
+// out[2] = w_max = log2(e) / sigma_min^2 (capped at kMaxW), out[3] = sqrt(w_max)
+// with one core radius, else 0: the pair kernel's scales, formed once on the
+// device so that every kernel and the host use the same values.
+__global__ void minmax_finish_kernel(double *out) {
+  const double w = out[1] > 0.0 ? fmin(1.4426950408889634 / (out[1] * out[1]), kMaxW) : kMaxW;
+  out[2] = w;
+  out[3] = out[0] == out[1] ? sqrt(w) : 0.0;
+}
+
 // Child 0 of a node starts at the node's first blob, so following child 0
 // down to a leaf gives the subtree's first blob.
 __global__ __launch_bounds__(256) void node_first_blob_kernel(const NodeRec *__restrict__ nodes,
@@ -338,6 +352,8 @@ int launch_minmax_reduce(const double *d_v, int n, double *d_out, hipStream_t s)
   if (nb > 1024) nb = 1024;
   if (nb < 1) nb = 1;
   minmax_reduce_kernel<<<nb, 256, 0, s>>>(d_v, n, d_out);
+  FMM_HIP_TRY(hipGetLastError());
+  minmax_finish_kernel<<<1, 1, 0, s>>>(d_out);
   FMM_HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -97,6 +97,8 @@ static int scan_rec(const uint32_t *d_in, uint32_t *d_out, int n, uint32_t *d_to
   return 0;
 }
 
+// One workspace serves every scan: scans on two streams must be ordered
+// (the callers order them by events).
 int scan_u32(const uint32_t *d_in, uint32_t *d_out, int n, uint32_t *d_total, hipStream_t s) {
   if (n <= 0) {
     if (d_total) FMM_HIP_TRY(hipMemsetAsync(d_total, 0, sizeof(uint32_t), s));
